@@ -1,0 +1,60 @@
+"""Builds the HIP library (libcolo.so) and the drop-in executable for gfx950.
+
+Run as a script or call build(); the outputs land next to this file so they
+travel with the source tree.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libcolo.so")
+EXE = os.path.join(HERE, "colo_miner")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("COLO_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off keeps a*a + b*b as two roundings (the reference's
+# binary32 distance); divisions stay IEEE correctly rounded; packed FP32
+# instructions are disabled.
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+SOURCES = ["colo_device.hip", "colo_miner.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + out)
+    return out
+
+
+def build(verbose=False):
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        out = _run([HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        if verbose and out.strip():
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + COMMON + ["-shared", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    _run([HIPCC, "-O2", "-std=c++17", os.path.join(CSRC, "colo_main.cpp"), "-o", EXE,
+          "-L" + HERE, "-lcolo", "-Wl,-rpath,$ORIGIN"])
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

@@ -1,0 +1,355 @@
+"""Python binding of libcolo.so (the C ABI in include/colo.h).
+
+Mirrors the reference's device entry points (Codes/DeviceProcess.h:4-36) and
+its driver (colocationFinder::Begin, Codes/colocationFinder.cpp:1068-1147) for
+tests and benchmarks.  Device buffers are torch tensors on cuda:0 (memory and
+stream plumbing only); all compute runs in the HIP kernels of libcolo.so.
+There is no CPU fallback: every call fails loudly when the library or the GPU
+is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcolo.so")
+
+COLO_SCHED_PLAIN = 0
+COLO_SCHED_PERSISTENT = 1
+COLO_SCHED_LEARNED = 2
+
+ERRORS = {0: "ok", 1: "file", 2: "parse", 3: "bounds", 4: "nomem", 5: "hip", 6: "arg", 7: "unsupported"}
+
+
+class ColoError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"colo error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Dataset(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("F", C.c_uint32), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("fcount", C.c_void_p), ("minX", C.c_float), ("minY", C.c_float), ("maxX", C.c_float),
+                ("maxY", C.c_float), ("labels", C.c_void_p)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("minX", C.c_float), ("minY", C.c_float), ("cellWidth", C.c_float), ("r2", C.c_float),
+                ("gx", C.c_uint32), ("gy", C.c_uint32), ("cells", C.c_uint32), ("n", C.c_uint32),
+                ("F", C.c_uint32)]
+
+
+class MineOpts(C.Structure):
+    _fields_ = [("theta", C.c_float), ("max_k", C.c_uint32), ("pi_mode", C.c_int32), ("filter_on", C.c_int32),
+                ("sched", C.c_int32), ("hash_tables", C.c_int32), ("out_prefix", C.c_char_p)]
+
+
+class MineStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("filter_ms", "count_ms", "pi_ms", "scan_ms", "write_ms", "host_ms",
+                                          "total_ms", "l2_join_ms")] + \
+               [(n, C.c_uint64) for n in ("l2_rows", "l2_pairs", "levels", "patterns", "candidates", "launches")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+
+SYMBOLS = [
+    "colo_ctx_create", "colo_ctx_destroy", "colo_get_grid", "colo_last_error", "colo_binning_export",
+    "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
+    "colo_joink_count", "colo_joink_write", "colo_pi", "colo_participation", "colo_mine", "colo_free_str",
+    "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
+    "colo_candidates",
+]
+
+
+def lib():
+    """Loads libcolo.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ColoError(5, f"{LIB_PATH} is missing: build it first (python gpu-colocationmininig_amd/build_lib.py)")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, u64, f32, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_int
+        L.colo_ctx_create.argtypes = [C.POINTER(Dataset), f32, vp, C.POINTER(vp)]
+        L.colo_ctx_destroy.argtypes = [vp]
+        L.colo_get_grid.argtypes = [vp, C.POINTER(Grid)]
+        L.colo_last_error.restype = C.c_char_p
+        L.colo_last_error.argtypes = [vp]
+        L.colo_binning_export.argtypes = [vp, vp, vp, vp, C.POINTER(u64), vp]
+        L.colo_filter.argtypes = [vp, vp, u32, u32, f32, vp, vp, vp]
+        L.colo_bits_words.restype = u64
+        L.colo_bits_words.argtypes = [vp, u32, vp]
+        L.colo_join2_count.argtypes = [vp, u32, u32, vp, vp, C.POINTER(u64), vp]
+        L.colo_scan_slots.argtypes = [vp, vp, u64, vp, C.POINTER(u64), vp]
+        L.colo_join2_write.argtypes = [vp, u32, u32, vp, vp, vp, vp]
+        L.colo_joink_count.argtypes = [vp, vp, u64, u32, vp, vp, vp, C.POINTER(u64), vp]
+        L.colo_joink_write.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp, vp]
+        L.colo_pi.argtypes = [vp, vp, u32, vp, vp, vp, C.POINTER(f32), C.POINTER(f32), vp]
+        L.colo_participation.argtypes = [vp, vp, u32, vp, vp, vp]
+        L.colo_mine.argtypes = [vp, C.POINTER(MineOpts), C.POINTER(vp), C.POINTER(MineStats)]
+        L.colo_free_str.argtypes = [vp]
+        L.colo_run_config.argtypes = [C.c_char_p, C.POINTER(MineOpts), C.POINTER(vp)]
+        L.colo_load_csv.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.colo_host_dataset.argtypes = [vp, C.POINTER(Dataset)]
+        L.colo_host_label.restype = C.c_char_p
+        L.colo_host_label.argtypes = [vp, u32]
+        L.colo_host_free.argtypes = [vp]
+        L.colo_candidates.restype = u64
+        L.colo_candidates.argtypes = [vp, u64, u32, vp, u64]
+        for name in SYMBOLS:
+            if name in ("colo_free_str", "colo_host_free"):
+                getattr(L, name).restype = None
+            elif name not in ("colo_last_error", "colo_bits_words", "colo_host_label", "colo_candidates"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise ColoError(5, "no GPU visible: the colocation kernels run on the device only")
+    return torch
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _u32_host(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _on_stream(fn):
+    """Runs a Context method with the context's torch stream current, so that
+    torch allocations, fills and copies are ordered with the HIP launches."""
+    import functools
+
+    @functools.wraps(fn)
+    def wrapped(self, *a, **kw):
+        with self.torch.cuda.stream(self.tstream):
+            return fn(self, *a, **kw)
+    return wrapped
+
+
+class HostData:
+    """A parsed dataset CSV (colocationFinder::populateData, cpp:139-254)."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        rc = lib().colo_load_csv(path.encode(), C.byref(h))
+        if rc:
+            raise ColoError(rc, f"cannot load {path}")
+        self.h = h
+        ds = Dataset()
+        lib().colo_host_dataset(h, C.byref(ds))
+        self.n, self.F = ds.n, ds.F
+        self.x = np.ctypeslib.as_array(C.cast(ds.x, C.POINTER(C.c_float)), (ds.n,)).copy() if ds.n else np.zeros(0, np.float32)
+        self.y = np.ctypeslib.as_array(C.cast(ds.y, C.POINTER(C.c_float)), (ds.n,)).copy() if ds.n else np.zeros(0, np.float32)
+        self.fcount = np.ctypeslib.as_array(C.cast(ds.fcount, C.POINTER(C.c_uint32)), (ds.F,)).copy()
+        self.bounds = np.array([ds.minX, ds.minY, ds.maxX, ds.maxY], np.float32)
+        self.labels = [lib().colo_host_label(h, f).decode() for f in range(ds.F)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().colo_host_free(self.h)
+            self.h = None
+
+
+class Context:
+    """Device-resident dataset + grid + binning (the reference's __constant__
+    state, Codes/Constant.h:5-22, as an explicit handle)."""
+
+    def __init__(self, x, y, fcount, bounds, r, labels=None):
+        torch = _torch()
+        self.torch = torch
+        L = lib()
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        fcount = _u32_host(fcount)
+        bounds = np.asarray(bounds, np.float32)
+        self.fcount = fcount
+        self.fstart = np.concatenate([[0], np.cumsum(fcount)[:-1]]).astype(np.uint32) if len(fcount) else fcount
+        self.n, self.F = len(x), len(fcount)
+        ds = Dataset(self.n, self.F, x.ctypes.data, y.ctypes.data, fcount.ctypes.data,
+                     float(bounds[0]), float(bounds[1]), float(bounds[2]), float(bounds[3]), None)
+        self._labels = None
+        if labels is not None:
+            arr = (C.c_char_p * self.F)(*[str(l).encode() for l in labels])
+            self._labels = arr
+            ds.labels = C.cast(arr, C.c_void_p)
+        self.tstream = torch.cuda.Stream()
+        self.stream = self.tstream.cuda_stream
+        h = C.c_void_p()
+        rc = L.colo_ctx_create(C.byref(ds), C.c_float(r), C.c_void_p(self.stream), C.byref(h))
+        self.h = h
+        if rc:
+            msg = L.colo_last_error(h).decode() if h else ""
+            L.colo_ctx_destroy(h)
+            self.h = None
+            raise ColoError(rc, msg)
+        self.r = r
+
+    @classmethod
+    def from_host(cls, hd, r):
+        return cls(hd.x, hd.y, hd.fcount, hd.bounds, r, labels=hd.labels)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().colo_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise ColoError(rc, lib().colo_last_error(self.h).decode())
+
+    def _u32(self, n):
+        return self.torch.zeros(max(int(n), 1), dtype=self.torch.int32, device="cuda")
+
+    @staticmethod
+    def _np_u32(t, n=None):
+        a = t.cpu().numpy().view(np.uint32)
+        return a if n is None else a[:n]
+
+    @_on_stream
+    def grid(self):
+        g = Grid()
+        self._check(lib().colo_get_grid(self.h, C.byref(g)))
+        return {n: getattr(g, n) for n, _ in Grid._fields_}
+
+    @_on_stream
+    def binning(self):
+        """cell of every instance (input order), instances in sorted order,
+        dense directory F x (cells+1)."""
+        dl = C.c_uint64()
+        self._check(lib().colo_binning_export(self.h, None, None, None, C.byref(dl), C.c_void_p(self.stream)))
+        cell, sidx, d = self._u32(self.n), self._u32(self.n), self._u32(dl.value)
+        self._check(lib().colo_binning_export(self.h, _dptr(cell), _dptr(sidx), _dptr(d), C.byref(dl),
+                                              C.c_void_p(self.stream)))
+        return self._np_u32(cell, self.n), self._np_u32(sidx, self.n), self._np_u32(d, dl.value)
+
+    @_on_stream
+    def filter(self, cands, k, theta):
+        cands = _u32_host(cands).reshape(-1)
+        Cn = len(cands) // k
+        torch = self.torch
+        dc = torch.from_numpy(cands.view(np.int32)).cuda() if Cn else self._u32(1)
+        counts = self._u32(Cn * k)
+        passed = torch.zeros(max(Cn, 1), dtype=torch.uint8, device="cuda")
+        self._check(lib().colo_filter(self.h, _dptr(dc), Cn, k, C.c_float(theta), _dptr(counts), _dptr(passed),
+                                      C.c_void_p(self.stream)))
+        return self._np_u32(counts, Cn * k).reshape(Cn, k), passed.cpu().numpy()[:Cn].astype(bool)
+
+    def bits_words(self, cand):
+        cand = _u32_host(cand)
+        return lib().colo_bits_words(self.h, len(cand), cand.ctypes.data)
+
+    @_on_stream
+    def join2_count(self, a, b):
+        Na = int(self.fcount[a])
+        slots = self._u32(Na)
+        bits = self._u32(self.bits_words([a, b]))
+        rows = C.c_uint64()
+        self._check(lib().colo_join2_count(self.h, a, b, _dptr(slots), _dptr(bits), C.byref(rows),
+                                           C.c_void_p(self.stream)))
+        return rows.value, slots, bits
+
+    @_on_stream
+    def scan(self, slots, n):
+        offs = self.torch.zeros(max(int(n), 1), dtype=self.torch.int64, device="cuda")
+        tot = C.c_uint64()
+        self._check(lib().colo_scan_slots(self.h, _dptr(slots), n, _dptr(offs), C.byref(tot), C.c_void_p(self.stream)))
+        return offs, tot.value
+
+    @_on_stream
+    def join2_write(self, a, b, slots):
+        Na = int(self.fcount[a])
+        offs, tot = self.scan(slots, Na)
+        table = self._u32(2 * tot)
+        self._check(lib().colo_join2_write(self.h, a, b, _dptr(slots), _dptr(offs), _dptr(table),
+                                           C.c_void_p(self.stream)))
+        return table, tot
+
+    @_on_stream
+    def joink_count(self, parent, R, k, cand):
+        cand = _u32_host(cand)
+        slots = self._u32(R)
+        bits = self._u32(self.bits_words(cand))
+        rows = C.c_uint64()
+        self._check(lib().colo_joink_count(self.h, _dptr(parent), R, k, cand.ctypes.data, _dptr(slots), _dptr(bits),
+                                           C.byref(rows), C.c_void_p(self.stream)))
+        return rows.value, slots, bits
+
+    @_on_stream
+    def joink_write(self, parent, R, k, cand, slots):
+        cand = _u32_host(cand)
+        offs, tot = self.scan(slots, R)
+        table = self._u32(k * tot)
+        self._check(lib().colo_joink_write(self.h, _dptr(parent), R, k, cand.ctypes.data, _dptr(slots), _dptr(offs),
+                                           _dptr(table), C.c_void_p(self.stream)))
+        return table, tot
+
+    @_on_stream
+    def pi(self, bits, cand):
+        cand = _u32_host(cand)
+        k = len(cand)
+        nc = np.zeros(k, np.uint64)
+        ne = np.zeros(k, np.uint64)
+        pc, pe = C.c_float(), C.c_float()
+        self._check(lib().colo_pi(self.h, _dptr(bits), k, cand.ctypes.data, nc.ctypes.data, ne.ctypes.data,
+                                  C.byref(pc), C.byref(pe), C.c_void_p(self.stream)))
+        return nc, ne, pc.value, pe.value
+
+    @_on_stream
+    def participation(self, bits, cand):
+        cand = _u32_host(cand)
+        tot = int(sum(int(self.fcount[f]) for f in cand))
+        flags = self.torch.zeros(max(tot, 1), dtype=self.torch.uint8, device="cuda")
+        self._check(lib().colo_participation(self.h, _dptr(bits), len(cand), cand.ctypes.data, _dptr(flags),
+                                             C.c_void_p(self.stream)))
+        return flags.cpu().numpy()[:tot]
+
+    @_on_stream
+    def mine(self, theta, max_k=0, pi_mode=0, filter_on=True, sched=COLO_SCHED_PERSISTENT, hash_tables=True,
+             out_prefix=None):
+        o = MineOpts(C.c_float(theta), max_k, pi_mode, int(filter_on), sched, int(hash_tables),
+                     out_prefix.encode() if out_prefix else None)
+        rep = C.c_void_p()
+        st = MineStats()
+        rc = lib().colo_mine(self.h, C.byref(o), C.byref(rep), C.byref(st))
+        text = None
+        if rep.value:
+            text = C.string_at(rep.value).decode()
+            lib().colo_free_str(rep)
+        self._check(rc)
+        return text, st.as_dict()
+
+
+def candidates(L, km1):
+    """candidateColocationGeneral (cpp:517-557) on the host: returns the
+    next-level candidates of the prevalent list L (rows of km1 ids)."""
+    L = _u32_host(L).reshape(-1)
+    P = len(L) // km1 if km1 else 0
+    n = lib().colo_candidates(L.ctypes.data, P, km1, None, 0)
+    out = np.zeros(max(n * (km1 + 1), 1), np.uint32)
+    lib().colo_candidates(L.ctypes.data, P, km1, out.ctypes.data, n)
+    return out[: n * (km1 + 1)].reshape(n, km1 + 1)
+
+
+def run_config(path, sched=COLO_SCHED_PERSISTENT, max_k=0, pi_mode=0, filter_on=True):
+    """colocationFinder::Begin(argc, argv): config.txt -> pattern files."""
+    _torch()
+    o = MineOpts(0.0, max_k, pi_mode, int(filter_on), sched, 0, None)
+    line = C.c_void_p()
+    rc = lib().colo_run_config(path.encode(), C.byref(o), C.byref(line))
+    if rc:
+        raise ColoError(rc, f"run_config({path})")
+    s = C.string_at(line.value).decode()
+    lib().colo_free_str(line)
+    return s

@@ -1,0 +1,1385 @@
+// colo_device.hip -- HIP kernels for gfx950 (MI355X) and the device half of
+// the C ABI: binning, filter, neighbour-cell join (count / write), PI.
+//
+// Reference (spatialdatasciencegroup/GPU-ColocationMininig):
+//   Codes/DeviceProcess_Kernel.h (K), Codes/DeviceProcess.cu (cu),
+//   Codes/Utility_Inline.h (U), Codes/colocationFinder.cpp (cpp).
+//
+// Numerics: all coordinate arithmetic is binary32 with round-to-nearest and
+// no contraction (this file is compiled with -ffp-contract=off), matching
+// the reference's Real = float (Codes/DataTypes.h:11):
+//   cell  = gx * trunc((y - minY) / r) + trunc((x - minX) / r)   (K:615-625)
+//   d2    = (dx*dx) + (dy*dy), neighbour iff d2 <= r2               (U:25-33)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "colo_internal.h"
+
+#pragma clang fp contract(off)
+
+using namespace colo;
+
+namespace colo {
+
+int set_error(colo_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+hipStream_t pick_stream(colo_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+
+int num_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+}  // namespace colo
+
+colo::DevData colo_ctx::dev() const {
+    DevData d;
+    d.g = DevGrid{minX, minY, r, r2, gx, gy, cells, F};
+    d.x = d_x; d.y = d_y; d.cell_of = d_cell_of; d.sidx = d_sidx; d.spos = d_spos;
+    d.sx = d_sx; d.sy = d_sy; d.scell = d_scell; d.dir = d_dir;
+    d.fstart = d_fstart; d.fcount = d_fcount;
+    return d;
+}
+
+// ------------------------------------------------------------ utilities
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+static unsigned grid_for(uint64_t n, unsigned cap = 65535u * 4) {
+    uint64_t b = (n + kBlock - 1) / kBlock;
+    if (b == 0) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+int colo::fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (!n) return COLO_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 8192)), dim3(kBlock), 0, s, p, n, v);
+    return hipGetLastError() == hipSuccess ? COLO_OK : COLO_EHIP;
+}
+
+int colo::scratch(colo_ctx *c, size_t bytes, void **p) {
+    colo_ctx *ctx_for_err = c;
+    if (bytes > c->scratch_bytes) {
+        if (c->d_scratch) COLO_HIP(hipFree(c->d_scratch));
+        c->d_scratch = nullptr;
+        size_t nb = std::max(bytes, c->scratch_bytes * 2);
+        if (hipMalloc(&c->d_scratch, nb) != hipSuccess) {
+            c->scratch_bytes = 0;
+            return set_error(c, COLO_ENOMEM, "scratch allocation failed");
+        }
+        c->scratch_bytes = nb;
+    }
+    *p = c->d_scratch;
+    return COLO_OK;
+}
+
+// ------------------------------------------------------------------ scan
+// Three-phase exclusive scan: tile sums, scan of tile sums (recursive),
+// tile scan + carry.  Tile = 256 threads x 8 items.
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ inline uint64_t wave_incl_scan(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint64_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t &excl) {
+    __shared__ uint64_t wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint64_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) {
+        if (i < w) base += wsum[i];
+        tot += wsum[i];
+    }
+    excl = base + inc - v;
+    __syncthreads();
+    return tot;
+}
+
+template <typename Tin>
+__global__ void k_scan_reduce(const Tin *in, uint64_t n, uint64_t *partial) {
+    uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++)
+        if (base + i < n) s += in[base + i];
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(s, ex);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+template <typename Tin, typename Tout>
+__global__ void k_scan_apply(const Tin *in, uint64_t n, const uint64_t *carry, Tout *out, uint64_t *total) {
+    uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = (base + i < n) ? (uint64_t)in[base + i] : 0;
+        s += v[i];
+    }
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(s, ex);
+    uint64_t run = ex + (carry ? carry[blockIdx.x] : 0);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < n) out[base + i] = (Tout)run;
+        run += v[i];
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1)
+        *total = (carry ? carry[blockIdx.x] : 0) + tot;
+}
+
+// Exclusive scan in -> out (may alias), 64-bit carries; *total (device) =
+// sum.  tmp must hold scan_tmp_words(n) uint64.
+static uint64_t scan_tmp_words(uint64_t n) {
+    uint64_t w = 0;
+    uint64_t b = (n + kScanTile - 1) / kScanTile;
+    while (b > 1) {
+        w += b + 1;
+        b = (b + kScanTile - 1) / kScanTile;
+    }
+    return w + 2;
+}
+
+template <typename Tin, typename Tout>
+static void scan_rec(const Tin *in, uint64_t n, Tout *out, uint64_t *total, uint64_t *tmp, hipStream_t s) {
+    uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb <= 1) {
+        hipLaunchKernelGGL((k_scan_apply<Tin, Tout>), dim3(1), dim3(kBlock), 0, s, in, n,
+                           (const uint64_t *)nullptr, out, total);
+        return;
+    }
+    uint64_t *partial = tmp;
+    hipLaunchKernelGGL((k_scan_reduce<Tin>), dim3((unsigned)nb), dim3(kBlock), 0, s, in, n, partial);
+    // exclusive scan of the partials in place, then the total of the last
+    // tile is carry[nb-1] + its own sum (written by the apply kernel)
+    scan_rec<uint64_t, uint64_t>(partial, nb, partial, nullptr, tmp + nb + 1, s);
+    hipLaunchKernelGGL((k_scan_apply<Tin, Tout>), dim3((unsigned)nb), dim3(kBlock), 0, s, in, n,
+                       (const uint64_t *)partial, out, total);
+}
+
+int colo::scan_u32_u64(colo_ctx *c, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *total_dev,
+                       hipStream_t s) {
+    void *tmp;
+    int rc = scratch(c, scan_tmp_words(n) * 8, &tmp);
+    if (rc) return rc;
+    if (n == 0) {
+        if (total_dev) return fill_u32((uint32_t *)total_dev, 2, 0, s);
+        return COLO_OK;
+    }
+    scan_rec<uint32_t, uint64_t>(in, n, out, total_dev, (uint64_t *)tmp, s);
+    return hipGetLastError() == hipSuccess ? COLO_OK : COLO_EHIP;
+}
+
+// ------------------------------------------------------------- binning
+
+// getCellID (K:615-625) in binary32; key = fid << cellbits | cell.
+template <typename K>
+__global__ void k_cell_keys(const float *x, const float *y, uint32_t n, const uint32_t *fstart, uint32_t F,
+                            DevGrid g, uint32_t cellbits, uint32_t *cell_of, K *keys, uint32_t *vals,
+                            uint32_t *flag) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float rx = (x[p] - g.minX) / g.r;
+    float ry = (y[p] - g.minY) / g.r;
+    uint32_t i = (uint32_t)rx;
+    uint32_t j = (uint32_t)ry;
+    uint32_t cell = g.gx * j + i;
+    if (cell >= g.cells || !(rx > -1.0f) || !(ry > -1.0f) || !(rx < 4294967296.0f) || !(ry < 4294967296.0f)) {
+        atomicOr(flag, 1u);
+        cell = 0;
+    }
+    // feature of input index p: last f with fstart[f] <= p
+    uint32_t lo = 0, hi = F;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (fstart[mid] <= p) lo = mid; else hi = mid;
+    }
+    cell_of[p] = cell;
+    keys[p] = ((K)lo << cellbits) | (K)cell;
+    vals[p] = p;
+}
+
+// LSD radix sort, 8 bits per pass, stable.  Tile = 256 threads x 16 keys.
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kBlock * kRadixItems;
+constexpr int kWaveKeys = kRadixTile / (kBlock / 64);
+
+template <typename K>
+__global__ void k_radix_hist(const K *keys, uint64_t n, int shift, uint32_t *hist, uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * kRadixTile;
+#pragma unroll 4
+    for (int i = 0; i < kRadixItems; i++) {
+        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Scatter with wave-ballot digit ranking: each wave ranks its contiguous
+// quarter of the tile in rounds of 64 keys (8 ballots give the lanes sharing
+// a digit); per-wave digit counters in LDS make ranks stable across rounds;
+// an exclusive scan over the 4 waves orders the waves.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const K *kin, const uint32_t *vin, K *kout,
+                                                           uint32_t *vout, uint64_t n, int shift,
+                                                           const uint32_t *hscan, uint32_t ntiles) {
+    __shared__ uint32_t wcnt[kBlock / 64][256];
+    __shared__ uint32_t tbase[256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = 0; i < kBlock / 64; i++) wcnt[i][threadIdx.x] = 0;
+    tbase[threadIdx.x] = hscan[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    __syncthreads();
+    constexpr int R = kWaveKeys / 64;
+    K key[R];
+    uint32_t val[R], rank[R], dig[R];
+    const uint64_t wbase = (uint64_t)blockIdx.x * kRadixTile + (uint64_t)w * kWaveKeys;
+    const uint64_t lt = __lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        uint64_t idx = wbase + (uint64_t)r * 64 + lane;
+        bool valid = idx < n;
+        key[r] = valid ? kin[idx] : (K)0;
+        val[r] = valid ? vin[idx] : 0u;
+        uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+        dig[r] = d;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t before = (uint32_t)__popcll(peers & lt);
+        uint32_t prev = valid ? wcnt[w][d] : 0u;
+        rank[r] = prev + before;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0) wcnt[w][d] = prev + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        uint32_t s = 0;
+        for (int i = 0; i < kBlock / 64; i++) {
+            uint32_t t = wcnt[i][threadIdx.x];
+            wcnt[i][threadIdx.x] = s;
+            s += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        uint64_t idx = wbase + (uint64_t)r * 64 + lane;
+        if (idx < n) {
+            uint32_t dst = tbase[dig[r]] + wcnt[w][dig[r]] + rank[r];
+            kout[dst] = key[r];
+            vout[dst] = val[r];
+        }
+    }
+}
+
+template <typename K>
+static int radix_sort(colo_ctx *c, K *keys, uint32_t *vals, K *keys2, uint32_t *vals2, uint64_t n, int bits,
+                      uint32_t *hist, uint64_t *scan_tmp, K **kres, uint32_t **vres, hipStream_t s) {
+    uint32_t ntiles = (uint32_t)((n + kRadixTile - 1) / kRadixTile);
+    K *ki = keys, *ko = keys2;
+    uint32_t *vi = vals, *vo = vals2;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL((k_radix_hist<K>), dim3(ntiles), dim3(kBlock), 0, s, ki, n, shift, hist, ntiles);
+        scan_rec<uint32_t, uint32_t>(hist, (uint64_t)ntiles * 256, hist, nullptr, scan_tmp, s);
+        hipLaunchKernelGGL((k_radix_scatter<K>), dim3(ntiles), dim3(kBlock), 0, s, ki, vi, ko, vo, n, shift,
+                           hist, ntiles);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    *kres = ki;
+    *vres = vi;
+    (void)c;
+    return hipGetLastError() == hipSuccess ? COLO_OK : COLO_EHIP;
+}
+
+// Sorted-order gathers and the inverse permutation.
+template <typename K>
+__global__ void k_gather_sorted(const K *skeys, const uint32_t *sidx, uint32_t n, uint32_t cellbits,
+                                const float *x, const float *y, float *sx, float *sy, uint32_t *spos,
+                                uint32_t *scell) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t i = sidx[p];
+    sx[p] = x[i];
+    sy[p] = y[i];
+    spos[i] = p;
+    scell[p] = (uint32_t)(skeys[p] & (((K)1 << cellbits) - 1));
+}
+
+// Per-(feature, cell) counts into the dense directory, one atomic per run of
+// equal keys inside a wavefront; also flags run starts for the filter.
+template <typename K>
+__global__ void k_dir_counts(const K *skeys, uint32_t n, uint32_t cellbits, uint32_t cells, uint32_t *dir,
+                             uint32_t *run_flag) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool valid = p < n;
+    K k = valid ? skeys[p] : (K)0;
+    K kp = (valid && p > 0) ? skeys[p - 1] : (K)~(K)0;
+    K kn = (valid && p + 1 < n) ? skeys[p + 1] : (K)~(K)0;
+    bool gstart = valid && (p == 0 || kp != k);
+    bool start = valid && (lane == 0 || kp != k);
+    bool end = valid && (lane == 63 || p + 1 == n || kn != k);
+    uint64_t sm = __ballot(start);
+    if (valid) run_flag[p] = gstart ? 1u : 0u;
+    if (end) {
+        uint64_t upto = lane == 63 ? sm : (sm & ((2ull << lane) - 1));
+        int s = 63 - __builtin_clzll(upto);
+        uint32_t f = (uint32_t)(k >> cellbits);
+        uint32_t cell = (uint32_t)(k & (((K)1 << cellbits) - 1));
+        atomicAdd(&dir[(uint64_t)f * (cells + 1) + cell], (uint32_t)(lane - s + 1));
+    }
+}
+
+__global__ void k_runs(const uint32_t *run_flag, const uint64_t *run_idx, const uint32_t *scell, uint32_t n,
+                       uint32_t *run_cell) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !run_flag[p]) return;
+    run_cell[run_idx[p]] = scell[p];
+}
+
+// Presence mask of every cell (bit f set when feature f occurs), from the runs.
+__global__ void k_cellmask(const uint32_t *run_cell, const uint64_t *run_off, uint32_t F, uint64_t H,
+                           unsigned long long *cellmask) {
+    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    uint32_t lo = 0, hi = F;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (run_off[mid] <= h) lo = mid; else hi = mid;
+    }
+    atomicOr(&cellmask[run_cell[h]], 1ull << lo);
+}
+
+// Quadrant masks: OR of the 4 cells of quadrant (qx, qy), qx < gx-1, qy < gy-1.
+__global__ void k_qmask(const uint64_t *cellmask, uint32_t gx, uint32_t gy, uint64_t *qmask) {
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t nq = (uint64_t)(gx - 1) * (gy - 1);
+    if (q >= nq) return;
+    uint32_t qy = (uint32_t)(q / (gx - 1)), qx = (uint32_t)(q - (uint64_t)qy * (gx - 1));
+    uint64_t c0 = (uint64_t)qy * gx + qx;
+    qmask[q] = cellmask[c0] | cellmask[c0 + 1] | cellmask[c0 + gx] | cellmask[c0 + gx + 1];
+}
+
+__global__ void k_u64_at(const uint64_t *a, const uint32_t *idx, uint32_t m, uint64_t *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = a[idx[i]];
+}
+
+// ------------------------------------------------------------ filter
+// A.6 / filter2_Kernel (K:469-509) + threshold_Kernel (K:545-573) as a
+// segmented reduction: for candidate P and member t, sum the counts of the
+// non-empty cells c of feature p_t for which one of the <= 4 quadrants
+// containing c holds every feature of P (qmask superset of mask(P)).
+
+struct FilterItem {
+    uint32_t cand, t;
+    uint64_t run_begin, run_end;
+};
+
+constexpr int kFilterRuns = kBlock * 8;
+
+__global__ __launch_bounds__(kBlock) void k_filter(const FilterItem *items, const uint64_t *chunk_prefix,
+                                                   uint64_t n_items, const uint32_t *cands, uint32_t k,
+                                                   const uint32_t *run_cell, const uint32_t *dir,
+                                                   const uint64_t *qmask, DevGrid g, uint32_t *counts) {
+    // find the item of this block
+    uint64_t b = blockIdx.x;
+    uint64_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (chunk_prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    const FilterItem it = items[lo];
+    uint64_t r0 = it.run_begin + (b - chunk_prefix[lo]) * kFilterRuns;
+    uint64_t r1 = r0 + kFilterRuns < it.run_end ? r0 + kFilterRuns : it.run_end;
+    const uint32_t *cand = cands + (uint64_t)it.cand * k;
+    uint64_t mask = 0;
+    for (uint32_t t = 0; t < k; t++) mask |= 1ull << cand[t];
+    const uint32_t f = cand[it.t];
+    const uint32_t *dirf = dir + (uint64_t)f * (g.cells + 1);
+    uint64_t sum = 0;
+    if (g.gx >= 2 && g.gy >= 2) {
+        for (uint64_t h = r0 + threadIdx.x; h < r1; h += kBlock) {
+            uint32_t c = run_cell[h];
+            uint32_t cy = c / g.gx, cx = c - cy * g.gx;
+            bool ok = false;
+#pragma unroll
+            for (int dq = 0; dq < 4; dq++) {
+                uint32_t dqy = dq >> 1, dqx = dq & 1;
+                if (cy < dqy || cx < dqx) continue;
+                uint32_t qy = cy - dqy, qx = cx - dqx;
+                if (qy + 1 >= g.gy || qx + 1 >= g.gx) continue;
+                uint64_t qm = qmask[(uint64_t)qy * (g.gx - 1) + qx];
+                ok |= (qm & mask) == mask;
+            }
+            if (ok) sum += dirf[c + 1] - dirf[c];
+        }
+    }
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(sum, ex);
+    if (threadIdx.x == 0 && tot) atomicAdd(&counts[(uint64_t)it.cand * k + it.t], (uint32_t)tot);
+}
+
+// threshold_Kernel (K:545-573): ub_t = fl32(double(cnt)/double(fl32(N))),
+// pass iff min_t ub_t >= theta.
+__global__ void k_threshold(const uint32_t *counts, const uint32_t *cands, uint32_t C, uint32_t k,
+                            const uint32_t *fcount, float theta, uint8_t *pass) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    float PI = 1.0f;
+    for (uint32_t t = 0; t < k; t++) {
+        double q = (double)counts[(uint64_t)i * k + t] / (double)(float)fcount[cands[(uint64_t)i * k + t]];
+        float ub = (float)q;
+        if (t == 0 || PI > ub) PI = ub;
+    }
+    pass[i] = PI >= theta ? 1 : 0;
+}
+
+int colo::filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
+                       uint32_t *d_counts, uint8_t *d_pass, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (C == 0) return COLO_OK;
+    if (c->F > 64) return set_error(c, COLO_EUNSUPPORTED, "filter supports at most 64 features");
+    std::vector<uint32_t> hc((uint64_t)C * k);
+    COLO_HIP(hipMemcpyAsync(hc.data(), d_cands, hc.size() * 4, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    std::vector<FilterItem> items;
+    std::vector<uint64_t> prefix;
+    uint64_t chunks = 0;
+    items.reserve((uint64_t)C * k);
+    for (uint32_t i = 0; i < C; i++)
+        for (uint32_t t = 0; t < k; t++) {
+            uint32_t f = hc[(uint64_t)i * k + t];
+            if (f >= c->F) return set_error(c, COLO_EARG, "candidate feature out of range");
+            uint64_t rb = c->run_off[f], re = c->run_off[f + 1];
+            if (re == rb) continue;
+            items.push_back(FilterItem{i, t, rb, re});
+            prefix.push_back(chunks);
+            chunks += (re - rb + kFilterRuns - 1) / kFilterRuns;
+        }
+    int rc = fill_u32(d_counts, (uint64_t)C * k, 0, s);
+    if (rc) return rc;
+    if (!items.empty()) {
+        size_t bytes = items.size() * sizeof(FilterItem) + prefix.size() * 8;
+        void *tmp;
+        rc = scratch(c, bytes, &tmp);
+        if (rc) return rc;
+        FilterItem *d_items = (FilterItem *)tmp;
+        uint64_t *d_prefix = (uint64_t *)((char *)tmp + items.size() * sizeof(FilterItem));
+        COLO_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(FilterItem), hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice, s));
+        DevGrid g = c->dev().g;
+        hipLaunchKernelGGL(k_filter, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_items, d_prefix,
+                           (uint64_t)items.size(), d_cands, k, c->d_run_cell, c->d_dir, c->d_qmask, g, d_counts);
+        COLO_HIP(hipGetLastError());
+        // the item arrays live in scratch: finish before the next scratch use
+        COLO_HIP(hipStreamSynchronize(s));
+    }
+    hipLaunchKernelGGL(k_threshold, dim3(grid_for(C)), dim3(kBlock), 0, s, d_counts, d_cands, C, k,
+                       c->d_fcount, theta, d_pass);
+    COLO_HIP(hipGetLastError());
+    return COLO_OK;
+}
+
+// ------------------------------------------------------------------ join
+// Neighbour cells of anchor cell (ax, ay) per getNeighborCellID (U:35-53)
+// and the range check of K:28-29: rows ay-1..ay+1 clipped to [0, gy);
+// columns ax-1..ax+1 clipped to [0, gx) when gx >= 3; only column ax when
+// gx == 2 (the reference drops the other column as a wrap); none when
+// gx == 1.  Scan order: rows outer, columns inner, then sorted position,
+// which in the feature-major order is (cell, input index) -- the order of
+// the reference's 3x3 loop (K:24-38).  For a fixed row the columns are
+// adjacent cells, so one row is one contiguous range of sorted positions.
+
+struct RowSpan {
+    uint32_t lo[3], hi[3];
+};
+
+__device__ inline int row_spans(const DevGrid &g, const uint32_t *dirf, uint32_t cell, RowSpan &rs) {
+    if (g.gx == 1) return 0;
+    uint32_t ay = cell / g.gx, ax = cell - ay * g.gx;
+    uint32_t cl, ch;
+    if (g.gx == 2) {
+        cl = ch = ax;
+    } else {
+        cl = ax ? ax - 1 : 0;
+        ch = ax + 1 < g.gx ? ax + 1 : g.gx - 1;
+    }
+    uint32_t r0 = ay ? ay - 1 : 0;
+    uint32_t r1 = ay + 1 < g.gy ? ay + 1 : g.gy - 1;
+    int m = 0;
+    for (uint32_t ry = r0; ry <= r1; ry++) {
+        uint64_t base = (uint64_t)ry * g.gx;
+        rs.lo[m] = dirf[base + cl];
+        rs.hi[m] = dirf[base + ch + 1];
+        m++;
+    }
+    return m;
+}
+
+__device__ inline float sqdist(float x1, float y1, float x2, float y2) {
+    float dx = x2 - x1;
+    float dy = y2 - y1;
+    float sx = dx * dx;
+    float sy = dy * dy;
+    return sx + sy;
+}
+
+__device__ inline void set_bit(uint32_t *bits, uint64_t i) {
+    uint32_t m = 1u << (i & 31);
+    uint32_t *w = bits + (i >> 5);
+    if (!(__atomic_load_n(w, __ATOMIC_RELAXED) & m)) atomicOr(w, m);
+}
+
+// Level-2 count for one anchor at sorted position p of feature a
+// (degree2TableSlotCounter_Kernel, K:12-55).
+__device__ inline void join2_count_anchor(const DevData &D, uint32_t a, uint32_t b, uint32_t p, uint32_t *slots,
+                                          uint32_t *bits_a, uint32_t *bits_b) {
+    const uint32_t sa = D.fstart[a], sb = D.fstart[b];
+    const uint32_t g = D.sidx[p];
+    const uint32_t t = g - sa;
+    const float xa = D.sx[p], ya = D.sy[p];
+    const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
+    RowSpan rs;
+    int m = row_spans(D.g, dirb, D.scell[p], rs);
+    uint32_t cnt = 0;
+    for (int r = 0; r < m; r++) {
+        for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
+            if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
+                cnt++;
+                set_bit(bits_b, q - sb);
+            }
+        }
+    }
+    slots[t] = cnt;
+    if (cnt) set_bit(bits_a, t);
+}
+
+// Level-k count for one parent row (generalInstanceTableSlotCounter_Kernel,
+// K:255-316): members row[0..km1-1], anchor row[0], last feature fl.
+__device__ inline void joink_count_row(const DevData &D, const uint32_t *row, uint32_t km1, const uint32_t *feat,
+                                       uint32_t fl, uint32_t *slot, uint32_t *bits, const uint64_t *seg_off) {
+    float mx[kMaxK], my[kMaxK];
+    for (uint32_t m = 0; m < km1; m++) {
+        mx[m] = D.x[row[m]];
+        my[m] = D.y[row[m]];
+    }
+    const uint32_t sl = D.fstart[fl];
+    const uint32_t *dirl = D.dir + (uint64_t)fl * (D.g.cells + 1);
+    RowSpan rs;
+    int nr = row_spans(D.g, dirl, D.cell_of[row[0]], rs);
+    uint32_t cnt = 0;
+    for (int r = 0; r < nr; r++) {
+        for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
+            const float qx = D.sx[q], qy = D.sy[q];
+            bool ok = true;
+            for (uint32_t m = 0; m < km1; m++)
+                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+            if (ok) {
+                cnt++;
+                set_bit(bits + seg_off[km1], q - sl);
+            }
+        }
+    }
+    *slot = cnt;
+    if (cnt)
+        for (uint32_t m = 0; m < km1; m++) set_bit(bits + seg_off[m], row[m] - D.fstart[feat[m]]);
+}
+
+// Level-2 write for the anchor of input rank t (degree2TableGenerator_Kernel,
+// K:56-100): rows (g, q).
+__device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t b, uint64_t t, uint32_t nrows,
+                                          uint64_t off, uint32_t *table) {
+    const uint32_t g = D.fstart[a] + (uint32_t)t;
+    const float xa = D.x[g], ya = D.y[g];
+    const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
+    RowSpan rs;
+    int m = row_spans(D.g, dirb, D.cell_of[g], rs);
+    uint32_t *out = table + off * 2;
+    uint32_t w = 0;
+    for (int r = 0; r < m && w < nrows; r++) {
+        for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
+            if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
+                out[2 * w] = g;
+                out[2 * w + 1] = D.sidx[q];
+                w++;
+            }
+        }
+    }
+}
+
+// Level-k write for one parent row (generateInstanceTableGeneral_Kernel,
+// K:318-385): rows row ++ [q].
+__device__ inline void joink_write_row(const DevData &D, const uint32_t *row, uint32_t km1, uint32_t fl,
+                                       uint32_t nrows, uint64_t off, uint32_t *table) {
+    float mx[kMaxK], my[kMaxK];
+    for (uint32_t m = 0; m < km1; m++) {
+        mx[m] = D.x[row[m]];
+        my[m] = D.y[row[m]];
+    }
+    const uint32_t *dirl = D.dir + (uint64_t)fl * (D.g.cells + 1);
+    RowSpan rs;
+    int nr = row_spans(D.g, dirl, D.cell_of[row[0]], rs);
+    const uint32_t k = km1 + 1;
+    uint32_t *out = table + off * k;
+    uint32_t w = 0;
+    for (int r = 0; r < nr && w < nrows; r++) {
+        for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
+            const float qx = D.sx[q], qy = D.sy[q];
+            bool ok = true;
+            for (uint32_t m = 0; m < km1; m++)
+                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+            if (ok) {
+                for (uint32_t m = 0; m < km1; m++) out[(uint64_t)w * k + m] = row[m];
+                out[(uint64_t)w * k + km1] = D.sidx[q];
+                w++;
+            }
+        }
+    }
+}
+
+// ---- per-candidate kernels (the plain schedule and the C ABI)
+
+__global__ void k_join2_count(DevData D, uint32_t a, uint32_t b, uint32_t *slots, uint32_t *bits_a,
+                              uint32_t *bits_b) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t Na = D.fcount[a];
+    if (i >= Na) return;
+    join2_count_anchor(D, a, b, D.fstart[a] + i, slots, bits_a, bits_b);
+}
+
+__global__ void k_join2_write(DevData D, uint32_t a, uint32_t b, const uint32_t *slots, const uint64_t *offs,
+                              uint32_t *table) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.fcount[a]) return;
+    uint32_t nr = slots[t];
+    if (!nr) return;
+    join2_write_anchor(D, a, b, t, nr, offs[t], table);
+}
+
+struct CandK {
+    uint32_t km1, fl;
+    uint32_t feat[kMaxK];
+    uint64_t seg_off[kMaxK];
+};
+
+__global__ void k_joink_count(DevData D, const uint32_t *parent, uint64_t R, CandK ck, uint32_t *slots,
+                              uint32_t *bits) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    joink_count_row(D, parent + i * ck.km1, ck.km1, ck.feat, ck.fl, slots + i, bits, ck.seg_off);
+}
+
+__global__ void k_joink_write(DevData D, const uint32_t *parent, uint64_t R, CandK ck, const uint32_t *slots,
+                              const uint64_t *offs, uint32_t *table) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    uint32_t nr = slots[i];
+    if (!nr) return;
+    joink_write_row(D, parent + i * ck.km1, ck.km1, ck.fl, nr, offs[i], table);
+}
+
+// ---- batched (persistent) kernels: a fixed grid drains a work queue of
+// (candidate, anchor range) items in queue order.
+
+constexpr int kItemAnchors = kBlock * 4;
+
+__device__ inline bool next_item(uint32_t *queue, uint64_t n_items, uint64_t &item) {
+    __shared__ uint64_t s_item;
+    if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long *)queue, 1ull);
+    __syncthreads();
+    item = s_item;
+    __syncthreads();
+    return item < n_items;
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_batch(DevData D, const CandDesc *desc, const WorkItem *items,
+                                                        uint64_t n_items, uint32_t *slots, uint32_t *bits,
+                                                        uint32_t *queue) {
+    uint64_t it;
+    while (next_item(queue, n_items, it)) {
+        const WorkItem wi = items[it];
+        const CandDesc &cd = desc[wi.cand];
+        uint32_t *sl = slots + cd.slots_off;
+        uint32_t *bt = bits + cd.bits_off;
+        if (cd.k == 2) {
+            const uint32_t a = cd.f[0], b = cd.f[1];
+            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+                join2_count_anchor(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+        } else {
+            const uint32_t km1 = cd.k - 1;
+            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+                joink_count_row(D, cd.parent + i * km1, km1, cd.f, cd.f[km1], sl + i, bt, cd.seg_off);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_write_batch(DevData D, const CandDesc *desc, const WorkItem *items,
+                                                        uint64_t n_items, const uint32_t *slots, uint32_t *queue) {
+    uint64_t it;
+    while (next_item(queue, n_items, it)) {
+        const WorkItem wi = items[it];
+        const CandDesc &cd = desc[wi.cand];
+        const uint32_t *sl = slots + cd.slots_off;
+        if (cd.k == 2) {
+            for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
+                uint32_t nr = sl[t];
+                if (nr) join2_write_anchor(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table);
+            }
+        } else {
+            const uint32_t km1 = cd.k - 1;
+            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock) {
+                uint32_t nr = sl[i];
+                if (nr) joink_write_row(D, cd.parent + i * km1, km1, cd.f[km1], nr, cd.offsets[i] - cd.off_base, cd.table);
+            }
+        }
+    }
+}
+
+static unsigned persistent_grid(uint64_t n_items) {
+    uint64_t g = (uint64_t)num_cus() * 8;
+    if (g > n_items) g = n_items;
+    return (unsigned)(g ? g : 1);
+}
+
+int colo::count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items, uint32_t k,
+                      uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s) {
+    (void)k;
+    if (!n_items) return COLO_OK;
+    int rc = fill_u32(d_queue, 2, 0, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_count_batch, dim3(persistent_grid(n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
+                       d_items, n_items, d_slots, d_bits, d_queue);
+    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "count_batch launch");
+}
+
+int colo::write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items, uint32_t k,
+                      const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s) {
+    (void)k;
+    if (!n_items) return COLO_OK;
+    int rc = fill_u32(d_queue, 2, 0, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_write_batch, dim3(persistent_grid(n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
+                       d_items, n_items, d_slots, d_queue);
+    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "write_batch launch");
+}
+
+// -------------------------------------------------------------------- PI
+// One workgroup per bitmap segment: popcount reduction plus the two
+// boundary bits the reference's off-by-one windows need (cu:113-124).
+
+__global__ __launch_bounds__(kBlock) void k_pi_abs(const uint32_t *bits, const SegDesc *segs, SegOut *out) {
+    const SegDesc sd = segs[blockIdx.x];
+    const uint32_t *w = bits + sd.off;
+    uint64_t s = 0;
+    for (uint64_t i = threadIdx.x; i < sd.words; i += kBlock) s += __popc(w[i]);
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(s, ex);
+    if (threadIdx.x == 0) {
+        SegOut o;
+        o.pop = tot;
+        o.first = (w[sd.first_bit >> 5] >> (sd.first_bit & 31)) & 1u;
+        o.last = (w[sd.last_bit >> 5] >> (sd.last_bit & 31)) & 1u;
+        out[blockIdx.x] = o;
+    }
+}
+
+void colo::seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, uint64_t *seg_off,
+                      uint64_t *words_total) {
+    uint64_t o = bits_off;
+    for (uint32_t j = 0; j < k; j++) {
+        seg_off[j] = o;
+        o += (c->fcount[cand[j]] + 31) / 32;
+    }
+    if (words_total) *words_total = o - bits_off;
+}
+
+void colo::seg_descs(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, SegDesc *out) {
+    uint64_t so[kMaxK];
+    seg_layout(c, k, cand, bits_off, so, nullptr);
+    for (uint32_t j = 0; j < k; j++) {
+        uint32_t f = cand[j], N = c->fcount[f];
+        out[j].off = so[j];
+        out[j].words = (N + 31) / 32;
+        if (j + 1 < k) {  // input-rank indexed
+            out[j].first_bit = 0;
+            out[j].last_bit = N - 1;
+        } else {          // last feature: sorted-rank indexed
+            out[j].first_bit = c->first_sorted[f];
+            out[j].last_bit = c->last_sorted[f];
+        }
+    }
+}
+
+// Numerators and PI (cu:108-132).  Reference windows over its 0/1 words:
+// j = 0: [0, N0-1); j >= 1: [to-1, to-1+N_j) with to advanced per feature.
+// With the reference's layouts (degree 2: b's slot 0 shares a word with a's
+// slot N_a-1, K:45-47; degree k: features back to back, K:298-310) these
+// windows sum to
+//   num_0 = pop(P_0) - P_0[N_0-1]
+//   k == 2: num_1 = (P_1[0] | P_0[N_0-1]) + pop(P_1) - P_1[0]
+//   k >= 3: num_j = P_{j-1}[N_{j-1}-1] + pop(P_j) - P_j[N_j-1]
+// pr_j = fl32(num_j) / fl32(N_j), PI = min(1, min_j pr_j).
+void colo::pi_from_segs(const colo_ctx *c, uint32_t k, const uint32_t *cand, const SegOut *so, uint64_t *num_compat,
+                        uint64_t *num_exact, float *pi_compat, float *pi_exact) {
+    float pc = 1.0f, pe = 1.0f;
+    for (uint32_t j = 0; j < k; j++) {
+        uint64_t ex = so[j].pop, cm;
+        if (j == 0) cm = so[0].pop - so[0].last;
+        else if (k == 2) cm = (uint64_t)(so[1].first | so[0].last) + so[1].pop - so[1].first;
+        else cm = (uint64_t)so[j - 1].last + so[j].pop - so[j].last;
+        float N = (float)c->fcount[cand[j]];
+        volatile float prc = (float)cm / N;
+        volatile float pre = (float)ex / N;
+        if (prc < pc) pc = prc;
+        if (pre < pe) pe = pre;
+        if (num_compat) num_compat[j] = cm;
+        if (num_exact) num_exact[j] = ex;
+    }
+    if (pi_compat) *pi_compat = pc;
+    if (pi_exact) *pi_exact = pe;
+}
+
+// participation expansion (parity hook)
+__global__ void k_participation(const uint32_t *bits, SegDesc sd, uint32_t N, uint32_t f, bool sorted_rank,
+                                const uint32_t *spos, const uint32_t *fstart, uint8_t *flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint64_t b = sorted_rank ? (uint64_t)(spos[fstart[f] + i] - fstart[f]) : i;
+    flags[i] = (bits[sd.off + (b >> 5)] >> (b & 31)) & 1u;
+}
+
+// ------------------------------------------------------------ C ABI
+
+static int alloc_dev(colo_ctx *c, void **p, size_t bytes) {
+    if (hipMalloc(p, bytes ? bytes : 4) != hipSuccess) {
+        *p = nullptr;
+        return set_error(c, COLO_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return COLO_OK;
+}
+
+static uint32_t bits_for(uint64_t v) {
+    uint32_t b = 0;
+    while (b < 64 && (v >> b)) b++;
+    return b;
+}
+
+template <typename K>
+static int bin_impl(colo_ctx *c, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    const uint32_t n = c->n;
+    K *keys = nullptr, *keys2 = nullptr;
+    uint32_t *vals2 = nullptr;
+    int rc;
+    if ((rc = alloc_dev(c, (void **)&keys, sizeof(K) * (uint64_t)n))) return rc;
+    if ((rc = alloc_dev(c, (void **)&keys2, sizeof(K) * (uint64_t)n))) return rc;
+    if ((rc = alloc_dev(c, (void **)&vals2, 4ull * n))) return rc;
+    uint32_t ntiles = (uint32_t)((n + kRadixTile - 1) / kRadixTile);
+    uint32_t *hist = nullptr;
+    if ((rc = alloc_dev(c, (void **)&hist, 4ull * 256 * (ntiles ? ntiles : 1)))) return rc;
+    uint64_t tmpw = std::max(scan_tmp_words((uint64_t)ntiles * 256), scan_tmp_words(std::max<uint64_t>(c->dir_len, n)));
+    uint64_t *stmp = nullptr;
+    if ((rc = alloc_dev(c, (void **)&stmp, 8 * tmpw))) return rc;
+    DevGrid g = c->dev().g;
+    if (n) {
+        hipLaunchKernelGGL((k_cell_keys<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->d_x, c->d_y, n,
+                           c->d_fstart, c->F, g, c->cellbits, c->d_cell_of, keys, c->d_sidx, c->d_flag);
+        COLO_HIP(hipGetLastError());
+    }
+    uint32_t hflag = 0;
+    COLO_HIP(hipMemcpyAsync(&hflag, c->d_flag, 4, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    if (hflag) return set_error(c, COLO_EBOUNDS, "a point lies outside the grid (cell id >= gx*gy)");
+    K *sk = keys;
+    uint32_t *sv = c->d_sidx;
+    if (n) {
+        int bits = (int)(c->cellbits + c->fidbits);
+        if ((rc = radix_sort<K>(c, keys, c->d_sidx, keys2, vals2, n, bits, hist, stmp, &sk, &sv, s))) return rc;
+        if (sv != c->d_sidx) COLO_HIP(hipMemcpyAsync(c->d_sidx, sv, 4ull * n, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL((k_gather_sorted<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk,
+                           c->d_sidx, n, c->cellbits, c->d_x, c->d_y, c->d_sx, c->d_sy, c->d_spos, c->d_scell);
+    }
+    // directory: counts per (f, cell) then exclusive scan in place
+    if ((rc = fill_u32(c->d_dir, c->dir_len, 0, s))) return rc;
+    uint32_t *run_flag = vals2;  // reuse
+    if (n) {
+        hipLaunchKernelGGL((k_dir_counts<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk, n,
+                           c->cellbits, c->cells, c->d_dir, run_flag);
+    }
+    scan_rec<uint32_t, uint32_t>(c->d_dir, c->dir_len, c->d_dir, nullptr, stmp, s);
+    COLO_HIP(hipGetLastError());
+    // runs (distinct (feature, cell) pairs) for the filter
+    uint64_t *run_idx = (uint64_t *)keys2;  // reuse: needs 8n bytes
+    uint64_t *run_idx_buf = nullptr;
+    if (sizeof(K) < 8) {
+        if ((rc = alloc_dev(c, (void **)&run_idx_buf, 8ull * n + 8))) return rc;
+        run_idx = run_idx_buf;
+    }
+    uint64_t *d_H = stmp + tmpw - 1;
+    if (n) scan_rec<uint32_t, uint64_t>(run_flag, n, run_idx, d_H, stmp, s);
+    uint64_t H = 0;
+    if (n) {
+        COLO_HIP(hipMemcpyAsync(&H, d_H, 8, hipMemcpyDeviceToHost, s));
+        COLO_HIP(hipStreamSynchronize(s));
+    }
+    c->H = H;
+    if ((rc = alloc_dev(c, (void **)&c->d_run_cell, 4 * (H ? H : 1)))) return rc;
+    if (n) hipLaunchKernelGGL(k_runs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, run_flag, run_idx,
+                              c->d_scell, n, c->d_run_cell);
+    // run offsets per feature: run index at sorted position fstart[f]
+    c->run_off.assign(c->F + 1, H);
+    if (c->F && n) {
+        uint64_t *d_ro = stmp;
+        hipLaunchKernelGGL(k_u64_at, dim3((c->F + kBlock - 1) / kBlock), dim3(kBlock), 0, s, run_idx, c->d_fstart,
+                           c->F, d_ro);
+        COLO_HIP(hipMemcpyAsync(c->run_off.data(), d_ro, 8ull * c->F, hipMemcpyDeviceToHost, s));
+        COLO_HIP(hipStreamSynchronize(s));
+        c->run_off[c->F] = H;
+    }
+    // cell and quadrant masks (F <= 64)
+    if (c->F <= 64) {
+        if ((rc = alloc_dev(c, (void **)&c->d_cellmask, 8ull * c->cells))) return rc;
+        uint64_t nq = (c->gx >= 2 && c->gy >= 2) ? (uint64_t)(c->gx - 1) * (c->gy - 1) : 0;
+        if ((rc = alloc_dev(c, (void **)&c->d_qmask, 8 * (nq ? nq : 1)))) return rc;
+        if ((rc = fill_u32((uint32_t *)c->d_cellmask, 2ull * c->cells, 0, s))) return rc;
+        uint64_t *d_roff = stmp;
+        COLO_HIP(hipMemcpyAsync(d_roff, c->run_off.data(), 8ull * (c->F + 1), hipMemcpyHostToDevice, s));
+        if (H) hipLaunchKernelGGL(k_cellmask, dim3(grid_for(H, 1u << 30)), dim3(kBlock), 0, s, c->d_run_cell,
+                                  d_roff, c->F, H, (unsigned long long *)c->d_cellmask);
+        if (nq) hipLaunchKernelGGL(k_qmask, dim3(grid_for(nq, 1u << 30)), dim3(kBlock), 0, s, c->d_cellmask, c->gx,
+                                   c->gy, c->d_qmask);
+        COLO_HIP(hipGetLastError());
+    }
+    // sorted ranks of each feature's first and last input instance
+    c->first_sorted.assign(c->F, 0);
+    c->last_sorted.assign(c->F, 0);
+    if (c->F && n) {
+        std::vector<uint32_t> spos(n);
+        COLO_HIP(hipMemcpyAsync(spos.data(), c->d_spos, 4ull * n, hipMemcpyDeviceToHost, s));
+        COLO_HIP(hipStreamSynchronize(s));
+        for (uint32_t f = 0; f < c->F; f++) {
+            c->first_sorted[f] = spos[c->fstart[f]] - c->fstart[f];
+            c->last_sorted[f] = spos[c->fstart[f] + c->fcount[f] - 1] - c->fstart[f];
+        }
+    }
+    COLO_HIP(hipStreamSynchronize(s));
+    (void)hipFree(keys);
+    (void)hipFree(keys2);
+    (void)hipFree(vals2);
+    (void)hipFree(hist);
+    (void)hipFree(stmp);
+    if (run_idx_buf) (void)hipFree(run_idx_buf);
+    return COLO_OK;
+}
+
+extern "C" {
+
+const char *colo_last_error(const colo_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **out) {
+    if (!ds || !out) return COLO_EARG;
+    *out = nullptr;
+    colo_ctx *c = new colo_ctx();
+    colo_ctx *ctx_for_err = c;
+    int rc = COLO_OK;
+    c->n = ds->n;
+    c->F = ds->F;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return COLO_EHIP;
+        }
+        c->own_stream = true;
+    }
+    hipStream_t s = c->stream;
+    c->fstart.resize(c->F);
+    c->fcount.resize(c->F);
+    uint64_t tot = 0;
+    for (uint32_t f = 0; f < c->F; f++) {
+        c->fstart[f] = (uint32_t)tot;
+        c->fcount[f] = ds->fcount[f];
+        if (!ds->fcount[f]) rc = set_error(c, COLO_EARG, "feature with no instances");
+        tot += ds->fcount[f];
+        c->labels.push_back(ds->labels ? std::string(ds->labels[f]) : std::to_string(f));
+    }
+    if (!rc && tot != c->n) rc = set_error(c, COLO_EARG, "sum of fcount != n");
+    c->minX = ds->minX; c->minY = ds->minY; c->maxX = ds->maxX; c->maxY = ds->maxY;
+    // setGrid (cpp:102-115), binary32 throughout
+    c->r = r;
+    {
+        volatile float r2 = r * r;
+        c->r2 = r2;
+        volatile float maxBy = ds->maxY + r;
+        volatile float gsx = (ds->maxX - ds->minX) / r;
+        volatile float gsy = (maxBy - ds->minY) / r;
+        float cx = ceilf(gsx), cy = ceilf(gsy);
+        if (!(r > 0.f) || !(cx >= 1.f) || !(cy >= 1.f) || cx > 4.0e9f || cy > 4.0e9f)
+            rc = rc ? rc : set_error(c, COLO_EARG, "degenerate grid");
+        else {
+            c->gx = (uint32_t)cx;
+            c->gy = (uint32_t)cy;
+            uint64_t cells = (uint64_t)c->gx * c->gy;
+            if (cells >= (1ull << 31)) rc = rc ? rc : set_error(c, COLO_EUNSUPPORTED, "more than 2^31 cells");
+            c->cells = (uint32_t)cells;
+        }
+    }
+    if (!rc) {
+        c->cellbits = std::max<uint32_t>(1, bits_for(c->cells));
+        c->fidbits = bits_for(c->F ? c->F - 1 : 0);
+        c->dir_len = (uint64_t)c->F * (c->cells + 1ull);
+        const uint64_t n = c->n;
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_x, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_y, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_sx, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_sy, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_cell_of, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_sidx, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_spos, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_scell, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_dir, 4 * c->dir_len);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_fstart, 4ull * c->F);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_fcount, 4ull * c->F);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_flag, 16);
+        if (!rc && hipHostMalloc((void **)&c->h_pinned, 4096, hipHostMallocDefault) != hipSuccess)
+            rc = set_error(c, COLO_ENOMEM, "pinned staging");
+        if (!rc) {
+            hipError_t e = hipSuccess;
+            if (n) {
+                e = hipMemcpyAsync(c->d_x, ds->x, 4 * n, hipMemcpyHostToDevice, s);
+                if (e == hipSuccess) e = hipMemcpyAsync(c->d_y, ds->y, 4 * n, hipMemcpyHostToDevice, s);
+            }
+            if (e == hipSuccess && c->F) e = hipMemcpyAsync(c->d_fstart, c->fstart.data(), 4ull * c->F, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess && c->F) e = hipMemcpyAsync(c->d_fcount, c->fcount.data(), 4ull * c->F, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) rc = set_error(c, COLO_EHIP, hipGetErrorString(e));
+        }
+        if (!rc) rc = fill_u32(c->d_flag, 4, 0, s);
+        if (!rc) rc = (c->cellbits + c->fidbits <= 32) ? bin_impl<uint32_t>(c, s) : bin_impl<uint64_t>(c, s);
+    }
+    *out = c;  // returned even on error so colo_last_error can be read
+    return rc;
+}
+
+int colo_ctx_destroy(colo_ctx *c) {
+    if (!c) return COLO_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->d_x, c->d_y, c->d_sx, c->d_sy, c->d_cell_of, c->d_sidx, c->d_spos, c->d_scell, c->d_dir,
+                    c->d_fstart, c->d_fcount, c->d_cellmask, c->d_qmask, c->d_run_cell, c->d_flag, c->d_scratch};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return COLO_OK;
+}
+
+int colo_get_grid(const colo_ctx *c, colo_grid *o) {
+    if (!c || !o) return COLO_EARG;
+    o->minX = c->minX; o->minY = c->minY; o->cellWidth = c->r; o->r2 = c->r2;
+    o->gx = c->gx; o->gy = c->gy; o->cells = c->cells; o->n = c->n; o->F = c->F;
+    return COLO_OK;
+}
+
+int colo_binning_export(colo_ctx *c, uint32_t *d_cell_of, uint32_t *d_sorted_idx, uint32_t *d_dir,
+                        uint64_t *dir_len, void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    hipStream_t s = pick_stream(c, stream);
+    if (d_cell_of && c->n) COLO_HIP(hipMemcpyAsync(d_cell_of, c->d_cell_of, 4ull * c->n, hipMemcpyDeviceToDevice, s));
+    if (d_sorted_idx && c->n) COLO_HIP(hipMemcpyAsync(d_sorted_idx, c->d_sidx, 4ull * c->n, hipMemcpyDeviceToDevice, s));
+    if (d_dir) COLO_HIP(hipMemcpyAsync(d_dir, c->d_dir, 4ull * c->dir_len, hipMemcpyDeviceToDevice, s));
+    if (dir_len) *dir_len = c->dir_len;
+    return COLO_OK;
+}
+
+int colo_filter(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta, uint32_t *d_counts,
+                uint8_t *d_pass, void *stream) {
+    if (!c || k < 2 || k > kMaxK) return COLO_EARG;
+    return filter_batch(c, d_cands, C, k, theta, d_counts, d_pass, pick_stream(c, stream));
+}
+
+uint64_t colo_bits_words(const colo_ctx *c, uint32_t k, const uint32_t *cand) {
+    uint64_t so[kMaxK], w = 0;
+    if (!c || k < 2 || k > kMaxK) return 0;
+    seg_layout(c, k, cand, 0, so, &w);
+    return w;
+}
+
+static int check_cand(colo_ctx *c, uint32_t k, const uint32_t *cand) {
+    if (k < 2 || k > kMaxK) return set_error(c, COLO_EARG, "pattern size out of range");
+    for (uint32_t j = 0; j < k; j++)
+        if (cand[j] >= c->F) return set_error(c, COLO_EARG, "feature id out of range");
+    return COLO_OK;
+}
+
+static int sum_slots(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *rows, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (!rows) return COLO_OK;
+    if (!n) { *rows = 0; return COLO_OK; }
+    void *tmp;
+    int rc = scratch(c, 8 * n + 8 * scan_tmp_words(n) + 16, &tmp);
+    if (rc) return rc;
+    uint64_t *offs = (uint64_t *)tmp;
+    uint64_t *tot = offs + n;
+    scan_rec<uint32_t, uint64_t>(d_slots, n, offs, tot, tot + 1, s);
+    COLO_HIP(hipMemcpyAsync(c->h_pinned, tot, 8, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    *rows = c->h_pinned[0];
+    return COLO_OK;
+}
+
+int colo_join2_count(colo_ctx *c, uint32_t a, uint32_t b, uint32_t *d_slots, uint32_t *d_bits, uint64_t *rows,
+                     void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    uint32_t cand[2] = {a, b};
+    int rc = check_cand(c, 2, cand);
+    if (rc) return rc;
+    hipStream_t s = pick_stream(c, stream);
+    uint64_t so[2], words;
+    seg_layout(c, 2, cand, 0, so, &words);
+    if ((rc = fill_u32(d_bits, words, 0, s))) return rc;
+    uint32_t Na = c->fcount[a];
+    hipLaunchKernelGGL(k_join2_count, dim3((Na + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->dev(), a, b, d_slots,
+                       d_bits + so[0], d_bits + so[1]);
+    COLO_HIP(hipGetLastError());
+    return sum_slots(c, d_slots, Na, rows, s);
+}
+
+int colo_scan_slots(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, uint64_t *total,
+                    void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    hipStream_t s = pick_stream(c, stream);
+    if (!n) {
+        if (total) *total = 0;
+        return COLO_OK;
+    }
+    void *tmp;
+    int rc = scratch(c, 8 * scan_tmp_words(n) + 16, &tmp);
+    if (rc) return rc;
+    uint64_t *tot = (uint64_t *)tmp;
+    scan_rec<uint32_t, uint64_t>(d_slots, n, d_offsets, tot, tot + 1, s);
+    COLO_HIP(hipGetLastError());
+    if (total) {
+        COLO_HIP(hipMemcpyAsync(c->h_pinned, tot, 8, hipMemcpyDeviceToHost, s));
+        COLO_HIP(hipStreamSynchronize(s));
+        *total = c->h_pinned[0];
+    }
+    return COLO_OK;
+}
+
+int colo_join2_write(colo_ctx *c, uint32_t a, uint32_t b, const uint32_t *d_slots, const uint64_t *d_offsets,
+                     uint32_t *d_table, void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    uint32_t cand[2] = {a, b};
+    int rc = check_cand(c, 2, cand);
+    if (rc) return rc;
+    hipStream_t s = pick_stream(c, stream);
+    uint32_t Na = c->fcount[a];
+    hipLaunchKernelGGL(k_join2_write, dim3((Na + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->dev(), a, b, d_slots,
+                       d_offsets, d_table);
+    COLO_HIP(hipGetLastError());
+    return COLO_OK;
+}
+
+static CandK make_candk(const colo_ctx *c, uint32_t k, const uint32_t *cand) {
+    CandK ck{};
+    ck.km1 = k - 1;
+    ck.fl = cand[k - 1];
+    for (uint32_t j = 0; j < k; j++) ck.feat[j] = cand[j];
+    seg_layout(c, k, cand, 0, ck.seg_off, nullptr);
+    return ck;
+}
+
+int colo_joink_count(colo_ctx *c, const uint32_t *d_parent, uint64_t R, uint32_t k, const uint32_t *cand,
+                     uint32_t *d_slots, uint32_t *d_bits, uint64_t *rows, void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    int rc = check_cand(c, k, cand);
+    if (rc) return rc;
+    if (k < 3) return set_error(c, COLO_EARG, "colo_joink_count needs k >= 3");
+    hipStream_t s = pick_stream(c, stream);
+    CandK ck = make_candk(c, k, cand);
+    uint64_t words;
+    uint64_t so[kMaxK];
+    seg_layout(c, k, cand, 0, so, &words);
+    if ((rc = fill_u32(d_bits, words, 0, s))) return rc;
+    if (R) {
+        hipLaunchKernelGGL(k_joink_count, dim3((unsigned)((R + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, c->dev(),
+                           d_parent, R, ck, d_slots, d_bits);
+        COLO_HIP(hipGetLastError());
+    }
+    return sum_slots(c, d_slots, R, rows, s);
+}
+
+int colo_joink_write(colo_ctx *c, const uint32_t *d_parent, uint64_t R, uint32_t k, const uint32_t *cand,
+                     const uint32_t *d_slots, const uint64_t *d_offsets, uint32_t *d_table, void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    int rc = check_cand(c, k, cand);
+    if (rc) return rc;
+    if (k < 3) return set_error(c, COLO_EARG, "colo_joink_write needs k >= 3");
+    hipStream_t s = pick_stream(c, stream);
+    CandK ck = make_candk(c, k, cand);
+    if (R) {
+        hipLaunchKernelGGL(k_joink_write, dim3((unsigned)((R + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, c->dev(),
+                           d_parent, R, ck, d_slots, d_offsets, d_table);
+        COLO_HIP(hipGetLastError());
+    }
+    return COLO_OK;
+}
+
+int colo_pi(colo_ctx *c, const uint32_t *d_bits, uint32_t k, const uint32_t *cand, uint64_t *num_compat,
+            uint64_t *num_exact, float *pi_compat, float *pi_exact, void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    int rc = check_cand(c, k, cand);
+    if (rc) return rc;
+    hipStream_t s = pick_stream(c, stream);
+    SegDesc sd[kMaxK];
+    seg_descs(c, k, cand, 0, sd);
+    void *tmp;
+    if ((rc = scratch(c, sizeof(sd) + sizeof(SegOut) * kMaxK, &tmp))) return rc;
+    SegDesc *d_sd = (SegDesc *)tmp;
+    SegOut *d_so = (SegOut *)((char *)tmp + sizeof(sd));
+    COLO_HIP(hipMemcpyAsync(d_sd, sd, sizeof(SegDesc) * k, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pi_abs, dim3(k), dim3(kBlock), 0, s, d_bits, d_sd, d_so);
+    COLO_HIP(hipGetLastError());
+    SegOut so[kMaxK];
+    COLO_HIP(hipMemcpyAsync(so, d_so, sizeof(SegOut) * k, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    pi_from_segs(c, k, cand, so, num_compat, num_exact, pi_compat, pi_exact);
+    return COLO_OK;
+}
+
+int colo_participation(colo_ctx *c, const uint32_t *d_bits, uint32_t k, const uint32_t *cand, uint8_t *d_flags,
+                       void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    int rc = check_cand(c, k, cand);
+    if (rc) return rc;
+    hipStream_t s = pick_stream(c, stream);
+    SegDesc sd[kMaxK];
+    seg_descs(c, k, cand, 0, sd);
+    uint64_t o = 0;
+    for (uint32_t j = 0; j < k; j++) {
+        uint32_t N = c->fcount[cand[j]];
+        hipLaunchKernelGGL(k_participation, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_bits, sd[j], N,
+                           cand[j], j + 1 == k, c->d_spos, c->d_fstart, d_flags + o);
+        o += N;
+    }
+    COLO_HIP(hipGetLastError());
+    return COLO_OK;
+}
+
+}  // extern "C"
+
+// batched PI over absolute segment offsets (the miner's arena)
+namespace colo {
+int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
+                 hipStream_t s) {
+    if (!n_segs) return COLO_OK;
+    hipLaunchKernelGGL(k_pi_abs, dim3((unsigned)n_segs), dim3(kBlock), 0, s, d_bits, d_segs, d_out);
+    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "pi launch");
+}
+int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s) {
+    if (!n) return COLO_OK;
+    void *tmp;
+    int rc = scratch(c, 8 * scan_tmp_words(n) + 16, &tmp);
+    if (rc) return rc;
+    scan_rec<uint32_t, uint64_t>(d_slots, n, d_offsets, (uint64_t *)tmp, (uint64_t *)tmp + 1, s);
+    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "scan launch");
+}
+}  // namespace colo
+
+// Position-sensitive table checksum: sum over i of mix64((i << 32) ^ t[i]).
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_table_hash(const uint32_t *t, uint64_t n, unsigned long long *out) {
+    uint64_t s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        s += mix64((i << 32) ^ t[i]);
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(s, ex);
+    if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)tot);
+}
+
+__global__ void k_gather_u64(const uint64_t *a, const uint64_t *idx, uint64_t m, uint64_t *out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = a[idx[i]];
+}
+
+namespace colo {
+int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t n, uint64_t *h, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    void *tmp;
+    int rc = scratch(c, 16, &tmp);
+    if (rc) return rc;
+    if ((rc = fill_u32((uint32_t *)tmp, 2, 0, s))) return rc;
+    if (n) hipLaunchKernelGGL(k_table_hash, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, d_t, n,
+                              (unsigned long long *)tmp);
+    COLO_HIP(hipGetLastError());
+    COLO_HIP(hipMemcpyAsync(c->h_pinned, tmp, 8, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    *h = c->h_pinned[0];
+    return COLO_OK;
+}
+int gather_u64(colo_ctx *c, const uint64_t *d_a, const uint64_t *d_idx, uint64_t m, uint64_t *d_out, hipStream_t s) {
+    if (!m) return COLO_OK;
+    hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(m, 1u << 30)), dim3(kBlock), 0, s, d_a, d_idx, m, d_out);
+    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "gather launch");
+}
+}  // namespace colo

@@ -1,0 +1,138 @@
+// colo_internal.h -- shared definitions of the device layer and the host
+// miner.  Not part of the C ABI (include/colo.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/colo.h"
+
+namespace colo {
+
+constexpr int kMaxK = 16;          // largest pattern size the kernels accept
+constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
+
+// Grid geometry as the kernels read it (gridBox, Codes/DataTypes.h:77-83).
+struct DevGrid {
+    float minX, minY, r, r2;
+    uint32_t gx, gy, cells, F;
+};
+
+// Device-resident dataset after binning.  Feature-major sorted order: the
+// instances of feature f occupy sorted positions [fstart[f], fstart[f]+N_f),
+// ordered by (cell, input index).
+struct DevData {
+    DevGrid g;
+    const float *x, *y;            // input order
+    const uint32_t *cell_of;       // input order -> cell
+    const uint32_t *sidx;          // sorted position -> input index
+    const uint32_t *spos;          // input index -> sorted position
+    const float *sx, *sy;          // coordinates in sorted order
+    const uint32_t *scell;         // cell in sorted order
+    const uint32_t *dir;           // F x (cells+1): first sorted position of (f, c)
+    const uint32_t *fstart;        // F
+    const uint32_t *fcount;        // F
+};
+
+// One candidate of a batched (persistent) level launch.
+struct CandDesc {
+    uint32_t k;
+    uint32_t f[kMaxK];
+    uint64_t n_anchor;             // N_{p0} at level 2, parent rows at level k
+    const uint32_t *parent;        // level k: parent table (n_anchor x (k-1))
+    uint64_t slots_off;            // offset of this candidate's slots in the arena
+    uint64_t bits_off;             // word offset of the bitmap in the bits arena
+    uint64_t seg_off[kMaxK];       // word offset of segment j inside the bitmap
+    const uint64_t *offsets;       // write pass: row offsets (n_anchor), global
+    uint64_t off_base;             // write pass: global offset of row 0
+    uint32_t *table;               // write pass: output table
+};
+
+// One work item of a batched launch: anchors [begin, end) of candidate cand.
+struct WorkItem {
+    uint32_t cand;
+    uint32_t pad;
+    uint64_t begin, end;
+};
+
+// PI segment descriptor: popcount of words [off, off+words) plus the bits of
+// the instances with input rank 0 and N-1.
+struct SegDesc {
+    uint64_t off, words;
+    uint64_t first_bit, last_bit;
+};
+struct SegOut {
+    uint64_t pop;
+    uint32_t first, last;
+};
+
+}  // namespace colo
+
+struct colo_ctx {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t n = 0, F = 0;
+    std::vector<uint32_t> fstart, fcount;
+    std::vector<uint32_t> first_sorted, last_sorted;  // sorted rank of input rank 0 / N-1
+    std::vector<std::string> labels;
+    float minX = 0, minY = 0, maxX = 0, maxY = 0, r = 0, r2 = 0;
+    uint32_t gx = 0, gy = 0, cells = 0;
+    uint32_t cellbits = 0, fidbits = 0;
+    // device buffers
+    float *d_x = nullptr, *d_y = nullptr, *d_sx = nullptr, *d_sy = nullptr;
+    uint32_t *d_cell_of = nullptr, *d_sidx = nullptr, *d_spos = nullptr, *d_scell = nullptr;
+    uint32_t *d_dir = nullptr;
+    uint64_t dir_len = 0;
+    uint32_t *d_fstart = nullptr, *d_fcount = nullptr;
+    // filter support
+    uint64_t *d_cellmask = nullptr, *d_qmask = nullptr;
+    uint32_t *d_run_cell = nullptr;
+    uint64_t H = 0;
+    std::vector<uint64_t> run_off;  // F+1
+    uint32_t *d_flag = nullptr;     // device error flags
+    // reusable scratch
+    void *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    uint64_t *h_pinned = nullptr;   // small pinned host staging
+    std::string err;
+    colo::DevData dev() const;
+};
+
+namespace colo {
+// Device-layer helpers used by the miner (colo_device.hip).
+int set_error(colo_ctx *c, int code, const std::string &msg);
+hipStream_t pick_stream(colo_ctx *c, void *s);
+int scratch(colo_ctx *c, size_t bytes, void **p);
+int scan_u32_u64(colo_ctx *c, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *total_dev,
+                 hipStream_t s);
+int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+int filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
+                 uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
+// Batched passes over many candidates (one launch each).
+int count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items,
+                uint32_t k, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s);
+int write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items,
+                uint32_t k, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
+int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
+                 hipStream_t s);
+int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t n, uint64_t *h, hipStream_t s);
+int gather_u64(colo_ctx *c, const uint64_t *d_a, const uint64_t *d_idx, uint64_t m, uint64_t *d_out, hipStream_t s);
+int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s);
+// Segment descriptors of one candidate's bitmap (bits_off = base word).
+void seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off,
+                uint64_t *seg_off, uint64_t *words_total);
+void seg_descs(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, SegDesc *out);
+void pi_from_segs(const colo_ctx *c, uint32_t k, const uint32_t *cand, const SegOut *so,
+                  uint64_t *num_compat, uint64_t *num_exact, float *pi_compat, float *pi_exact);
+int num_cus();
+}  // namespace colo
+
+#define COLO_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return colo::set_error(ctx_for_err, COLO_EHIP,                              \
+                                   std::string(#call) + ": " + hipGetErrorString(e_));  \
+    } while (0)

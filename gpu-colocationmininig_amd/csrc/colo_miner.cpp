@@ -1,0 +1,798 @@
+// colo_miner.cpp -- host side of the miner: the colocationFinder role
+// (Codes/colocationFinder.cpp) on top of the device layer.
+//
+//  - input: config.txt (cpp:1077-1093), parameter.txt (cpp:57-74), the
+//    dataset CSV (cpp:139-254);
+//  - the Apriori level loop (cpp:1104-1137), candidate generation with the
+//    reference's first-failure break (cpp:517-557, 598-701);
+//  - per level: filter, count pass, PI, prevalence, instance tables for the
+//    prefixes of the next level's candidates (cpp:256-425, 736-909);
+//  - output: ColocationDegree_<k>.txt (cpp:960-997) and a report.
+//
+// Two schedules share the kernels: PLAIN launches and synchronises per
+// candidate (the reference's host loops cpp:315-348, 780-842, 860-908);
+// PERSISTENT runs each phase of a level as one launch over a device work
+// queue of (candidate, anchor range) items.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "colo_internal.h"
+
+using namespace colo;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+struct Str {
+    std::string s;
+    void printf(const char *fmt, ...) __attribute__((format(printf, 2, 3))) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        int k = vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        if (k < (int)sizeof buf) {
+            s.append(buf, k > 0 ? k : 0);
+            return;
+        }
+        std::vector<char> big(k + 1);
+        va_start(ap, fmt);
+        vsnprintf(big.data(), big.size(), fmt, ap);
+        va_end(ap);
+        s.append(big.data(), k);
+    }
+};
+
+uint32_t fbits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+// Hash set of patterns of one size (the subset checks of cpp:618-651).
+struct PatSet {
+    uint32_t w;
+    struct H {
+        uint32_t w;
+        size_t operator()(const std::vector<uint32_t> &v) const {
+            uint64_t h = 1469598103934665603ull;
+            for (uint32_t x : v) h = (h ^ x) * 1099511628211ull;
+            return (size_t)h;
+        }
+    };
+    std::unordered_set<std::vector<uint32_t>, H> set;
+    PatSet(const uint32_t *L, uint64_t P, uint32_t w_) : w(w_), set(P * 2 + 1, H{w_}) {
+        for (uint64_t i = 0; i < P; i++) set.emplace(L + i * w, L + (i + 1) * w);
+    }
+    bool has(const std::vector<uint32_t> &v) const { return set.count(v) != 0; }
+};
+
+// candidateColocationGeneral (cpp:517-557): L holds P patterns of size km1
+// in list order.  For each i, j runs over the following patterns while they
+// share L[i]'s prefix; a candidate whose (km1)-subsets at positions 2..km1
+// of the lexicographic subset enumeration (cpp:598-616, 669-679) are not
+// all in L ends the j loop (cpp:541-542).
+std::vector<uint32_t> gen_candidates(const std::vector<uint32_t> &L, uint32_t km1) {
+    std::vector<uint32_t> out;
+    const uint64_t P = km1 ? L.size() / km1 : 0;
+    if (P < 2) return out;
+    PatSet S(L.data(), P, km1);
+    const uint32_t deg = km1 + 1, li = deg - 2;
+    std::vector<uint32_t> inter(deg), sub(km1);
+    for (uint64_t i = 0; i < P; i++) {
+        const uint32_t *a = &L[i * km1];
+        for (uint64_t j = i + 1; j < P; j++) {
+            const uint32_t *b = &L[j * km1];
+            bool same = std::equal(a, a + li, b);
+            if (!(same && a[li] < b[li])) break;
+            std::copy(a, a + km1, inter.begin());
+            inter[km1] = b[li];
+            bool ok = true;
+            for (uint32_t s = 2; s < deg && ok; s++) {
+                uint32_t omit = deg - 1 - s, w = 0;
+                for (uint32_t t = 0; t < deg; t++)
+                    if (t != omit) sub[w++] = inter[t];
+                ok = S.has(sub);
+            }
+            if (!ok) break;
+            out.insert(out.end(), inter.begin(), inter.end());
+        }
+    }
+    return out;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    bool ensure(size_t b) {
+        if (b <= bytes && p) return true;
+        release();
+        if (hipMalloc(&p, b ? b : 8) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        bytes = b ? b : 8;
+        return true;
+    }
+    template <typename T>
+    T *as() const { return (T *)p; }
+};
+
+// A prevalent pattern of the current level.
+struct Prev {
+    std::vector<uint32_t> pat;
+    std::shared_ptr<DevBuf> slots;  // count-pass slots (until its table is built)
+    uint64_t slots_off = 0;         // offset inside a shared slots arena
+    uint64_t nslots = 0;
+    uint64_t rows = 0;
+    int64_t parent = -1;            // index into the previous level's list
+    std::shared_ptr<DevBuf> table;  // instance table, rows x k
+};
+
+struct Miner {
+    colo_ctx *c;
+    const colo_mine_opts &o;
+    colo_mine_stats st{};
+    hipStream_t s;
+    Str rep;
+    colo_ctx *ctx_for_err;
+
+    Miner(colo_ctx *c_, const colo_mine_opts &o_) : c(c_), o(o_), s(c_->stream), ctx_for_err(c_) {}
+
+    int sync() {
+        COLO_HIP(hipStreamSynchronize(s));
+        return COLO_OK;
+    }
+
+    void write_patterns(uint32_t k, const std::vector<Prev> &L) {
+        if (!o.out_prefix) return;
+        std::string path = std::string(o.out_prefix) + "ColocationDegree_" + std::to_string(k) + ".txt";
+        FILE *fp = fopen(path.c_str(), "wb");
+        if (!fp) return;
+        std::string buf;
+        for (const Prev &p : L) {  // savetoFile2 / savetoFileGen, cpp:960-997
+            buf += "( ";
+            for (uint32_t j = 0; j < k; j++) {
+                buf += c->labels[p.pat[j]];
+                buf += " ";
+                buf += j != k - 1 ? "| " : ")\n";
+            }
+        }
+        fwrite(buf.data(), 1, buf.size(), fp);
+        fclose(fp);
+    }
+
+    static int64_t find(const std::vector<Prev> &L, const uint32_t *v, uint32_t w) {
+        for (size_t i = 0; i < L.size(); i++)
+            if (std::equal(v, v + w, L[i].pat.begin())) return (int64_t)i;
+        return -1;
+    }
+
+    // Per-candidate result of the count pass.
+    struct CandRes {
+        bool pass = false;
+        uint64_t rows = 0;
+        uint64_t numc[kMaxK], nume[kMaxK];
+        float pic = 1.f, pie = 1.f;
+        bool prevalent = false;
+    };
+
+    int run_level_plain(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
+                        const std::vector<Prev> &Lprev, std::vector<CandRes> &res, std::vector<Prev> &L) {
+        const uint64_t nC = C.size() / k;
+        DevBuf bits;
+        uint64_t max_words = 0;
+        for (uint64_t i = 0; i < nC; i++)
+            if (pass[i]) max_words = std::max(max_words, colo_bits_words(c, k, &C[i * k]));
+        if (!bits.ensure(max_words * 4)) return set_error(c, COLO_ENOMEM, "bits buffer");
+        for (uint64_t i = 0; i < nC; i++) {
+            if (!pass[i]) continue;
+            const uint32_t *cand = &C[i * k];
+            CandRes &r = res[i];
+            r.pass = true;
+            auto slots = std::make_shared<DevBuf>();
+            uint64_t nslots;
+            int64_t parent = -1;
+            auto t0 = Clock::now();
+            int rc;
+            if (k == 2) {
+                nslots = c->fcount[cand[0]];
+                if (!slots->ensure(nslots * 4)) return set_error(c, COLO_ENOMEM, "slots");
+                rc = colo_join2_count(c, cand[0], cand[1], slots->as<uint32_t>(), bits.as<uint32_t>(), &r.rows, s);
+            } else {
+                parent = find(Lprev, cand, k - 1);
+                if (parent < 0 || !Lprev[parent].table) return set_error(c, COLO_EARG, "missing parent table");
+                nslots = Lprev[parent].rows;
+                if (!slots->ensure(nslots * 4)) return set_error(c, COLO_ENOMEM, "slots");
+                rc = colo_joink_count(c, Lprev[parent].table->as<uint32_t>(), nslots, k, cand,
+                                      slots->as<uint32_t>(), bits.as<uint32_t>(), &r.rows, s);
+            }
+            if (rc) return rc;
+            st.count_ms += ms_since(t0);
+            st.launches += 2;
+            auto t1 = Clock::now();
+            rc = colo_pi(c, bits.as<uint32_t>(), k, cand, r.numc, r.nume, &r.pic, &r.pie, s);
+            if (rc) return rc;
+            st.pi_ms += ms_since(t1);
+            st.launches += 1;
+            if (k == 2) st.l2_pairs += r.rows;
+            float pim = o.pi_mode ? r.pie : r.pic;
+            r.prevalent = !(pim < o.theta);  // cpp:335, 827
+            if (r.prevalent) {
+                Prev p;
+                p.pat.assign(cand, cand + k);
+                p.slots = slots;
+                p.nslots = nslots;
+                p.rows = r.rows;
+                p.parent = parent;
+                L.push_back(std::move(p));
+            }
+        }
+        return COLO_OK;
+    }
+
+    int build_tables_plain(uint32_t k, std::vector<Prev> &L, const std::vector<uint8_t> &need,
+                           const std::vector<Prev> &Lprev) {
+        DevBuf offs;
+        for (size_t i = 0; i < L.size(); i++) {
+            if (!need[i]) continue;
+            Prev &p = L[i];
+            auto t0 = Clock::now();
+            if (!offs.ensure(std::max<uint64_t>(p.nslots, 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets");
+            uint64_t total = 0;
+            int rc = colo_scan_slots(c, p.slots->as<uint32_t>() + p.slots_off, p.nslots, offs.as<uint64_t>(), &total, s);
+            if (rc) return rc;
+            st.scan_ms += ms_since(t0);
+            auto t1 = Clock::now();
+            p.table = std::make_shared<DevBuf>();
+            if (!p.table->ensure(std::max<uint64_t>(total * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
+            if (k == 2)
+                rc = colo_join2_write(c, p.pat[0], p.pat[1], p.slots->as<uint32_t>() + p.slots_off, offs.as<uint64_t>(),
+                                      p.table->as<uint32_t>(), s);
+            else
+                rc = colo_joink_write(c, Lprev[p.parent].table->as<uint32_t>(), p.nslots, k, p.pat.data(),
+                                      p.slots->as<uint32_t>() + p.slots_off, offs.as<uint64_t>(), p.table->as<uint32_t>(), s);
+            if (rc) return rc;
+            if ((rc = sync())) return rc;
+            st.write_ms += ms_since(t1);
+            st.launches += 2;
+            if (k == 2) st.l2_rows += total;
+        }
+        return COLO_OK;
+    }
+
+    // PERSISTENT: one count launch, one PI launch, one scan for the level.
+    DevBuf arena_slots, arena_bits, arena_offs, d_desc, d_items, d_segs, d_segout, d_queue, d_idx, d_rowsel;
+
+    static void make_items(uint64_t cand, uint64_t n, uint64_t chunk, std::vector<WorkItem> &items) {
+        for (uint64_t b = 0; b < n; b += chunk) items.push_back(WorkItem{(uint32_t)cand, 0, b, std::min(n, b + chunk)});
+    }
+
+    int run_level_persistent(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
+                             const std::vector<Prev> &Lprev, std::vector<CandRes> &res, std::vector<Prev> &L,
+                             std::shared_ptr<DevBuf> &slots_arena) {
+        const uint64_t nC = C.size() / k;
+        std::vector<uint64_t> idx;  // passing candidates
+        for (uint64_t i = 0; i < nC; i++)
+            if (pass[i]) idx.push_back(i);
+        const uint64_t m = idx.size();
+        if (!m) return COLO_OK;
+        std::vector<CandDesc> desc(m);
+        std::vector<int64_t> parent(m, -1);
+        uint64_t slots_tot = 0, bits_tot = 0;
+        for (uint64_t u = 0; u < m; u++) {
+            const uint32_t *cand = &C[idx[u] * k];
+            CandDesc &d = desc[u];
+            memset(&d, 0, sizeof d);
+            d.k = k;
+            for (uint32_t j = 0; j < k; j++) d.f[j] = cand[j];
+            if (k == 2) {
+                d.n_anchor = c->fcount[cand[0]];
+            } else {
+                parent[u] = find(Lprev, cand, k - 1);
+                if (parent[u] < 0 || !Lprev[parent[u]].table) return set_error(c, COLO_EARG, "missing parent table");
+                d.n_anchor = Lprev[parent[u]].rows;
+                d.parent = Lprev[parent[u]].table->as<uint32_t>();
+            }
+            d.slots_off = slots_tot;
+            slots_tot += d.n_anchor;
+            uint64_t words;
+            seg_layout(c, k, cand, 0, d.seg_off, &words);
+            d.bits_off = bits_tot;
+            bits_tot += words;
+        }
+        // work queue: largest candidates first so the tail of the launch is short
+        std::vector<uint64_t> order(m);
+        for (uint64_t u = 0; u < m; u++) order[u] = u;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint64_t a, uint64_t b) { return desc[a].n_anchor > desc[b].n_anchor; });
+        std::vector<WorkItem> items;
+        for (uint64_t u : order) make_items(u, desc[u].n_anchor, k == 2 ? 1024 : 1024, items);
+        slots_arena = std::make_shared<DevBuf>();
+        if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "slots arena");
+        if (!arena_bits.ensure(std::max<uint64_t>(bits_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "bits arena");
+        if (!d_desc.ensure(m * sizeof(CandDesc)) || !d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(WorkItem)) ||
+            !d_queue.ensure(16))
+            return set_error(c, COLO_ENOMEM, "descriptors");
+        COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), m * sizeof(CandDesc), hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+        auto t0 = Clock::now();
+        int rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
+        if (rc) return rc;
+        rc = count_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), items.size(), k, slots_arena->as<uint32_t>(),
+                         arena_bits.as<uint32_t>(), d_queue.as<uint32_t>(), s);
+        if (rc) return rc;
+        st.launches += 2;
+        // rows per candidate: one scan over the whole slots arena
+        if (!arena_offs.ensure((slots_tot + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
+        rc = scan_batch(c, slots_arena->as<uint32_t>(), slots_tot, arena_offs.as<uint64_t>(), s);
+        if (rc) return rc;
+        std::vector<uint64_t> bounds(m + 1);
+        for (uint64_t u = 0; u < m; u++) bounds[u] = desc[u].slots_off;
+        bounds[m] = slots_tot;
+        // offset at position slots_tot = total: computed as the last offset + last slot
+        if (!d_idx.ensure((m + 1) * 8) || !d_rowsel.ensure((m + 2) * 8)) return set_error(c, COLO_ENOMEM, "idx");
+        COLO_HIP(hipMemcpyAsync(d_idx.p, bounds.data(), m * 8, hipMemcpyHostToDevice, s));
+        rc = gather_u64(c, arena_offs.as<uint64_t>(), d_idx.as<uint64_t>(), m, d_rowsel.as<uint64_t>(), s);
+        if (rc) return rc;
+        std::vector<uint64_t> starts(m + 1, 0);
+        uint32_t last_slot = 0;
+        uint64_t last_off = 0;
+        COLO_HIP(hipMemcpyAsync(starts.data(), d_rowsel.p, m * 8, hipMemcpyDeviceToHost, s));
+        if (slots_tot) {
+            COLO_HIP(hipMemcpyAsync(&last_slot, slots_arena->as<uint32_t>() + slots_tot - 1, 4, hipMemcpyDeviceToHost, s));
+            COLO_HIP(hipMemcpyAsync(&last_off, arena_offs.as<uint64_t>() + slots_tot - 1, 8, hipMemcpyDeviceToHost, s));
+        }
+        // PI segments of every candidate, one launch
+        std::vector<SegDesc> segs(m * k);
+        for (uint64_t u = 0; u < m; u++) seg_descs(c, k, &C[idx[u] * k], desc[u].bits_off, &segs[u * k]);
+        if (!d_segs.ensure(segs.size() * sizeof(SegDesc)) || !d_segout.ensure(segs.size() * sizeof(SegOut)))
+            return set_error(c, COLO_ENOMEM, "segments");
+        COLO_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
+        rc = pi_batch_abs(c, arena_bits.as<uint32_t>(), d_segs.as<SegDesc>(), segs.size(), d_segout.as<SegOut>(), s);
+        if (rc) return rc;
+        std::vector<SegOut> so(segs.size());
+        COLO_HIP(hipMemcpyAsync(so.data(), d_segout.p, so.size() * sizeof(SegOut), hipMemcpyDeviceToHost, s));
+        if ((rc = sync())) return rc;
+        st.count_ms += ms_since(t0);
+        st.launches += 4;
+        starts[m] = slots_tot ? last_off + last_slot : 0;
+        for (uint64_t u = 0; u < m; u++) {
+            const uint32_t *cand = &C[idx[u] * k];
+            CandRes &r = res[idx[u]];
+            r.pass = true;
+            r.rows = starts[u + 1] - starts[u];
+            pi_from_segs(c, k, cand, &so[u * k], r.numc, r.nume, &r.pic, &r.pie);
+            if (k == 2) st.l2_pairs += r.rows;
+            float pim = o.pi_mode ? r.pie : r.pic;
+            r.prevalent = !(pim < o.theta);
+            if (r.prevalent) {
+                Prev p;
+                p.pat.assign(cand, cand + k);
+                p.slots = slots_arena;
+                p.slots_off = desc[u].slots_off;
+                p.nslots = desc[u].n_anchor;
+                p.rows = r.rows;
+                p.parent = parent[u];
+                L.push_back(std::move(p));
+            }
+        }
+        level_starts_ = std::move(starts);
+        level_desc_ = std::move(desc);
+        level_idx_ = std::move(idx);
+        return COLO_OK;
+    }
+    std::vector<uint64_t> level_starts_;
+    std::vector<CandDesc> level_desc_;
+    std::vector<uint64_t> level_idx_;
+
+    int build_tables_persistent(uint32_t k, std::vector<Prev> &L, const std::vector<uint8_t> &need,
+                                const std::vector<Prev> &Lprev, const std::vector<uint32_t> &C) {
+        // the offsets arena still holds the level's global row offsets
+        std::vector<CandDesc> desc;
+        std::vector<WorkItem> items;
+        auto t0 = Clock::now();
+        std::vector<uint64_t> cand_of_prev(L.size(), UINT64_MAX);
+        for (uint64_t u = 0, pi = 0; u < level_idx_.size() && pi < L.size(); u++)
+            if (std::equal(L[pi].pat.begin(), L[pi].pat.end(), &C[level_idx_[u] * k])) cand_of_prev[pi++] = u;
+        for (size_t i = 0; i < L.size(); i++) {
+            if (!need[i]) continue;
+            Prev &p = L[i];
+            uint64_t u = cand_of_prev[i];
+            if (u == UINT64_MAX) return set_error(c, COLO_EARG, "internal: prevalent pattern not in level");
+            p.table = std::make_shared<DevBuf>();
+            if (!p.table->ensure(std::max<uint64_t>(p.rows * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
+            CandDesc d = level_desc_[u];
+            d.offsets = arena_offs.as<uint64_t>() + d.slots_off;
+            d.off_base = level_starts_[u];
+            d.table = p.table->as<uint32_t>();
+            make_items(desc.size(), d.n_anchor, 1024, items);
+            desc.push_back(d);
+            if (k == 2) st.l2_rows += p.rows;
+        }
+        if (desc.empty()) return COLO_OK;
+        if (!d_desc.ensure(desc.size() * sizeof(CandDesc)) || !d_items.ensure(items.size() * sizeof(WorkItem)))
+            return set_error(c, COLO_ENOMEM, "descriptors");
+        COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+        int rc = write_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), items.size(), k,
+                             L[0].slots ? L[0].slots->as<uint32_t>() : nullptr, d_queue.as<uint32_t>(), s);
+        if (rc) return rc;
+        if ((rc = sync())) return rc;
+        st.write_ms += ms_since(t0);
+        st.launches += 2;
+        (void)Lprev;
+        return COLO_OK;
+    }
+
+    int run(char **report) {
+        auto T0 = Clock::now();
+        const uint32_t F = c->F;
+        rep.printf("colo-report v1\n");
+        rep.printf("grid gx=%u gy=%u cells=%u n=%u F=%u r2=%08x theta=%08x\n", c->gx, c->gy, c->cells, c->n, F,
+                   fbits(c->r2), fbits(o.theta));
+        std::vector<uint32_t> C;
+        for (uint32_t i = 0; i + 1 < F; i++)  // cpp:75-82
+            for (uint32_t j = i + 1; j < F; j++) {
+                C.push_back(i);
+                C.push_back(j);
+            }
+        std::vector<Prev> Lprev;
+        uint32_t k = 2;
+        uint64_t total_prev = 0;
+        uint32_t levels = 0;
+        DevBuf d_cands, d_counts, d_pass;
+        while (!C.empty() && k <= F && (o.max_k == 0 || k <= o.max_k)) {
+            levels++;
+            const uint64_t nC = C.size() / k;
+            st.candidates += nC;
+            // filter (cpp:297, 760)
+            auto tf = Clock::now();
+            std::vector<uint32_t> fcnt(nC * k, 0);
+            std::vector<uint8_t> pass(nC, 1);
+            if (o.filter_on) {
+                if (!d_cands.ensure(C.size() * 4) || !d_counts.ensure(C.size() * 4) || !d_pass.ensure(nC))
+                    return set_error(c, COLO_ENOMEM, "filter buffers");
+                COLO_HIP(hipMemcpyAsync(d_cands.p, C.data(), C.size() * 4, hipMemcpyHostToDevice, s));
+                int rc = filter_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
+                                      d_pass.as<uint8_t>(), s);
+                if (rc) return rc;
+                COLO_HIP(hipMemcpyAsync(fcnt.data(), d_counts.p, fcnt.size() * 4, hipMemcpyDeviceToHost, s));
+                COLO_HIP(hipMemcpyAsync(pass.data(), d_pass.p, nC, hipMemcpyDeviceToHost, s));
+                if ((rc = sync())) return rc;
+                st.launches += 2;
+            }
+            st.filter_ms += ms_since(tf);
+            uint64_t npass = 0;
+            for (uint8_t p : pass) npass += p;
+            std::vector<CandRes> res(nC);
+            std::vector<Prev> L;
+            std::shared_ptr<DevBuf> arena;
+            auto tl = Clock::now();
+            int rc = o.sched == COLO_SCHED_PLAIN ? run_level_plain(k, C, pass, Lprev, res, L)
+                                                 : run_level_persistent(k, C, pass, Lprev, res, L, arena);
+            if (rc) return rc;
+            double level_count_ms = ms_since(tl);
+            rep.printf("level %u cand %llu pass %llu prev %llu\n", k, (unsigned long long)nC,
+                       (unsigned long long)npass, (unsigned long long)L.size());
+            for (uint64_t i = 0; i < nC; i++) {
+                const uint32_t *cand = &C[i * k];
+                rep.printf("c");
+                for (uint32_t m = 0; m < k; m++) rep.printf(" %u", cand[m]);
+                rep.printf(" f");
+                for (uint32_t m = 0; m < k; m++) rep.printf(" %u", fcnt[i * k + m]);
+                rep.printf(" p %u", (unsigned)pass[i]);
+                if (!pass[i]) {
+                    rep.printf("\n");
+                    continue;
+                }
+                const CandRes &r = res[i];
+                rep.printf(" r %llu n", (unsigned long long)r.rows);
+                for (uint32_t m = 0; m < k; m++) rep.printf(" %llu", (unsigned long long)r.numc[m]);
+                rep.printf(" e");
+                for (uint32_t m = 0; m < k; m++) rep.printf(" %llu", (unsigned long long)r.nume[m]);
+                rep.printf(" pi %08x %08x v %d\n", fbits(r.pic), fbits(r.pie), r.prevalent ? 1 : 0);
+            }
+            total_prev += L.size();
+            auto th = Clock::now();
+            write_patterns(k, L);
+            // next level candidates (cpp:356, 1124)
+            std::vector<uint32_t> N;
+            if (o.max_k == 0 || k + 1 <= o.max_k) {
+                std::vector<uint32_t> flat;
+                flat.reserve(L.size() * k);
+                for (const Prev &p : L) flat.insert(flat.end(), p.pat.begin(), p.pat.end());
+                N = gen_candidates(flat, k);
+                // tableGenRequired (cpp:479-515) joins the candidates themselves;
+                // record where it would have skipped tables the next level needs
+                if (k >= 3 && !N.empty() && gen_candidates(C, k).empty()) rep.printf("q10 level %u\n", k);
+            }
+            std::vector<uint8_t> need(L.size(), 0);
+            {
+                // prefixes of the next level's candidates (cpp:364-372, 1055-1065)
+                std::unordered_set<std::vector<uint32_t>, PatSet::H> pref(16, PatSet::H{k});
+                for (uint64_t i = 0; i < N.size() / (k + 1); i++)
+                    pref.emplace(N.begin() + i * (k + 1), N.begin() + i * (k + 1) + k);
+                for (size_t i = 0; i < L.size(); i++) need[i] = pref.count(L[i].pat) ? 1 : 0;
+            }
+            st.host_ms += ms_since(th);
+            auto tt = Clock::now();
+            rc = o.sched == COLO_SCHED_PLAIN ? build_tables_plain(k, L, need, Lprev)
+                                             : build_tables_persistent(k, L, need, Lprev, C);
+            if (rc) return rc;
+            if (k == 2) st.l2_join_ms = level_count_ms + ms_since(tt);
+            for (size_t i = 0; i < L.size(); i++) {
+                if (!need[i]) continue;
+                const Prev &p = L[i];
+                rep.printf("t");
+                for (uint32_t m = 0; m < k; m++) rep.printf(" %u", p.pat[m]);
+                rep.printf(" rows %llu", (unsigned long long)p.rows);
+                if (o.hash_tables) {
+                    uint64_t h = 0;
+                    if ((rc = table_hash(c, p.table->as<uint32_t>(), p.rows * k, &h, s))) return rc;
+                    rep.printf(" h %016llx", (unsigned long long)h);
+                }
+                rep.printf("\n");
+            }
+            for (Prev &p : L) p.slots.reset();
+            Lprev = std::move(L);
+            C = std::move(N);
+            k++;
+        }
+        rep.printf("end levels %u patterns %llu\n", levels, (unsigned long long)total_prev);
+        st.levels = levels;
+        st.patterns = total_prev;
+        st.total_ms = ms_since(T0);
+        if (report) {
+            *report = (char *)malloc(rep.s.size() + 1);
+            memcpy(*report, rep.s.c_str(), rep.s.size() + 1);
+        }
+        return COLO_OK;
+    }
+};
+
+// ------------------------------------------------------------- parsing
+
+// stoi (libstdc++: strtol base 10) on a field.
+bool parse_long(const char *s, long *v) {
+    char *e;
+    errno = 0;
+    long x = strtol(s, &e, 10);
+    if (e == s || errno) return false;
+    *v = x;
+    return true;
+}
+// stof (strtof: decimal -> binary32 correctly rounded).
+bool parse_flt(const char *s, float *v) {
+    char *e;
+    float x = strtof(s, &e);
+    if (e == s) return false;
+    *v = x;
+    return true;
+}
+
+// Field split of one line (boost::escaped_list_separator on numeric rows,
+// cpp:156,166): comma separated, surrounding double quotes dropped.
+int split(char *line, char **f, int maxf) {
+    int k = 0;
+    char *p = line;
+    while (k < maxf) {
+        if (*p == '"') {
+            char *e = strchr(p + 1, '"');
+            if (!e) return -1;
+            *e = 0;
+            f[k++] = p + 1;
+            p = e + 1;
+            if (*p == ',') { p++; continue; }
+            break;
+        }
+        f[k++] = p;
+        char *cm = strchr(p, ',');
+        if (!cm) break;
+        *cm = 0;
+        p = cm + 1;
+    }
+    return k;
+}
+
+}  // namespace
+
+struct colo_host_data {
+    std::vector<float> x, y;
+    std::vector<uint32_t> fcount;
+    std::vector<std::string> labels;
+    std::vector<const char *> label_ptrs;
+    float b[4] = {0, 0, 0, 0};
+    uint32_t n = 0;
+};
+
+extern "C" {
+
+void colo_free_str(char *s) { free(s); }
+
+uint64_t colo_candidates(const uint32_t *L, uint64_t P, uint32_t km1, uint32_t *out, uint64_t cap) {
+    if (!km1 || (!L && P)) return 0;
+    std::vector<uint32_t> l(L, L + P * km1);
+    std::vector<uint32_t> c = gen_candidates(l, km1);
+    uint64_t n = c.size() / (km1 + 1);
+    if (out) std::copy(c.begin(), c.begin() + std::min<uint64_t>(n, cap) * (km1 + 1), out);
+    return n;
+}
+
+int colo_load_csv(const char *path, colo_host_data **out) {
+    if (!path || !out) return COLO_EARG;
+    *out = nullptr;
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return COLO_EFILE;
+    fseek(fp, 0, SEEK_END);
+    long sz = ftell(fp);
+    fseek(fp, 0, SEEK_SET);
+    std::string buf((size_t)(sz > 0 ? sz : 0), '\0');
+    size_t got = sz > 0 ? fread(&buf[0], 1, (size_t)sz, fp) : 0;
+    fclose(fp);
+    buf.resize(got);
+    buf.push_back('\0');
+    auto h = std::make_unique<colo_host_data>();
+    char *p = &buf[0];
+    char *end = p + got;
+    auto next_line = [&](char *&ln) -> bool {  // getline semantics
+        if (p >= end) { ln = nullptr; return false; }
+        ln = p;
+        char *nl = (char *)memchr(p, '\n', (size_t)(end - p));
+        if (nl) { *nl = 0; p = nl + 1; } else { p = end; }
+        return true;
+    };
+    char *ln;
+    long nl;
+    if (!next_line(ln) || !parse_long(ln, &nl) || nl < 0 || nl > 0xFFFFFFFFL) return COLO_EPARSE;
+    h->n = (uint32_t)nl;
+    h->x.resize(h->n);
+    h->y.resize(h->n);
+    uint64_t row = 0;
+    long last = 0;
+    bool have = false;
+    for (;;) {  // rows until the first empty line (cpp:164)
+        if (!next_line(ln) || ln[0] == 0) break;
+        char *fld[3];
+        long lab;
+        if (split(ln, fld, 3) < 3 || !parse_long(fld[0], &lab)) return COLO_EPARSE;
+        if (row >= h->n) return COLO_EPARSE;
+        if (!have || lab != last) {  // cpp:168-183
+            h->fcount.push_back(0);
+            h->labels.emplace_back(fld[0]);
+            last = lab;
+            have = true;
+        }
+        if (!parse_flt(fld[1], &h->x[row]) || !parse_flt(fld[2], &h->y[row])) return COLO_EPARSE;
+        h->fcount.back()++;
+        row++;
+    }
+    if (row != h->n) return COLO_EPARSE;
+    char *fld[4];
+    if (!next_line(ln) || split(ln, fld, 4) < 4) return COLO_EPARSE;
+    for (int i = 0; i < 4; i++)
+        if (!parse_flt(fld[i], &h->b[i])) return COLO_EPARSE;
+    for (auto &l : h->labels) h->label_ptrs.push_back(l.c_str());
+    *out = h.release();
+    return COLO_OK;
+}
+
+int colo_host_dataset(const colo_host_data *h, colo_dataset *ds) {
+    if (!h || !ds) return COLO_EARG;
+    ds->n = h->n;
+    ds->F = (uint32_t)h->fcount.size();
+    ds->x = h->x.data();
+    ds->y = h->y.data();
+    ds->fcount = h->fcount.data();
+    ds->minX = h->b[0]; ds->minY = h->b[1]; ds->maxX = h->b[2]; ds->maxY = h->b[3];
+    ds->labels = h->label_ptrs.data();
+    return COLO_OK;
+}
+
+const char *colo_host_label(const colo_host_data *h, uint32_t f) {
+    return (h && f < h->labels.size()) ? h->labels[f].c_str() : "";
+}
+
+void colo_host_free(colo_host_data *h) { delete h; }
+
+int colo_mine(colo_ctx *c, const colo_mine_opts *opts, char **report, colo_mine_stats *stats) {
+    if (!c || !opts) return COLO_EARG;
+    if (report) *report = nullptr;
+    Miner M(c, *opts);
+    int rc = M.run(report);
+    if (stats) *stats = M.st;
+    return rc;
+}
+
+int colo_run_config(const char *config_path, const colo_mine_opts *opts, char **timing_line) {
+    using Clock = std::chrono::steady_clock;
+    auto T0 = Clock::now();
+    FILE *fp = fopen(config_path, "rb");
+    if (!fp) return COLO_EFILE;
+    std::vector<std::string> lines;
+    {
+        char b[8192];
+        while (lines.size() < 5 && fgets(b, sizeof b, fp)) {
+            size_t l = strlen(b);
+            if (l && b[l - 1] == '\n') b[--l] = 0;
+            lines.emplace_back(b);
+        }
+    }
+    fclose(fp);
+    while (lines.size() < 5) lines.emplace_back();
+    const std::string &location = lines[0], &param = lines[1], &dataset = lines[2];
+    // parameter.txt: r theta filterFlag (cpp:66-68)
+    float r = 0, theta = 0;
+    {
+        FILE *pf = fopen(param.c_str(), "rb");
+        if (!pf) return COLO_EFILE;
+        char a[128], b2[128], d[128];
+        int k = fscanf(pf, "%127s %127s %127s", a, b2, d);
+        fclose(pf);
+        if (k < 2 || !parse_flt(a, &r) || !parse_flt(b2, &theta)) return COLO_EPARSE;
+    }
+    auto tio = Clock::now();
+    colo_host_data *h = nullptr;
+    int rc = colo_load_csv(dataset.c_str(), &h);
+    if (rc) return rc;
+    double io_ms = std::chrono::duration<double, std::milli>(Clock::now() - tio).count();
+    colo_dataset ds;
+    colo_host_dataset(h, &ds);
+    auto tg = Clock::now();
+    colo_ctx *ctx = nullptr;
+    rc = colo_ctx_create(&ds, r, nullptr, &ctx);
+    double grid_ms = std::chrono::duration<double, std::milli>(Clock::now() - tg).count();
+    if (rc) {
+        if (ctx) fprintf(stderr, "colo: %s\n", colo_last_error(ctx));
+        colo_ctx_destroy(ctx);
+        colo_host_free(h);
+        return rc;
+    }
+    colo_mine_opts o{};
+    if (opts) o = *opts;
+    else { o.filter_on = 1; o.sched = COLO_SCHED_PERSISTENT; }
+    o.theta = theta;
+    o.out_prefix = location.c_str();
+    colo_mine_stats st{};
+    rc = colo_mine(ctx, &o, nullptr, &st);
+    if (rc) fprintf(stderr, "colo: %s\n", colo_last_error(ctx));
+    double e2e = std::chrono::duration<double, std::milli>(Clock::now() - T0).count();
+    if (timing_line && !rc) {
+        // the reference's summary line (cpp:1144): dataset, program, theta,
+        // io, transfer, grid, filter, refine, table generation, end to end (ms)
+        char b[1024];
+        snprintf(b, sizeof b, "%s , MI355X_colo , %g , %.3f , %.3f , %.3f , %.3f , %.3f , %.3f , %.3f", dataset.c_str(),
+                 (double)theta, io_ms, 0.0, grid_ms, st.filter_ms, st.count_ms + st.pi_ms, st.scan_ms + st.write_ms,
+                 e2e);
+        *timing_line = strdup(b);
+    }
+    colo_ctx_destroy(ctx);
+    colo_host_free(h);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+/*
+ * colo.h -- C ABI of the MI355X colocation-mining hot path.
+ *
+ * Drop-in boundary for the device layer of spatialdatasciencegroup/
+ * GPU-ColocationMininig (Codes/DeviceProcess.h:4-36, implemented in
+ * Codes/DeviceProcess.cu:56-374).  Each entry point below names the
+ * reference function it replaces.  Conventions:
+ *
+ *  - Plain C types only: pointers + sizes, POD structs, int return codes
+ *    (COLO_OK = 0).  Nothing calls exit().
+ *  - The reference keeps the dataset in __constant__ globals set by
+ *    initializeDeviceMemConst (Codes/Constant.h:5-22); here a context handle
+ *    (colo_ctx) owns the device-resident dataset, grid and binning.  Calls on
+ *    one context are single-threaded; separate contexts are independent.
+ *  - Pointers named d_* are device pointers (hipMalloc'd by the caller);
+ *    the others are host pointers.  "stream" is a hipStream_t (NULL = the
+ *    context's own stream).  Entry points that return host values
+ *    synchronise that stream; the others only enqueue work.
+ *  - Arithmetic follows the reference types (Codes/DataTypes.h:5-11):
+ *    coordinates are binary32, ids are uint32; offsets are 64-bit (the
+ *    reference's int32 offsets overflow past 2^31 rows).
+ *
+ * Participation bitmaps ("bits"): for a candidate P = (p_0..p_{k-1}) the
+ * buffer holds k segments, segment j = colo_bits_words() layout, one bit per
+ * instance of p_j.  The segment of the last feature is indexed by the
+ * instance's rank in the cell-sorted order of its feature, the others by the
+ * instance's rank in input order; colo_participation() expands a bitmap into
+ * per-instance flags in input order.
+ */
+#ifndef COLO_H
+#define COLO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes (the reference's StatusType, Codes/DataTypes.h:85-94, widened). */
+#define COLO_OK 0
+#define COLO_EFILE 1       /* input file missing or unreadable            */
+#define COLO_EPARSE 2      /* malformed config / parameter / CSV input    */
+#define COLO_EBOUNDS 3     /* a point lies outside the grid               */
+#define COLO_ENOMEM 4      /* device or host allocation failed            */
+#define COLO_EHIP 5        /* HIP runtime error                           */
+#define COLO_EARG 6        /* invalid argument                            */
+#define COLO_EUNSUPPORTED 7/* outside the supported envelope (e.g. F>64)  */
+
+typedef struct colo_ctx colo_ctx;
+
+/* Dataset as colocationFinder::populateData leaves it (cpp:139-254):
+ * n instances grouped by feature, fcount[f] instances of feature f, the
+ * bounds line of the CSV. */
+typedef struct colo_dataset {
+    uint32_t n;
+    uint32_t F;
+    const float *x;        /* host, n */
+    const float *y;        /* host, n */
+    const uint32_t *fcount;/* host, F */
+    float minX, minY, maxX, maxY;
+    const char *const *labels; /* host, F feature labels for the pattern files; NULL = "0".."F-1" */
+} colo_dataset;
+
+/* Grid geometry (gridBox, Codes/DataTypes.h:77-83, set by setGrid cpp:102-115). */
+typedef struct colo_grid {
+    float minX, minY, cellWidth, r2;
+    uint32_t gx, gy, cells;
+    uint32_t n, F;
+} colo_grid;
+
+/* Creates a context: copies the dataset to the device, sets the grid for
+ * neighbour distance r and bins the instances (on-device stable radix sort
+ * by (feature, cell), cell offsets by prefix sum).
+ * Replaces: initializeDeviceMemConst (DeviceProcess.h:11, cu:166-184) and
+ * CountCellFeatures (DeviceProcess.h:33-34, cu:312-369), called from
+ * colocationFinder::setGrid (cpp:102-126). */
+int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **out);
+int colo_ctx_destroy(colo_ctx *ctx);
+int colo_get_grid(const colo_ctx *ctx, colo_grid *out);
+/* Human-readable message for the last error on this context. */
+const char *colo_last_error(const colo_ctx *ctx);
+
+/* Copies the binning out (device -> device, input order for cell_of,
+ * sorted order for sorted_idx; dir = F x (cells+1) first sorted position of
+ * every (feature, cell)).  Any pointer may be NULL.  dir_len receives
+ * F*(cells+1).  Parity hook for the binning kernels. */
+int colo_binning_export(colo_ctx *ctx, uint32_t *d_cell_of, uint32_t *d_sorted_idx,
+                        uint32_t *d_dir, uint64_t *dir_len, void *stream);
+
+/* Cell-aggregate upper-bound filter for C candidates of size k (d_cands:
+ * C*k feature ids, device).  d_counts (C*k u32) receives the per-feature
+ * counts summed over the cells of qualifying 2x2 quadrants; d_pass (C
+ * bytes) receives min_t fl32(cnt_t / N_t) >= theta.
+ * Replaces: filter2 (DeviceProcess.h:18-19, cu:219-235) and filter2_2
+ * (DeviceProcess.h:20-21, cu:237-249; applied with the same rule). */
+int colo_filter(colo_ctx *ctx, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
+                uint32_t *d_counts, uint8_t *d_pass, void *stream);
+
+/* Number of 32-bit words of the participation bitmap of a candidate. */
+uint64_t colo_bits_words(const colo_ctx *ctx, uint32_t k, const uint32_t *cand);
+
+/* Level-2 count pass for candidate (a, b): d_slots[t] = number of
+ * b-instances within r of the t-th a-instance; d_bits (zeroed by the call)
+ * receives the participation bits.  *rows receives sum of slots if non-NULL
+ * (synchronises).
+ * Replaces: degree2InstanceTableSlotCounter (DeviceProcess.h:5, cu:58-68). */
+int colo_join2_count(colo_ctx *ctx, uint32_t a, uint32_t b, uint32_t *d_slots, uint32_t *d_bits,
+                     uint64_t *rows, void *stream);
+
+/* Exclusive scan of n slot counts into 64-bit row offsets; *total receives
+ * the sum (synchronises).
+ * Replaces: the host thrust::exclusive_scan (cpp:391, 887), getTotalInstances
+ * (DeviceProcess.h:13, cu:154-159) and scalebyConstant (DeviceProcess.h:15,
+ * cu:187-196; the write passes scale by the degree themselves). */
+int colo_scan_slots(colo_ctx *ctx, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets,
+                    uint64_t *total, void *stream);
+
+/* Level-2 write pass: rows (g, q) in a-instance order, neighbours in scan
+ * order, into d_table (2 * rows u32).
+ * Replaces: degree2InstanceTableGenerator (DeviceProcess.h:6, cu:69-79). */
+int colo_join2_write(colo_ctx *ctx, uint32_t a, uint32_t b, const uint32_t *d_slots,
+                     const uint64_t *d_offsets, uint32_t *d_table, void *stream);
+
+/* Level-k count pass (k >= 3) over the R rows of the parent table d_parent
+ * (R*(k-1) u32, instance table of cand[0..k-2]); cand is host, k ids.
+ * Replaces: generalInstanceTableSlotCounter (DeviceProcess.h:8, cu:82-92). */
+int colo_joink_count(colo_ctx *ctx, const uint32_t *d_parent, uint64_t R, uint32_t k,
+                     const uint32_t *cand, uint32_t *d_slots, uint32_t *d_bits, uint64_t *rows,
+                     void *stream);
+
+/* Level-k write pass: rows parent_row ++ [q] into d_table (k * rows u32).
+ * Replaces: calcInstanceTableGeneral (DeviceProcess.h:9, cu:94-105). */
+int colo_joink_write(colo_ctx *ctx, const uint32_t *d_parent, uint64_t R, uint32_t k,
+                     const uint32_t *cand, const uint32_t *d_slots, const uint64_t *d_offsets,
+                     uint32_t *d_table, void *stream);
+
+/* Participation-index numerators and PI of one candidate from its bitmap
+ * (synchronises).  num_compat: the reference's window sums (cu:113-124,
+ * off by one); num_exact: participating instances per feature; pi_*:
+ * min(1, min_j fl32(num_j) / fl32(N_j)).  Any output may be NULL.
+ * Replaces: getParticipationIndex (DeviceProcess.h:12, cu:108-132). */
+int colo_pi(colo_ctx *ctx, const uint32_t *d_bits, uint32_t k, const uint32_t *cand,
+            uint64_t *num_compat, uint64_t *num_exact, float *pi_compat, float *pi_exact,
+            void *stream);
+
+/* Expands a bitmap into d_flags (sum_j N_{p_j} bytes, feature segments in
+ * candidate order, instances in input order).  Parity hook. */
+int colo_participation(colo_ctx *ctx, const uint32_t *d_bits, uint32_t k, const uint32_t *cand,
+                       uint8_t *d_flags, void *stream);
+
+/* ------------------------------------------------------------- miner */
+
+/* Schedules of the per-level pipeline. */
+#define COLO_SCHED_PLAIN 0      /* one count launch + one PI sync per candidate */
+#define COLO_SCHED_PERSISTENT 1 /* one persistent launch per level phase       */
+#define COLO_SCHED_LEARNED 2    /* persistent, queue ordered/pre-sized by MLP  */
+
+typedef struct colo_mine_opts {
+    float theta;            /* prevalence threshold (parameter.txt line 2)   */
+    uint32_t max_k;         /* largest pattern size, 0 = no cap               */
+    int32_t pi_mode;        /* 0 = reference numerators, 1 = exact            */
+    int32_t filter_on;      /* 1 = cell-aggregate filter (reference default)  */
+    int32_t sched;          /* COLO_SCHED_*                                   */
+    int32_t hash_tables;    /* 1 = report a position-sensitive table checksum */
+    const char *out_prefix; /* ColocationDegree_<k>.txt prefix, NULL = none   */
+} colo_mine_opts;
+
+/* Per-run measurements (milliseconds unless stated). */
+typedef struct colo_mine_stats {
+    double filter_ms, count_ms, pi_ms, scan_ms, write_ms, host_ms, total_ms;
+    double l2_join_ms;      /* level-2 count + scan + write                 */
+    uint64_t l2_rows;       /* rows emitted by the level-2 tables            */
+    uint64_t l2_pairs;      /* neighbour pairs counted at level 2            */
+    uint64_t levels, patterns, candidates, launches;
+} colo_mine_stats;
+
+/* Runs the Apriori level loop (colocationFinder::Begin cpp:1104-1137 with
+ * degree2Processing cpp:256-425, generatePrevalentPatternsGeneral
+ * cpp:736-854, generateInstanceTableGeneral cpp:856-909) on the binned
+ * context; writes <out_prefix>ColocationDegree_<k>.txt (savetoFile2 /
+ * savetoFileGen format, cpp:960-997) and returns a report string
+ * (free with colo_free_str). */
+int colo_mine(colo_ctx *ctx, const colo_mine_opts *opts, char **report, colo_mine_stats *stats);
+void colo_free_str(char *s);
+
+/* End-to-end drop-in for colocationFinder::Begin(argc, argv): reads
+ * config.txt (cpp:1077-1093), parameter.txt (cpp:57-74) and the dataset CSV
+ * (cpp:139-254), mines, and writes the pattern files under the config's
+ * location line.  opts may be NULL (reference defaults); its theta is
+ * ignored (parameter.txt holds it).  timing_line receives the stdout line
+ * of cpp:1144 if non-NULL (free with colo_free_str). */
+int colo_run_config(const char *config_path, const colo_mine_opts *opts, char **timing_line);
+
+/* Next-level candidates from the prevalent list L (P patterns of size km1,
+ * in list order) into out (cap patterns of size km1+1); returns the number
+ * of candidates (call with cap = 0 to size out).  Host only.
+ * Replaces: colocationFinder::candidateColocationGeneral (cpp:517-557) with
+ * generateandCheckSubsets / subsetGen / checkSubset (cpp:598-701),
+ * including its stop at the first candidate with a missing subset. */
+uint64_t colo_candidates(const uint32_t *L, uint64_t P, uint32_t km1, uint32_t *out, uint64_t cap);
+
+/* Parses a dataset CSV (cpp:139-254) into host arrays owned by the
+ * returned handle. */
+typedef struct colo_host_data colo_host_data;
+int colo_load_csv(const char *path, colo_host_data **out);
+int colo_host_dataset(const colo_host_data *h, colo_dataset *ds);
+const char *colo_host_label(const colo_host_data *h, uint32_t f);
+void colo_host_free(colo_host_data *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COLO_H */

@@ -1,0 +1,250 @@
+"""GPU parity: HIP kernels (through the C ABI) against the CPU oracle.
+
+Bit-exact for ids, counts, offsets, tables, numerators; PI compared as
+binary32 bit patterns.  Every case prints its name before touching the GPU.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _workload(datagen, name, N=None, seed=None, **over):
+    labels, x, y, S, p = datagen.generate(name, N=N, seed=seed, **over)
+    fcount = np.bincount(labels, minlength=p["F"]).astype(np.uint32)
+    bounds = np.array([0, 0, S, S], np.float32)
+    return x.astype(np.float32), y.astype(np.float32), fcount, bounds, p
+
+
+@pytest.fixture(scope="module")
+def c1(datagen, colo_mod, oracle_mod):
+    _say("fixture c1 N=20000")
+    x, y, fc, b, p = _workload(datagen, "c1")
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    return ctx, orc, p
+
+
+@pytest.fixture(scope="module")
+def c2small(datagen, colo_mod, oracle_mod):
+    _say("fixture c2 N=60000")
+    x, y, fc, b, p = _workload(datagen, "c2", N=60000)
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    return ctx, orc, p
+
+
+def _binning_expect(orc, fcount):
+    cid, _ = orc.binning()
+    n = len(cid)
+    fid = np.repeat(np.arange(len(fcount)), fcount)
+    order = np.lexsort((np.arange(n), cid, fid))  # (feature, cell, input index)
+    return cid, order, fid
+
+
+@pytest.mark.parametrize("fx", ["c1", "c2small"])
+def test_binning(fx, request):
+    ctx, orc, p = request.getfixturevalue(fx)
+    _say(f"binning {fx}")
+    g = ctx.grid()
+    gx, gy, cells, r2 = orc.grid()
+    assert (g["gx"], g["gy"], g["cells"]) == (gx, gy, cells)
+    assert np.float32(g["r2"]).tobytes() == np.float32(r2).tobytes()
+    cell, sidx, d = ctx.binning()
+    cid, order, fid = _binning_expect(orc, ctx.fcount)
+    np.testing.assert_array_equal(cell, cid)
+    np.testing.assert_array_equal(sidx, order)
+    # directory: first sorted position of every (feature, cell)
+    key = fid.astype(np.int64) * (cells + 1) + cid
+    expect = np.searchsorted(np.sort(key), np.arange(ctx.F * (cells + 1)), side="left")
+    np.testing.assert_array_equal(d, expect.astype(np.uint32))
+
+
+@pytest.mark.parametrize("fx", ["c1", "c2small"])
+def test_filter_level2(fx, request):
+    ctx, orc, p = request.getfixturevalue(fx)
+    _say(f"filter level 2 {fx}")
+    F = ctx.F
+    cands = np.array([[i, j] for i in range(F) for j in range(i + 1, F)], np.uint32)
+    gc, gp = ctx.filter(cands, 2, p["theta"])
+    oc, op = orc.filter(cands, 2, p["theta"])
+    np.testing.assert_array_equal(gc, oc)
+    np.testing.assert_array_equal(gp, op)
+
+
+def test_filter_level3(c2small):
+    ctx, orc, p = c2small
+    _say("filter level 3 c2small")
+    F = ctx.F
+    rng = np.random.default_rng(7)
+    cands = np.sort(np.array([rng.choice(F, 3, replace=False) for _ in range(200)]), axis=1).astype(np.uint32)
+    gc, gp = ctx.filter(cands, 3, p["theta"])
+    oc, op = orc.filter(cands, 3, p["theta"])
+    np.testing.assert_array_equal(gc, oc)
+    np.testing.assert_array_equal(gp, op)
+
+
+def _pairs(F, n, seed):
+    rng = np.random.default_rng(seed)
+    allp = [(i, j) for i in range(F) for j in range(i + 1, F)]
+    idx = rng.choice(len(allp), size=min(n, len(allp)), replace=False)
+    return [allp[i] for i in idx]
+
+
+@pytest.mark.parametrize("fx", ["c1", "c2small"])
+def test_join2(fx, request):
+    ctx, orc, p = request.getfixturevalue(fx)
+    for a, b in _pairs(ctx.F, 12, 3):
+        _say(f"join2 {fx} ({a},{b})")
+        rows, slots, bits = ctx.join2_count(a, b)
+        orows, oslots, obm, oex = orc.join2_count(a, b)
+        assert rows == orows
+        Na, Nb = int(ctx.fcount[a]), int(ctx.fcount[b])
+        np.testing.assert_array_equal(ctx._np_u32(slots, Na), oslots)
+        part = ctx.participation(bits, [a, b])
+        np.testing.assert_array_equal(part, oex[: Na + Nb].astype(np.uint8))
+        nc, ne, pc, pe = ctx.pi(bits, [a, b])
+        opc, onc = orc.pi_compat(obm, [a, b])
+        ope, one = orc.pi_exact(oex, [a, b])
+        np.testing.assert_array_equal(nc, onc)
+        np.testing.assert_array_equal(ne, one)
+        assert np.float32(pc).tobytes() == np.float32(opc).tobytes()
+        assert np.float32(pe).tobytes() == np.float32(ope).tobytes()
+        table, tot = ctx.join2_write(a, b, slots)
+        assert tot == rows
+        np.testing.assert_array_equal(ctx._np_u32(table, 2 * tot), orc.join2_write(a, b, oslots))
+
+
+def test_joink(c2small, colo_mod):
+    ctx, orc, p = c2small
+    torch = ctx.torch
+    F = ctx.F
+    # grow level-3 and level-4 candidates from the densest pairs
+    fc = ctx.fcount
+    top = [int(v) for v in np.argsort(-fc)[:6]]
+    top.sort()
+    trip = [(top[0], top[1], top[2]), (top[1], top[3], top[4]), (top[0], top[2], top[5])]
+    for cand in trip:
+        a, b, cc = cand
+        _say(f"joink k=3 {cand}")
+        rows, slots, _ = ctx.join2_count(a, b)
+        par_t, R = ctx.join2_write(a, b, slots)
+        _, oslots, _, _ = orc.join2_count(a, b)
+        opar = orc.join2_write(a, b, oslots)
+        rows3, slots3, bits3 = ctx.joink_count(par_t, R, 3, list(cand))
+        orows3, oslots3, obm3 = orc.joink_count(opar, 3, list(cand))
+        assert rows3 == orows3
+        np.testing.assert_array_equal(ctx._np_u32(slots3, R), oslots3)
+        part = ctx.participation(bits3, list(cand))
+        tot = int(sum(int(fc[f]) for f in cand))
+        np.testing.assert_array_equal(part, obm3[:tot].astype(np.uint8))
+        nc, ne, pc, pe = ctx.pi(bits3, list(cand))
+        opc, onc = orc.pi_compat(obm3, list(cand))
+        ope, one = orc.pi_exact(obm3, list(cand))
+        np.testing.assert_array_equal(nc, onc)
+        np.testing.assert_array_equal(ne, one)
+        assert np.float32(pc).tobytes() == np.float32(opc).tobytes()
+        t3, tot3 = ctx.joink_write(par_t, R, 3, list(cand), slots3)
+        ot3 = orc.joink_write(opar, 3, list(cand), oslots3)
+        np.testing.assert_array_equal(ctx._np_u32(t3, 3 * tot3), ot3)
+        if tot3 == 0:
+            continue
+        # one more level on top
+        for d in range(F):
+            if d > int(cand[2]):
+                c4 = list(cand) + [d]
+                _say(f"joink k=4 {c4}")
+                r4, s4, b4 = ctx.joink_count(t3, tot3, 4, c4)
+                or4, os4, ob4 = orc.joink_count(ot3, 4, c4)
+                assert r4 == or4
+                np.testing.assert_array_equal(ctx._np_u32(s4, tot3), os4)
+                nc, ne, pc, pe = ctx.pi(b4, c4)
+                opc, onc = orc.pi_compat(ob4, c4)
+                np.testing.assert_array_equal(nc, onc)
+                t4, tot4 = ctx.joink_write(t3, tot3, 4, c4, s4)
+                np.testing.assert_array_equal(ctx._np_u32(t4, 4 * tot4), orc.joink_write(ot3, 4, c4, os4))
+                break
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+@pytest.mark.parametrize("fx", ["c1", "c2small"])
+def test_mine_report(fx, sched, request):
+    ctx, orc, p = request.getfixturevalue(fx)
+    _say(f"mine {fx} sched={sched}")
+    rep, st = ctx.mine(p["theta"], max_k=p["max_k"], sched=sched)
+    ref = orc.mine(p["r"], p["theta"], max_k=p["max_k"])
+    assert rep == ref
+    assert st["patterns"] > 0
+
+
+@pytest.mark.parametrize("pi_mode", [0, 1])
+def test_mine_modes(c2small, pi_mode):
+    ctx, orc, p = c2small
+    _say(f"mine c2small pi_mode={pi_mode} no-filter")
+    rep, _ = ctx.mine(0.15, max_k=4, pi_mode=pi_mode, filter_on=False)
+    ref = orc.mine(p["r"], 0.15, max_k=4, pi_mode=pi_mode, filter_on=False)
+    assert rep == ref
+
+
+def test_pattern_files(c1, tmp_path):
+    ctx, orc, p = c1
+    _say("pattern files c1")
+    g = str(tmp_path) + "/gpu_"
+    o = str(tmp_path) + "/orc_"
+    ctx.mine(p["theta"], max_k=p["max_k"], out_prefix=g)
+    orc.mine(p["r"], p["theta"], max_k=p["max_k"], out_prefix=o)
+    for k in (2, 3, 4):
+        fg, fo = g + f"ColocationDegree_{k}.txt", o + f"ColocationDegree_{k}.txt"
+        assert os.path.exists(fg) == os.path.exists(fo)
+        if os.path.exists(fo):
+            assert open(fg).read() == open(fo).read()
+
+
+@pytest.mark.parametrize("gx_case", ["gx1", "gx2", "gx3"])
+def test_narrow_grids(gx_case, colo_mod, oracle_mod):
+    """Grids one, two and three cells wide: the reference's neighbour-cell
+    wrap rules (U:35-53) drop every cell when gx == 1 and the side column
+    when gx == 2."""
+    _say(f"narrow grid {gx_case}")
+    rng = np.random.default_rng(11)
+    r = 10.0
+    W = {"gx1": 9.5, "gx2": 19.0, "gx3": 29.0}[gx_case]
+    F, n = 4, 800
+    x = rng.uniform(0, W, n).astype(np.float32)
+    y = rng.uniform(0, 200, n).astype(np.float32)
+    fc = np.full(F, n // F, np.uint32)
+    b = np.array([0, 0, W, 200], np.float32)
+    ctx = colo_mod.Context(x, y, fc, b, r)
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    rep, _ = ctx.mine(0.1, max_k=4)
+    assert rep == orc.mine(r, 0.1, max_k=4)
+
+
+def test_seattle_subset(colo_mod, oracle_mod):
+    """Real data: the reference's Seattle2012 dataset
+    (tests/golden/seattle2012.csv.gz), r = 10, theta = 0.3, sizes <= 3."""
+    import gzip
+    path = os.path.join(os.path.dirname(__file__), "golden", "seattle2012.csv.gz")
+    if not os.path.exists(path):
+        pytest.skip("fixture missing")
+    _say("seattle r=10 theta=0.3 max_k=3")
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        csv = os.path.join(d, "s.csv")
+        with gzip.open(path, "rb") as fi, open(csv, "wb") as fo:
+            fo.write(fi.read())
+        hd = colo_mod.HostData(csv)
+        orc = oracle_mod.Oracle.from_csv(csv)
+    ctx = colo_mod.Context.from_host(hd, 10.0)
+    rep, _ = ctx.mine(0.3, max_k=3)
+    assert rep == orc.mine(10.0, 0.3, max_k=3)

@@ -1,0 +1,87 @@
+"""CPU checks of the product library (no GPU calls): it loads, exports every
+entry point include/colo.h declares, and its host-side parts (CSV parser,
+candidate generation) agree with the oracle."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^[A-Za-z_][\w \*]*?\b(colo_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol(colo_mod):
+    L = colo_mod.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) <= set(colo_mod.SYMBOLS) | {"colo_candidates"}
+
+
+def test_csv_parser_matches_oracle(colo_mod, oracle_mod, datagen, tmp_path):
+    w = datagen.write_workload(str(tmp_path), "c1", N=3000)
+    hd = colo_mod.HostData(w["csv"])
+    o = oracle_mod.Oracle.from_csv(w["csv"])
+    x, y = o.points()
+    assert hd.n == o.n and hd.F == o.F
+    assert hd.x.view(np.uint32).tolist() == x.view(np.uint32).tolist()
+    assert hd.y.view(np.uint32).tolist() == y.view(np.uint32).tolist()
+    assert hd.fcount.tolist() == o.fcount.tolist()
+    assert hd.bounds.tolist() == o.bounds().tolist()
+    assert hd.labels == [o.label(f) for f in range(o.F)]
+
+
+def test_csv_parser_seattle(colo_mod, oracle_mod, tmp_path):
+    import gzip
+    src = os.path.join(ROOT, "tests", "golden", "seattle2012.csv.gz")
+    csv = tmp_path / "s.csv"
+    with gzip.open(src, "rb") as fi:
+        csv.write_bytes(fi.read())
+    hd = colo_mod.HostData(str(csv))
+    o = oracle_mod.Oracle.from_csv(str(csv))
+    x, y = o.points()
+    assert hd.n == 165458 and hd.F == 13
+    assert hd.x.view(np.uint32).tolist() == x.view(np.uint32).tolist()
+    assert hd.y.view(np.uint32).tolist() == y.view(np.uint32).tolist()
+    assert hd.fcount.tolist() == o.fcount.tolist()
+
+
+@pytest.mark.parametrize("body", ["2\n0,1,2\n0,3,4\n0,0,5,5", "3\n0,1,2\n0,3,4\n\n0,0,5,5",
+                                  "2\n0,1,2\nx,3,4\n\n0,0,5,5"])
+def test_csv_parser_rejects(colo_mod, tmp_path, body):
+    p = tmp_path / "bad.csv"
+    p.write_text(body)
+    with pytest.raises(colo_mod.ColoError):
+        colo_mod.HostData(str(p))
+
+
+def test_candidates_match_oracle(colo_mod, oracle_mod):
+    rng = np.random.default_rng(4)
+    for km1 in (2, 3, 4):
+        for trial in range(30):
+            F = int(rng.integers(km1 + 1, 12))
+            allp = [tuple(sorted(rng.choice(F, km1, replace=False).tolist())) for _ in range(60)]
+            L = np.array(sorted(set(allp)), np.uint32)
+            np.testing.assert_array_equal(colo_mod.candidates(L, km1), oracle_mod.candgen(L, km1))
+    L = np.array([[0, 1], [0, 2], [0, 3], [1, 3], [2, 3]], np.uint32)
+    assert colo_mod.candidates(L, 2).tolist() == [[0, 2, 3]]
+
+
+def test_no_gpu_fails_loudly(colo_mod):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(colo_mod.ColoError):
+        colo_mod.Context(np.zeros(4, np.float32), np.zeros(4, np.float32), [2, 2], [0, 0, 1, 1], 0.5)

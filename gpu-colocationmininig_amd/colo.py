@@ -48,7 +48,12 @@ class MineOpts(C.Structure):
 class MineStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("filter_ms", "count_ms", "pi_ms", "scan_ms", "write_ms", "host_ms",
                                           "total_ms", "l2_join_ms")] + \
-               [(n, C.c_uint64) for n in ("l2_rows", "l2_pairs", "levels", "patterns", "candidates", "launches")]
+               [(n, C.c_uint64) for n in ("l2_rows", "l2_pairs", "levels", "patterns", "candidates", "launches")] + \
+               [(n, C.c_double) for n in ("count_kernel_ms", "write_kernel_ms", "filter_kernel_ms", "pi_kernel_ms",
+                                          "count_bytes", "write_bytes")] + \
+               [(n, C.c_uint64) for n in ("count_launches", "write_launches")] + \
+               [(n, C.c_double) for n in ("l2_count_kernel_ms", "l2_count_bytes", "l2_write_kernel_ms",
+                                          "l2_write_bytes")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -57,7 +62,7 @@ class MineStats(C.Structure):
 _lib = None
 
 SYMBOLS = [
-    "colo_ctx_create", "colo_ctx_destroy", "colo_get_grid", "colo_last_error", "colo_binning_export",
+    "colo_ctx_create", "colo_ctx_destroy", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_pi", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
@@ -75,6 +80,7 @@ def lib():
         vp, u32, u64, f32, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_int
         L.colo_ctx_create.argtypes = [C.POINTER(Dataset), f32, vp, C.POINTER(vp)]
         L.colo_ctx_destroy.argtypes = [vp]
+        L.colo_bin.argtypes = [vp, f32, vp]
         L.colo_get_grid.argtypes = [vp, C.POINTER(Grid)]
         L.colo_last_error.restype = C.c_char_p
         L.colo_last_error.argtypes = [vp]
@@ -212,10 +218,18 @@ class Context:
     def _u32(self, n):
         return self.torch.zeros(max(int(n), 1), dtype=self.torch.int32, device="cuda")
 
-    @staticmethod
-    def _np_u32(t, n=None):
+    @_on_stream
+    def _np_u32(self, t, n=None):
+        """Device tensor -> host uint32 array, ordered after the context's
+        launches (the copy runs on the context stream)."""
         a = t.cpu().numpy().view(np.uint32)
         return a if n is None else a[:n]
+
+    @_on_stream
+    def rebin(self, r):
+        """setGrid + CountCellFeatures on the resident data (cpp:102-126)."""
+        self._check(lib().colo_bin(self.h, C.c_float(r), C.c_void_p(self.stream)))
+        self.r = r
 
     @_on_stream
     def grid(self):

@@ -921,7 +921,10 @@ static int bin_impl(colo_ctx *c, hipStream_t s) {
     uint32_t ntiles = (uint32_t)((n + kRadixTile - 1) / kRadixTile);
     uint32_t *hist = nullptr;
     if ((rc = alloc_dev(c, (void **)&hist, 4ull * 256 * (ntiles ? ntiles : 1)))) return rc;
+    // scan temporaries; also holds the F+1 per-feature run offsets, plus the
+    // run count in its last word
     uint64_t tmpw = std::max(scan_tmp_words((uint64_t)ntiles * 256), scan_tmp_words(std::max<uint64_t>(c->dir_len, n)));
+    tmpw = std::max<uint64_t>(tmpw, (uint64_t)c->F + 3);
     uint64_t *stmp = nullptr;
     if ((rc = alloc_dev(c, (void **)&stmp, 8 * tmpw))) return rc;
     DevGrid g = c->dev().g;
@@ -1016,9 +1019,45 @@ static int bin_impl(colo_ctx *c, hipStream_t s) {
     return COLO_OK;
 }
 
+// setGrid (cpp:102-115) for distance r, binary32 throughout; (re)allocates
+// the r-dependent buffers.
+static int set_grid(colo_ctx *c, float r) {
+    void *old[] = {c->d_dir, c->d_cellmask, c->d_qmask, c->d_run_cell};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    c->d_dir = nullptr; c->d_cellmask = nullptr; c->d_qmask = nullptr; c->d_run_cell = nullptr;
+    c->r = r;
+    volatile float r2 = r * r;
+    c->r2 = r2;
+    volatile float maxBy = c->maxY + r;
+    volatile float gsx = (c->maxX - c->minX) / r;
+    volatile float gsy = (maxBy - c->minY) / r;
+    float cx = ceilf(gsx), cy = ceilf(gsy);
+    if (!(r > 0.f) || !(cx >= 1.f) || !(cy >= 1.f) || cx > 4.0e9f || cy > 4.0e9f)
+        return set_error(c, COLO_EARG, "degenerate grid");
+    c->gx = (uint32_t)cx;
+    c->gy = (uint32_t)cy;
+    uint64_t cells = (uint64_t)c->gx * c->gy;
+    if (cells >= (1ull << 31)) return set_error(c, COLO_EUNSUPPORTED, "more than 2^31 cells");
+    c->cells = (uint32_t)cells;
+    c->cellbits = std::max<uint32_t>(1, bits_for(c->cells));
+    c->fidbits = bits_for(c->F ? c->F - 1 : 0);
+    c->dir_len = (uint64_t)c->F * (c->cells + 1ull);
+    return alloc_dev(c, (void **)&c->d_dir, 4 * c->dir_len);
+}
+
 extern "C" {
 
 const char *colo_last_error(const colo_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int colo_bin(colo_ctx *c, float r, void *stream) {
+    if (!c) return COLO_EARG;
+    hipStream_t s = pick_stream(c, stream);
+    int rc = set_grid(c, r);
+    if (!rc) rc = fill_u32(c->d_flag, 4, 0, s);
+    if (!rc) rc = (c->cellbits + c->fidbits <= 32) ? bin_impl<uint32_t>(c, s) : bin_impl<uint64_t>(c, s);
+    return rc;
+}
 
 int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **out) {
     if (!ds || !out) return COLO_EARG;
@@ -1050,29 +1089,7 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
     }
     if (!rc && tot != c->n) rc = set_error(c, COLO_EARG, "sum of fcount != n");
     c->minX = ds->minX; c->minY = ds->minY; c->maxX = ds->maxX; c->maxY = ds->maxY;
-    // setGrid (cpp:102-115), binary32 throughout
-    c->r = r;
-    {
-        volatile float r2 = r * r;
-        c->r2 = r2;
-        volatile float maxBy = ds->maxY + r;
-        volatile float gsx = (ds->maxX - ds->minX) / r;
-        volatile float gsy = (maxBy - ds->minY) / r;
-        float cx = ceilf(gsx), cy = ceilf(gsy);
-        if (!(r > 0.f) || !(cx >= 1.f) || !(cy >= 1.f) || cx > 4.0e9f || cy > 4.0e9f)
-            rc = rc ? rc : set_error(c, COLO_EARG, "degenerate grid");
-        else {
-            c->gx = (uint32_t)cx;
-            c->gy = (uint32_t)cy;
-            uint64_t cells = (uint64_t)c->gx * c->gy;
-            if (cells >= (1ull << 31)) rc = rc ? rc : set_error(c, COLO_EUNSUPPORTED, "more than 2^31 cells");
-            c->cells = (uint32_t)cells;
-        }
-    }
     if (!rc) {
-        c->cellbits = std::max<uint32_t>(1, bits_for(c->cells));
-        c->fidbits = bits_for(c->F ? c->F - 1 : 0);
-        c->dir_len = (uint64_t)c->F * (c->cells + 1ull);
         const uint64_t n = c->n;
         if (!rc) rc = alloc_dev(c, (void **)&c->d_x, 4 * n);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_y, 4 * n);
@@ -1082,7 +1099,6 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
         if (!rc) rc = alloc_dev(c, (void **)&c->d_sidx, 4 * n);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_spos, 4 * n);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_scell, 4 * n);
-        if (!rc) rc = alloc_dev(c, (void **)&c->d_dir, 4 * c->dir_len);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_fstart, 4ull * c->F);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_fcount, 4ull * c->F);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_flag, 16);
@@ -1099,8 +1115,7 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) rc = set_error(c, COLO_EHIP, hipGetErrorString(e));
         }
-        if (!rc) rc = fill_u32(c->d_flag, 4, 0, s);
-        if (!rc) rc = (c->cellbits + c->fidbits <= 32) ? bin_impl<uint32_t>(c, s) : bin_impl<uint64_t>(c, s);
+        if (!rc) rc = colo_bin(c, r, nullptr);
     }
     *out = c;  // returned even on error so colo_last_error can be read
     return rc;

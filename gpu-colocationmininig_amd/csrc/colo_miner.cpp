@@ -162,9 +162,66 @@ struct Miner {
     colo_ctx *ctx_for_err;
 
     Miner(colo_ctx *c_, const colo_mine_opts &o_) : c(c_), o(o_), s(c_->stream), ctx_for_err(c_) {}
+    ~Miner() {
+        for (auto &e : pool_) (void)hipEventDestroy(e);
+    }
+
+    // HIP-event timing of launches on the miner's stream: mark() records an
+    // event, span() pairs two marks with a category, harvest() (after a
+    // stream synchronisation) adds the elapsed times to the statistics.
+    enum Cat { kCount, kWrite, kFilter, kPi, kL2Count, kL2Write };
+    std::vector<hipEvent_t> pool_;
+    size_t used_ = 0;
+    struct Span { size_t a, b; int cat; };
+    std::vector<Span> spans_;
+    size_t mark() {
+        if (used_ == pool_.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return SIZE_MAX;
+            pool_.push_back(e);
+        }
+        (void)hipEventRecord(pool_[used_], s);
+        return used_++;
+    }
+    void span(size_t a, size_t b, int cat) {
+        if (a != SIZE_MAX && b != SIZE_MAX) spans_.push_back(Span{a, b, cat});
+    }
+    void harvest() {
+        for (const Span &sp : spans_) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, pool_[sp.a], pool_[sp.b]) != hipSuccess) continue;
+            switch (sp.cat) {
+                case kCount: st.count_kernel_ms += ms; break;
+                case kWrite: st.write_kernel_ms += ms; break;
+                case kFilter: st.filter_kernel_ms += ms; break;
+                case kPi: st.pi_kernel_ms += ms; break;
+                case kL2Count: st.l2_count_kernel_ms += ms; st.count_kernel_ms += ms; break;
+                case kL2Write: st.l2_write_kernel_ms += ms; st.write_kernel_ms += ms; break;
+            }
+        }
+        spans_.clear();
+        used_ = 0;
+    }
+
+    // Algorithmic bytes (each datum once): level 2 count reads both features'
+    // coordinates (8 B each), writes 4 B of slots per anchor and one bit per
+    // instance; level-k count reads the parent rows (4(k-1) B), the last
+    // feature's coordinates and writes slots and bits; writes add the
+    // offsets (8 B) and the table (4k B per row).
+    double count_bytes(uint32_t k, const uint32_t *cand, uint64_t n_anchor) const {
+        double Nsum = 0;
+        for (uint32_t j = 0; j < k; j++) Nsum += c->fcount[cand[j]];
+        if (k == 2) return 8.0 * Nsum + 4.0 * n_anchor + Nsum / 8.0;
+        return 4.0 * (k - 1) * n_anchor + 8.0 * c->fcount[cand[k - 1]] + 4.0 * n_anchor + Nsum / 8.0;
+    }
+    double write_bytes(uint32_t k, const uint32_t *cand, uint64_t n_anchor, uint64_t rows) const {
+        if (k == 2) return 8.0 * (c->fcount[cand[0]] + c->fcount[cand[1]]) + 12.0 * n_anchor + 8.0 * rows;
+        return 4.0 * (k - 1) * n_anchor + 12.0 * n_anchor + 8.0 * c->fcount[cand[k - 1]] + 4.0 * k * rows;
+    }
 
     int sync() {
         COLO_HIP(hipStreamSynchronize(s));
+        harvest();
         return COLO_OK;
     }
 
@@ -343,9 +400,18 @@ struct Miner {
         auto t0 = Clock::now();
         int rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
         if (rc) return rc;
+        size_t e0 = mark();
         rc = count_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), items.size(), k, slots_arena->as<uint32_t>(),
                          arena_bits.as<uint32_t>(), d_queue.as<uint32_t>(), s);
         if (rc) return rc;
+        span(e0, mark(), k == 2 ? kL2Count : kCount);
+        {
+            double by = 0;
+            for (uint64_t u = 0; u < m; u++) by += count_bytes(k, &C[idx[u] * k], desc[u].n_anchor);
+            st.count_bytes += by;
+            if (k == 2) st.l2_count_bytes += by;
+            st.count_launches++;
+        }
         st.launches += 2;
         // rows per candidate: one scan over the whole slots arena
         if (!arena_offs.ensure((slots_tot + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
@@ -373,8 +439,10 @@ struct Miner {
         if (!d_segs.ensure(segs.size() * sizeof(SegDesc)) || !d_segout.ensure(segs.size() * sizeof(SegOut)))
             return set_error(c, COLO_ENOMEM, "segments");
         COLO_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
+        size_t e1 = mark();
         rc = pi_batch_abs(c, arena_bits.as<uint32_t>(), d_segs.as<SegDesc>(), segs.size(), d_segout.as<SegOut>(), s);
         if (rc) return rc;
+        span(e1, mark(), kPi);
         std::vector<SegOut> so(segs.size());
         COLO_HIP(hipMemcpyAsync(so.data(), d_segout.p, so.size() * sizeof(SegOut), hipMemcpyDeviceToHost, s));
         if ((rc = sync())) return rc;
@@ -439,9 +507,23 @@ struct Miner {
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
         COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+        size_t e0 = mark();
         int rc = write_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), items.size(), k,
                              L[0].slots ? L[0].slots->as<uint32_t>() : nullptr, d_queue.as<uint32_t>(), s);
         if (rc) return rc;
+        span(e0, mark(), k == 2 ? kL2Write : kWrite);
+        {
+            double by = 0;
+            for (const CandDesc &d : desc) {
+                uint64_t rows = 0;
+                for (const Prev &p : L)
+                    if (p.table && p.table->p == (void *)d.table) rows = p.rows;
+                by += write_bytes(k, d.f, d.n_anchor, rows);
+            }
+            st.write_bytes += by;
+            if (k == 2) st.l2_write_bytes += by;
+            st.write_launches++;
+        }
         if ((rc = sync())) return rc;
         st.write_ms += ms_since(t0);
         st.launches += 2;
@@ -478,9 +560,11 @@ struct Miner {
                 if (!d_cands.ensure(C.size() * 4) || !d_counts.ensure(C.size() * 4) || !d_pass.ensure(nC))
                     return set_error(c, COLO_ENOMEM, "filter buffers");
                 COLO_HIP(hipMemcpyAsync(d_cands.p, C.data(), C.size() * 4, hipMemcpyHostToDevice, s));
+                size_t e0 = mark();
                 int rc = filter_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
                                       d_pass.as<uint8_t>(), s);
                 if (rc) return rc;
+                span(e0, mark(), kFilter);
                 COLO_HIP(hipMemcpyAsync(fcnt.data(), d_counts.p, fcnt.size() * 4, hipMemcpyDeviceToHost, s));
                 COLO_HIP(hipMemcpyAsync(pass.data(), d_pass.p, nC, hipMemcpyDeviceToHost, s));
                 if ((rc = sync())) return rc;

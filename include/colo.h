@@ -77,6 +77,11 @@ typedef struct colo_grid {
  * colocationFinder::setGrid (cpp:102-126). */
 int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **out);
 int colo_ctx_destroy(colo_ctx *ctx);
+/* Re-grids and re-bins the resident dataset for distance r (the same work
+ * colo_ctx_create does after its upload).
+ * Replaces: colocationFinder::setGrid (cpp:102-126) -> CountCellFeatures
+ * (DeviceProcess.h:33-34, cu:312-369). */
+int colo_bin(colo_ctx *ctx, float r, void *stream);
 int colo_get_grid(const colo_ctx *ctx, colo_grid *out);
 /* Human-readable message for the last error on this context. */
 const char *colo_last_error(const colo_ctx *ctx);
@@ -166,13 +171,21 @@ typedef struct colo_mine_opts {
     const char *out_prefix; /* ColocationDegree_<k>.txt prefix, NULL = none   */
 } colo_mine_opts;
 
-/* Per-run measurements (milliseconds unless stated). */
+/* Per-run measurements (milliseconds unless stated).  The *_kernel_ms
+ * fields are HIP-event times of the batched launches on the context's
+ * stream (persistent and learned schedules); *_bytes are the algorithmic
+ * bytes those launches must move (each input and output datum once). */
 typedef struct colo_mine_stats {
     double filter_ms, count_ms, pi_ms, scan_ms, write_ms, host_ms, total_ms;
     double l2_join_ms;      /* level-2 count + scan + write                 */
     uint64_t l2_rows;       /* rows emitted by the level-2 tables            */
     uint64_t l2_pairs;      /* neighbour pairs counted at level 2            */
     uint64_t levels, patterns, candidates, launches;
+    double count_kernel_ms, write_kernel_ms, filter_kernel_ms, pi_kernel_ms;
+    double count_bytes, write_bytes;
+    uint64_t count_launches, write_launches;
+    double l2_count_kernel_ms, l2_count_bytes;
+    double l2_write_kernel_ms, l2_write_bytes;
 } colo_mine_stats;
 
 /* Runs the Apriori level loop (colocationFinder::Begin cpp:1104-1137 with

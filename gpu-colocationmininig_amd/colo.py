@@ -64,7 +64,7 @@ _lib = None
 SYMBOLS = [
     "colo_ctx_create", "colo_ctx_destroy", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
-    "colo_joink_count", "colo_joink_write", "colo_pi", "colo_participation", "colo_mine", "colo_free_str",
+    "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
     "colo_candidates",
 ]
@@ -90,9 +90,11 @@ def lib():
         L.colo_bits_words.argtypes = [vp, u32, vp]
         L.colo_join2_count.argtypes = [vp, u32, u32, vp, vp, C.POINTER(u64), vp]
         L.colo_scan_slots.argtypes = [vp, vp, u64, vp, C.POINTER(u64), vp]
-        L.colo_join2_write.argtypes = [vp, u32, u32, vp, vp, vp, vp]
+        L.colo_join2_write.argtypes = [vp, u32, u32, vp, vp, vp, vp, vp]
         L.colo_joink_count.argtypes = [vp, vp, u64, u32, vp, vp, vp, C.POINTER(u64), vp]
         L.colo_joink_write.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp, vp]
+        L.colo_joink_count_nbr.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp, vp, C.POINTER(u64), vp]
+        L.colo_joink_write_nbr.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp, vp, vp, vp]
         L.colo_pi.argtypes = [vp, vp, u32, vp, vp, vp, C.POINTER(f32), C.POINTER(f32), vp]
         L.colo_participation.argtypes = [vp, vp, u32, vp, vp, vp]
         L.colo_mine.argtypes = [vp, C.POINTER(MineOpts), C.POINTER(vp), C.POINTER(MineStats)]
@@ -276,18 +278,24 @@ class Context:
 
     @_on_stream
     def scan(self, slots, n):
-        offs = self.torch.zeros(max(int(n), 1), dtype=self.torch.int64, device="cuda")
+        """Exclusive scan of n slot counts: n + 1 offsets (the sum last)."""
+        offs = self.torch.zeros(int(n) + 1, dtype=self.torch.int64, device="cuda")
         tot = C.c_uint64()
         self._check(lib().colo_scan_slots(self.h, _dptr(slots), n, _dptr(offs), C.byref(tot), C.c_void_p(self.stream)))
         return offs, tot.value
 
     @_on_stream
-    def join2_write(self, a, b, slots):
+    def join2_write(self, a, b, slots, lists=False):
+        """Level-2 table (and, with lists=True, the neighbour lists as sorted
+        positions plus the n+1 row offsets)."""
         Na = int(self.fcount[a])
         offs, tot = self.scan(slots, Na)
         table = self._u32(2 * tot)
+        nbr = self._u32(tot) if lists else None
         self._check(lib().colo_join2_write(self.h, a, b, _dptr(slots), _dptr(offs), _dptr(table),
-                                           C.c_void_p(self.stream)))
+                                           _dptr(nbr) if lists else None, C.c_void_p(self.stream)))
+        if lists:
+            return table, tot, nbr, offs
         return table, tot
 
     @_on_stream
@@ -307,6 +315,30 @@ class Context:
         table = self._u32(k * tot)
         self._check(lib().colo_joink_write(self.h, _dptr(parent), R, k, cand.ctypes.data, _dptr(slots), _dptr(offs),
                                            _dptr(table), C.c_void_p(self.stream)))
+        return table, tot
+
+    @_on_stream
+    def joink_count_nbr(self, parent, R, k, cand, table2, offs2):
+        """Level-k count with candidates from the anchors' neighbour lists
+        (table2 / offs2: neighbour lists of (cand[0], cand[-1]) from
+        join2_write(lists=True) and their n+1 row offsets)."""
+        cand = _u32_host(cand)
+        slots = self._u32(R)
+        bits = self._u32(self.bits_words(cand))
+        rows = C.c_uint64()
+        self._check(lib().colo_joink_count_nbr(self.h, _dptr(parent), R, k, cand.ctypes.data, _dptr(table2),
+                                               _dptr(offs2), _dptr(slots), _dptr(bits), C.byref(rows),
+                                               C.c_void_p(self.stream)))
+        return rows.value, slots, bits
+
+    @_on_stream
+    def joink_write_nbr(self, parent, R, k, cand, slots, table2, offs2):
+        cand = _u32_host(cand)
+        offs, tot = self.scan(slots, R)
+        table = self._u32(k * tot)
+        self._check(lib().colo_joink_write_nbr(self.h, _dptr(parent), R, k, cand.ctypes.data, _dptr(table2),
+                                               _dptr(offs2), _dptr(slots), _dptr(offs), _dptr(table),
+                                               C.c_void_p(self.stream)))
         return table, tot
 
     @_on_stream

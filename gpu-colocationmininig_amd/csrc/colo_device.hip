@@ -632,19 +632,21 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
 // Level-2 write for the anchor of input rank t (degree2TableGenerator_Kernel,
 // K:56-100): rows (g, q).
 __device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t b, uint64_t t, uint32_t nrows,
-                                          uint64_t off, uint32_t *table) {
+                                          uint64_t off, uint32_t *table, uint32_t *nbr) {
     const uint32_t g = D.fstart[a] + (uint32_t)t;
     const float xa = D.x[g], ya = D.y[g];
     const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
     RowSpan rs;
     int m = row_spans(D.g, dirb, D.cell_of[g], rs);
     uint32_t *out = table + off * 2;
+    uint32_t *nb = nbr ? nbr + off : nullptr;
     uint32_t w = 0;
     for (int r = 0; r < m && w < nrows; r++) {
         for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
             if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
                 out[2 * w] = g;
                 out[2 * w + 1] = D.sidx[q];
+                if (nb) nb[w] = q;
                 w++;
             }
         }
@@ -681,6 +683,69 @@ __device__ inline void joink_write_row(const DevData &D, const uint32_t *row, ui
     }
 }
 
+// Level-k passes over the anchor's neighbour list in the last feature.
+// The reference accepts q for a parent row when q lies in the anchor's 3x3
+// cells and d2(row[m], q) <= r2 for every member (K:268-290).  For m = 0
+// (the anchor) the first two conditions define exactly the rows of the
+// anchor in the level-2 table of (feature of row[0], last feature), written
+// in the same scan order (K:56-100); that pair is prevalent for every
+// level-k candidate (all 2-subsets of a candidate are prevalent), so its
+// table is kept.  Only members 1..k-2 remain to be tested.
+__device__ inline void joink_count_row_nbr(const DevData &D, const uint32_t *row, uint32_t km1, const uint32_t *feat,
+                                           uint32_t fl, uint32_t *slot, uint32_t *bits, const uint64_t *seg_off,
+                                           const uint32_t *nbr, const uint64_t *nb_off, uint64_t nb_base) {
+    float mx[kMaxK], my[kMaxK];
+    for (uint32_t m = 1; m < km1; m++) {
+        mx[m] = D.x[row[m]];
+        my[m] = D.y[row[m]];
+    }
+    const uint32_t t = row[0] - D.fstart[feat[0]];
+    const uint64_t b0 = nb_off[t] - nb_base, b1 = nb_off[t + 1] - nb_base;
+    const uint32_t sl = D.fstart[fl];
+    uint32_t cnt = 0;
+    for (uint64_t i = b0; i < b1; i++) {
+        const uint32_t q = nbr[i];  // sorted position
+        const float qx = D.sx[q], qy = D.sy[q];
+        bool ok = true;
+        for (uint32_t m = 1; m < km1; m++)
+            if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+        if (ok) {
+            cnt++;
+            set_bit(bits + seg_off[km1], q - sl);
+        }
+    }
+    *slot = cnt;
+    if (cnt)
+        for (uint32_t m = 0; m < km1; m++) set_bit(bits + seg_off[m], row[m] - D.fstart[feat[m]]);
+}
+
+__device__ inline void joink_write_row_nbr(const DevData &D, const uint32_t *row, uint32_t km1, const uint32_t *feat,
+                                           uint32_t nrows, uint64_t off, uint32_t *table, const uint32_t *nbr,
+                                           const uint64_t *nb_off, uint64_t nb_base) {
+    float mx[kMaxK], my[kMaxK];
+    for (uint32_t m = 1; m < km1; m++) {
+        mx[m] = D.x[row[m]];
+        my[m] = D.y[row[m]];
+    }
+    const uint32_t t = row[0] - D.fstart[feat[0]];
+    const uint64_t b0 = nb_off[t] - nb_base, b1 = nb_off[t + 1] - nb_base;
+    const uint32_t k = km1 + 1;
+    uint32_t *out = table + off * k;
+    uint32_t w = 0;
+    for (uint64_t i = b0; i < b1 && w < nrows; i++) {
+        const uint32_t q = nbr[i];
+        const float qx = D.sx[q], qy = D.sy[q];
+        bool ok = true;
+        for (uint32_t m = 1; m < km1; m++)
+            if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+        if (ok) {
+            for (uint32_t m = 0; m < km1; m++) out[(uint64_t)w * k + m] = row[m];
+            out[(uint64_t)w * k + km1] = D.sidx[q];
+            w++;
+        }
+    }
+}
+
 // ---- per-candidate kernels (the plain schedule and the C ABI)
 
 __global__ void k_join2_count(DevData D, uint32_t a, uint32_t b, uint32_t *slots, uint32_t *bits_a,
@@ -692,12 +757,12 @@ __global__ void k_join2_count(DevData D, uint32_t a, uint32_t b, uint32_t *slots
 }
 
 __global__ void k_join2_write(DevData D, uint32_t a, uint32_t b, const uint32_t *slots, const uint64_t *offs,
-                              uint32_t *table) {
+                              uint32_t *table, uint32_t *nbr) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.fcount[a]) return;
     uint32_t nr = slots[t];
     if (!nr) return;
-    join2_write_anchor(D, a, b, t, nr, offs[t], table);
+    join2_write_anchor(D, a, b, t, nr, offs[t], table, nbr);
 }
 
 struct CandK {
@@ -722,6 +787,29 @@ __global__ void k_joink_write(DevData D, const uint32_t *parent, uint64_t R, Can
     joink_write_row(D, parent + i * ck.km1, ck.km1, ck.fl, nr, offs[i], table);
 }
 
+struct NbrList {
+    const uint32_t *table;
+    const uint64_t *off;
+    uint64_t base;
+};
+
+__global__ void k_joink_count_nbr(DevData D, const uint32_t *parent, uint64_t R, CandK ck, NbrList nb,
+                                  uint32_t *slots, uint32_t *bits) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    joink_count_row_nbr(D, parent + i * ck.km1, ck.km1, ck.feat, ck.fl, slots + i, bits, ck.seg_off, nb.table,
+                        nb.off, nb.base);
+}
+
+__global__ void k_joink_write_nbr(DevData D, const uint32_t *parent, uint64_t R, CandK ck, NbrList nb,
+                                  const uint32_t *slots, const uint64_t *offs, uint32_t *table) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    uint32_t nr = slots[i];
+    if (!nr) return;
+    joink_write_row_nbr(D, parent + i * ck.km1, ck.km1, ck.feat, nr, offs[i], table, nb.table, nb.off, nb.base);
+}
+
 // ---- batched (persistent) kernels: a fixed grid drains a work queue of
 // (candidate, anchor range) items in queue order.
 
@@ -734,6 +822,46 @@ __device__ inline bool next_item(uint32_t *queue, uint64_t n_items, uint64_t &it
     item = s_item;
     __syncthreads();
     return item < n_items;
+}
+
+// Level-k count over the rows of one parent table for ncand candidates that
+// extend it: the row and its members' coordinates are loaded once.
+__device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
+                                             uint32_t *slots, uint32_t *bits) {
+    const CandDesc &c0 = descs[0];
+    const uint32_t km1 = c0.k - 1;
+    const uint32_t *row = c0.parent + i * km1;
+    uint32_t ids[kMaxK];
+    float mx[kMaxK], my[kMaxK];
+    for (uint32_t m = 0; m < km1; m++) {
+        ids[m] = row[m];
+        if (m) {
+            mx[m] = D.x[ids[m]];
+            my[m] = D.y[ids[m]];
+        }
+    }
+    const uint32_t t = ids[0] - D.fstart[c0.f[0]];
+    for (uint32_t j = 0; j < ncand; j++) {
+        const CandDesc &cd = descs[j];
+        const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
+        const uint32_t sl = D.fstart[cd.f[km1]];
+        uint32_t *bt = bits + cd.bits_off;
+        uint32_t cnt = 0;
+        for (uint64_t e = b0; e < b1; e++) {
+            const uint32_t q = cd.nb_table[e];
+            const float qx = D.sx[q], qy = D.sy[q];
+            bool ok = true;
+            for (uint32_t m = 1; m < km1; m++)
+                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+            if (ok) {
+                cnt++;
+                set_bit(bt + cd.seg_off[km1], q - sl);
+            }
+        }
+        slots[cd.slots_off + i] = cnt;
+        if (cnt)
+            for (uint32_t m = 0; m < km1; m++) set_bit(bt + cd.seg_off[m], ids[m] - D.fstart[cd.f[m]]);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_count_batch(DevData D, const CandDesc *desc, const WorkItem *items,
@@ -749,10 +877,54 @@ __global__ __launch_bounds__(kBlock) void k_count_batch(DevData D, const CandDes
             const uint32_t a = cd.f[0], b = cd.f[1];
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                 join2_count_anchor(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+        } else if (cd.nb_table) {
+            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+                joink_count_row_multi(D, desc + wi.cand, wi.ncand, i, slots, bits);
         } else {
             const uint32_t km1 = cd.k - 1;
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                 joink_count_row(D, cd.parent + i * km1, km1, cd.f, cd.f[km1], sl + i, bt, cd.seg_off);
+        }
+    }
+}
+
+// Level-k write over the rows of one parent table for ncand tables.
+__device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
+                                             const uint32_t *slots) {
+    const CandDesc &c0 = descs[0];
+    const uint32_t km1 = c0.k - 1, k = c0.k;
+    const uint32_t *row = c0.parent + i * km1;
+    uint32_t ids[kMaxK];
+    float mx[kMaxK], my[kMaxK];
+    bool any = false;
+    for (uint32_t j = 0; j < ncand; j++) any |= slots[descs[j].slots_off + i] != 0;
+    if (!any) return;
+    for (uint32_t m = 0; m < km1; m++) {
+        ids[m] = row[m];
+        if (m) {
+            mx[m] = D.x[ids[m]];
+            my[m] = D.y[ids[m]];
+        }
+    }
+    const uint32_t t = ids[0] - D.fstart[c0.f[0]];
+    for (uint32_t j = 0; j < ncand; j++) {
+        const CandDesc &cd = descs[j];
+        const uint32_t nrows = slots[cd.slots_off + i];
+        if (!nrows) continue;
+        const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
+        uint32_t *out = cd.table + (cd.offsets[i] - cd.off_base) * k;
+        uint32_t w = 0;
+        for (uint64_t e = b0; e < b1 && w < nrows; e++) {
+            const uint32_t q = cd.nb_table[e];
+            const float qx = D.sx[q], qy = D.sy[q];
+            bool ok = true;
+            for (uint32_t m = 1; m < km1; m++)
+                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+            if (ok) {
+                for (uint32_t m = 0; m < km1; m++) out[(uint64_t)w * k + m] = ids[m];
+                out[(uint64_t)w * k + km1] = D.sidx[q];
+                w++;
+            }
         }
     }
 }
@@ -767,8 +939,11 @@ __global__ __launch_bounds__(kBlock) void k_write_batch(DevData D, const CandDes
         if (cd.k == 2) {
             for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
                 uint32_t nr = sl[t];
-                if (nr) join2_write_anchor(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table);
+                if (nr) join2_write_anchor(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
             }
+        } else if (cd.nb_table) {
+            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+                joink_write_row_multi(D, desc + wi.cand, wi.ncand, i, slots);
         } else {
             const uint32_t km1 = cd.k - 1;
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock) {
@@ -1212,15 +1387,15 @@ int colo_scan_slots(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *
     if (!c) return COLO_EARG;
     colo_ctx *ctx_for_err = c;
     hipStream_t s = pick_stream(c, stream);
-    if (!n) {
-        if (total) *total = 0;
-        return COLO_OK;
-    }
     void *tmp;
     int rc = scratch(c, 8 * scan_tmp_words(n) + 16, &tmp);
     if (rc) return rc;
-    uint64_t *tot = (uint64_t *)tmp;
-    scan_rec<uint32_t, uint64_t>(d_slots, n, d_offsets, tot, tot + 1, s);
+    uint64_t *tot = d_offsets + n;  // offsets has n + 1 entries: the sum lands at [n]
+    if (!n) {
+        if ((rc = fill_u32((uint32_t *)tot, 2, 0, s))) return rc;
+    } else {
+        scan_rec<uint32_t, uint64_t>(d_slots, n, d_offsets, tot, (uint64_t *)tmp, s);
+    }
     COLO_HIP(hipGetLastError());
     if (total) {
         COLO_HIP(hipMemcpyAsync(c->h_pinned, tot, 8, hipMemcpyDeviceToHost, s));
@@ -1231,7 +1406,7 @@ int colo_scan_slots(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *
 }
 
 int colo_join2_write(colo_ctx *c, uint32_t a, uint32_t b, const uint32_t *d_slots, const uint64_t *d_offsets,
-                     uint32_t *d_table, void *stream) {
+                     uint32_t *d_table, uint32_t *d_nbr, void *stream) {
     if (!c) return COLO_EARG;
     colo_ctx *ctx_for_err = c;
     uint32_t cand[2] = {a, b};
@@ -1240,7 +1415,7 @@ int colo_join2_write(colo_ctx *c, uint32_t a, uint32_t b, const uint32_t *d_slot
     hipStream_t s = pick_stream(c, stream);
     uint32_t Na = c->fcount[a];
     hipLaunchKernelGGL(k_join2_write, dim3((Na + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->dev(), a, b, d_slots,
-                       d_offsets, d_table);
+                       d_offsets, d_table, d_nbr);
     COLO_HIP(hipGetLastError());
     return COLO_OK;
 }
@@ -1287,6 +1462,46 @@ int colo_joink_write(colo_ctx *c, const uint32_t *d_parent, uint64_t R, uint32_t
     if (R) {
         hipLaunchKernelGGL(k_joink_write, dim3((unsigned)((R + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, c->dev(),
                            d_parent, R, ck, d_slots, d_offsets, d_table);
+        COLO_HIP(hipGetLastError());
+    }
+    return COLO_OK;
+}
+
+int colo_joink_count_nbr(colo_ctx *c, const uint32_t *d_parent, uint64_t R, uint32_t k, const uint32_t *cand,
+                         const uint32_t *d_nbr2, const uint64_t *d_offsets2, uint32_t *d_slots, uint32_t *d_bits,
+                         uint64_t *rows, void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    int rc = check_cand(c, k, cand);
+    if (rc) return rc;
+    if (k < 3) return set_error(c, COLO_EARG, "colo_joink_count_nbr needs k >= 3");
+    hipStream_t s = pick_stream(c, stream);
+    CandK ck = make_candk(c, k, cand);
+    uint64_t words;
+    uint64_t so[kMaxK];
+    seg_layout(c, k, cand, 0, so, &words);
+    if ((rc = fill_u32(d_bits, words, 0, s))) return rc;
+    if (R) {
+        hipLaunchKernelGGL(k_joink_count_nbr, dim3((unsigned)((R + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           c->dev(), d_parent, R, ck, NbrList{d_nbr2, d_offsets2, 0}, d_slots, d_bits);
+        COLO_HIP(hipGetLastError());
+    }
+    return sum_slots(c, d_slots, R, rows, s);
+}
+
+int colo_joink_write_nbr(colo_ctx *c, const uint32_t *d_parent, uint64_t R, uint32_t k, const uint32_t *cand,
+                         const uint32_t *d_nbr2, const uint64_t *d_offsets2, const uint32_t *d_slots,
+                         const uint64_t *d_offsets, uint32_t *d_table, void *stream) {
+    if (!c) return COLO_EARG;
+    colo_ctx *ctx_for_err = c;
+    int rc = check_cand(c, k, cand);
+    if (rc) return rc;
+    if (k < 3) return set_error(c, COLO_EARG, "colo_joink_write_nbr needs k >= 3");
+    hipStream_t s = pick_stream(c, stream);
+    CandK ck = make_candk(c, k, cand);
+    if (R) {
+        hipLaunchKernelGGL(k_joink_write_nbr, dim3((unsigned)((R + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           c->dev(), d_parent, R, ck, NbrList{d_nbr2, d_offsets2, 0}, d_slots, d_offsets, d_table);
         COLO_HIP(hipGetLastError());
     }
     return COLO_OK;
@@ -1346,11 +1561,12 @@ int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uin
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "pi launch");
 }
 int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s) {
-    if (!n) return COLO_OK;
+    if (!n) return fill_u32((uint32_t *)d_offsets, 2, 0, s);
     void *tmp;
     int rc = scratch(c, 8 * scan_tmp_words(n) + 16, &tmp);
     if (rc) return rc;
-    scan_rec<uint32_t, uint64_t>(d_slots, n, d_offsets, (uint64_t *)tmp, (uint64_t *)tmp + 1, s);
+    // d_offsets has n + 1 entries; the sum lands at [n]
+    scan_rec<uint32_t, uint64_t>(d_slots, n, d_offsets, d_offsets + n, (uint64_t *)tmp, s);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "scan launch");
 }
 }  // namespace colo

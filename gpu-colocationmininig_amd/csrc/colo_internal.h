@@ -48,12 +48,23 @@ struct CandDesc {
     const uint64_t *offsets;       // write pass: row offsets (n_anchor), global
     uint64_t off_base;             // write pass: global offset of row 0
     uint32_t *table;               // write pass: output table
+    // level k: neighbour lists of the anchors in the last feature (sorted
+    // positions, scan order), i.e. the second column of the level-2 instance
+    // table of (f[0], f[k-1]), with its row offsets per anchor input rank
+    // (nb_off[t] - nb_base .. nb_off[t+1] - nb_base); nb_table == nullptr
+    // selects the 3x3 cell scan.
+    const uint32_t *nb_table;
+    const uint64_t *nb_off;
+    uint64_t nb_base;
+    uint32_t *nbr_out;             // level-2 write pass: neighbour-list output (optional)
 };
 
-// One work item of a batched launch: anchors [begin, end) of candidate cand.
+// One work item of a batched launch: anchors / parent rows [begin, end) of
+// candidates cand .. cand+ncand-1, which share their parent table (level k:
+// one pass over a parent row serves every candidate that extends it).
 struct WorkItem {
     uint32_t cand;
-    uint32_t pad;
+    uint32_t ncand;
     uint64_t begin, end;
 };
 

@@ -151,6 +151,10 @@ struct Prev {
     uint64_t rows = 0;
     int64_t parent = -1;            // index into the previous level's list
     std::shared_ptr<DevBuf> table;  // instance table, rows x k
+    std::shared_ptr<DevBuf> offs;   // row offsets of the table (nslots + 1 entries from offs_at)
+    std::shared_ptr<DevBuf> nbr;    // level 2: second column as sorted positions
+    uint64_t offs_at = 0;           // index of the first offset inside offs
+    uint64_t off_base = 0;          // value of the first offset
 };
 
 struct Miner {
@@ -285,8 +289,11 @@ struct Miner {
                 if (parent < 0 || !Lprev[parent].table) return set_error(c, COLO_EARG, "missing parent table");
                 nslots = Lprev[parent].rows;
                 if (!slots->ensure(nslots * 4)) return set_error(c, COLO_ENOMEM, "slots");
-                rc = colo_joink_count(c, Lprev[parent].table->as<uint32_t>(), nslots, k, cand,
-                                      slots->as<uint32_t>(), bits.as<uint32_t>(), &r.rows, s);
+                const Prev *nb = nbr_list(cand[0], cand[k - 1]);
+                if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
+                rc = colo_joink_count_nbr(c, Lprev[parent].table->as<uint32_t>(), nslots, k, cand,
+                                          nb->nbr->as<uint32_t>(), nb->offs->as<uint64_t>() + nb->offs_at,
+                                          slots->as<uint32_t>(), bits.as<uint32_t>(), &r.rows, s);
             }
             if (rc) return rc;
             st.count_ms += ms_since(t0);
@@ -312,27 +319,36 @@ struct Miner {
         return COLO_OK;
     }
 
-    int build_tables_plain(uint32_t k, std::vector<Prev> &L, const std::vector<uint8_t> &need,
+    int build_tables_plain(uint32_t k, std::vector<Prev> &L, const std::vector<uint8_t> &build,
                            const std::vector<Prev> &Lprev) {
-        DevBuf offs;
         for (size_t i = 0; i < L.size(); i++) {
-            if (!need[i]) continue;
+            if (!build[i]) continue;
             Prev &p = L[i];
             auto t0 = Clock::now();
-            if (!offs.ensure(std::max<uint64_t>(p.nslots, 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets");
+            p.offs = std::make_shared<DevBuf>();
+            if (!p.offs->ensure((p.nslots + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets");
+            p.offs_at = 0;
+            p.off_base = 0;
             uint64_t total = 0;
-            int rc = colo_scan_slots(c, p.slots->as<uint32_t>() + p.slots_off, p.nslots, offs.as<uint64_t>(), &total, s);
+            int rc = colo_scan_slots(c, p.slots->as<uint32_t>() + p.slots_off, p.nslots, p.offs->as<uint64_t>(), &total, s);
             if (rc) return rc;
             st.scan_ms += ms_since(t0);
             auto t1 = Clock::now();
             p.table = std::make_shared<DevBuf>();
             if (!p.table->ensure(std::max<uint64_t>(total * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
-            if (k == 2)
-                rc = colo_join2_write(c, p.pat[0], p.pat[1], p.slots->as<uint32_t>() + p.slots_off, offs.as<uint64_t>(),
-                                      p.table->as<uint32_t>(), s);
-            else
-                rc = colo_joink_write(c, Lprev[p.parent].table->as<uint32_t>(), p.nslots, k, p.pat.data(),
-                                      p.slots->as<uint32_t>() + p.slots_off, offs.as<uint64_t>(), p.table->as<uint32_t>(), s);
+            if (k == 2) {
+                p.nbr = std::make_shared<DevBuf>();
+                if (!p.nbr->ensure(std::max<uint64_t>(total, 1) * 4)) return set_error(c, COLO_ENOMEM, "neighbour list");
+                rc = colo_join2_write(c, p.pat[0], p.pat[1], p.slots->as<uint32_t>() + p.slots_off, p.offs->as<uint64_t>(),
+                                      p.table->as<uint32_t>(), p.nbr->as<uint32_t>(), s);
+            } else {
+                const Prev *nb = nbr_list(p.pat[0], p.pat[k - 1]);
+                if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
+                rc = colo_joink_write_nbr(c, Lprev[p.parent].table->as<uint32_t>(), p.nslots, k, p.pat.data(),
+                                          nb->nbr->as<uint32_t>(), nb->offs->as<uint64_t>() + nb->offs_at,
+                                          p.slots->as<uint32_t>() + p.slots_off, p.offs->as<uint64_t>(),
+                                          p.table->as<uint32_t>(), s);
+            }
             if (rc) return rc;
             if ((rc = sync())) return rc;
             st.write_ms += ms_since(t1);
@@ -342,11 +358,36 @@ struct Miner {
         return COLO_OK;
     }
 
-    // PERSISTENT: one count launch, one PI launch, one scan for the level.
-    DevBuf arena_slots, arena_bits, arena_offs, d_desc, d_items, d_segs, d_segout, d_queue, d_idx, d_rowsel;
+    // Level-2 patterns with tables, kept for the whole run as the anchors'
+    // neighbour lists of the level-k passes.
+    std::vector<Prev> l2_;
+    std::vector<int64_t> l2_index_;  // F x F -> index into l2_, -1 if none
+    const Prev *nbr_list(uint32_t a, uint32_t b) const {
+        if (l2_index_.empty()) return nullptr;
+        int64_t i = l2_index_[(size_t)a * c->F + b];
+        return (i >= 0 && l2_[i].nbr && l2_[i].offs) ? &l2_[i] : nullptr;
+    }
 
-    static void make_items(uint64_t cand, uint64_t n, uint64_t chunk, std::vector<WorkItem> &items) {
-        for (uint64_t b = 0; b < n; b += chunk) items.push_back(WorkItem{(uint32_t)cand, 0, b, std::min(n, b + chunk)});
+    // PERSISTENT: one count launch, one PI launch, one scan for the level.
+    std::shared_ptr<DevBuf> arena_offs;
+    DevBuf arena_slots, arena_bits, d_desc, d_items, d_segs, d_segout, d_queue, d_idx, d_rowsel;
+
+    static void make_items(uint64_t cand, uint64_t n, uint64_t chunk, std::vector<WorkItem> &items, uint32_t ncand = 1) {
+        for (uint64_t b = 0; b < n; b += chunk) items.push_back(WorkItem{(uint32_t)cand, ncand, b, std::min(n, b + chunk)});
+    }
+
+    // Runs of consecutive descriptors that share a parent table (level k);
+    // level 2 descriptors are runs of one.
+    static std::vector<std::pair<uint64_t, uint32_t>> parent_runs(const std::vector<CandDesc> &desc, uint32_t k) {
+        std::vector<std::pair<uint64_t, uint32_t>> runs;
+        for (uint64_t u = 0; u < desc.size();) {
+            uint64_t v = u + 1;
+            if (k > 2)
+                while (v < desc.size() && desc[v].parent == desc[u].parent && v - u < 64) v++;
+            runs.emplace_back(u, (uint32_t)(v - u));
+            u = v;
+        }
+        return runs;
     }
 
     int run_level_persistent(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
@@ -374,6 +415,11 @@ struct Miner {
                 if (parent[u] < 0 || !Lprev[parent[u]].table) return set_error(c, COLO_EARG, "missing parent table");
                 d.n_anchor = Lprev[parent[u]].rows;
                 d.parent = Lprev[parent[u]].table->as<uint32_t>();
+                const Prev *nb = nbr_list(cand[0], cand[k - 1]);
+                if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
+                d.nb_table = nb->nbr->as<uint32_t>();
+                d.nb_off = nb->offs->as<uint64_t>() + nb->offs_at;
+                d.nb_base = nb->off_base;
             }
             d.slots_off = slots_tot;
             slots_tot += d.n_anchor;
@@ -382,13 +428,15 @@ struct Miner {
             d.bits_off = bits_tot;
             bits_tot += words;
         }
-        // work queue: largest candidates first so the tail of the launch is short
-        std::vector<uint64_t> order(m);
-        for (uint64_t u = 0; u < m; u++) order[u] = u;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](uint64_t a, uint64_t b) { return desc[a].n_anchor > desc[b].n_anchor; });
+        // work queue: candidates sharing a parent table form one item stream;
+        // largest runs first so the tail of the launch is short
+        auto runs = parent_runs(desc, k);
+        std::stable_sort(runs.begin(), runs.end(), [&](const std::pair<uint64_t, uint32_t> &a,
+                                                       const std::pair<uint64_t, uint32_t> &b) {
+            return desc[a.first].n_anchor * a.second > desc[b.first].n_anchor * b.second;
+        });
         std::vector<WorkItem> items;
-        for (uint64_t u : order) make_items(u, desc[u].n_anchor, k == 2 ? 1024 : 1024, items);
+        for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
         slots_arena = std::make_shared<DevBuf>();
         if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "slots arena");
         if (!arena_bits.ensure(std::max<uint64_t>(bits_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "bits arena");
@@ -414,8 +462,9 @@ struct Miner {
         }
         st.launches += 2;
         // rows per candidate: one scan over the whole slots arena
-        if (!arena_offs.ensure((slots_tot + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
-        rc = scan_batch(c, slots_arena->as<uint32_t>(), slots_tot, arena_offs.as<uint64_t>(), s);
+        arena_offs = std::make_shared<DevBuf>();
+        if (!arena_offs->ensure((slots_tot + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
+        rc = scan_batch(c, slots_arena->as<uint32_t>(), slots_tot, arena_offs->as<uint64_t>(), s);
         if (rc) return rc;
         std::vector<uint64_t> bounds(m + 1);
         for (uint64_t u = 0; u < m; u++) bounds[u] = desc[u].slots_off;
@@ -423,7 +472,7 @@ struct Miner {
         // offset at position slots_tot = total: computed as the last offset + last slot
         if (!d_idx.ensure((m + 1) * 8) || !d_rowsel.ensure((m + 2) * 8)) return set_error(c, COLO_ENOMEM, "idx");
         COLO_HIP(hipMemcpyAsync(d_idx.p, bounds.data(), m * 8, hipMemcpyHostToDevice, s));
-        rc = gather_u64(c, arena_offs.as<uint64_t>(), d_idx.as<uint64_t>(), m, d_rowsel.as<uint64_t>(), s);
+        rc = gather_u64(c, arena_offs->as<uint64_t>(), d_idx.as<uint64_t>(), m, d_rowsel.as<uint64_t>(), s);
         if (rc) return rc;
         std::vector<uint64_t> starts(m + 1, 0);
         uint32_t last_slot = 0;
@@ -431,7 +480,7 @@ struct Miner {
         COLO_HIP(hipMemcpyAsync(starts.data(), d_rowsel.p, m * 8, hipMemcpyDeviceToHost, s));
         if (slots_tot) {
             COLO_HIP(hipMemcpyAsync(&last_slot, slots_arena->as<uint32_t>() + slots_tot - 1, 4, hipMemcpyDeviceToHost, s));
-            COLO_HIP(hipMemcpyAsync(&last_off, arena_offs.as<uint64_t>() + slots_tot - 1, 8, hipMemcpyDeviceToHost, s));
+            COLO_HIP(hipMemcpyAsync(&last_off, arena_offs->as<uint64_t>() + slots_tot - 1, 8, hipMemcpyDeviceToHost, s));
         }
         // PI segments of every candidate, one launch
         std::vector<SegDesc> segs(m * k);
@@ -478,7 +527,7 @@ struct Miner {
     std::vector<CandDesc> level_desc_;
     std::vector<uint64_t> level_idx_;
 
-    int build_tables_persistent(uint32_t k, std::vector<Prev> &L, const std::vector<uint8_t> &need,
+    int build_tables_persistent(uint32_t k, std::vector<Prev> &L, const std::vector<uint8_t> &build,
                                 const std::vector<Prev> &Lprev, const std::vector<uint32_t> &C) {
         // the offsets arena still holds the level's global row offsets
         std::vector<CandDesc> desc;
@@ -488,21 +537,29 @@ struct Miner {
         for (uint64_t u = 0, pi = 0; u < level_idx_.size() && pi < L.size(); u++)
             if (std::equal(L[pi].pat.begin(), L[pi].pat.end(), &C[level_idx_[u] * k])) cand_of_prev[pi++] = u;
         for (size_t i = 0; i < L.size(); i++) {
-            if (!need[i]) continue;
+            if (!build[i]) continue;
             Prev &p = L[i];
             uint64_t u = cand_of_prev[i];
             if (u == UINT64_MAX) return set_error(c, COLO_EARG, "internal: prevalent pattern not in level");
             p.table = std::make_shared<DevBuf>();
             if (!p.table->ensure(std::max<uint64_t>(p.rows * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
             CandDesc d = level_desc_[u];
-            d.offsets = arena_offs.as<uint64_t>() + d.slots_off;
+            d.offsets = arena_offs->as<uint64_t>() + d.slots_off;
             d.off_base = level_starts_[u];
+            p.offs = arena_offs;
+            p.offs_at = d.slots_off;
+            p.off_base = d.off_base;
+            if (k == 2) {
+                p.nbr = std::make_shared<DevBuf>();
+                if (!p.nbr->ensure(std::max<uint64_t>(p.rows, 1) * 4)) return set_error(c, COLO_ENOMEM, "neighbour list");
+                d.nbr_out = p.nbr->as<uint32_t>();
+            }
             d.table = p.table->as<uint32_t>();
-            make_items(desc.size(), d.n_anchor, 1024, items);
             desc.push_back(d);
             if (k == 2) st.l2_rows += p.rows;
         }
         if (desc.empty()) return COLO_OK;
+        for (auto &r : parent_runs(desc, k)) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
         if (!d_desc.ensure(desc.size() * sizeof(CandDesc)) || !d_items.ensure(items.size() * sizeof(WorkItem)))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
@@ -623,10 +680,16 @@ struct Miner {
                     pref.emplace(N.begin() + i * (k + 1), N.begin() + i * (k + 1) + k);
                 for (size_t i = 0; i < L.size(); i++) need[i] = pref.count(L[i].pat) ? 1 : 0;
             }
+            // tables to build: the reference's (prefixes of the next level's
+            // candidates); at level 2 every prevalent pair, whose tables serve
+            // as the anchors' neighbour lists at the later levels
+            std::vector<uint8_t> build = need;
+            if (k == 2 && !N.empty())
+                for (auto &b : build) b = 1;
             st.host_ms += ms_since(th);
             auto tt = Clock::now();
-            rc = o.sched == COLO_SCHED_PLAIN ? build_tables_plain(k, L, need, Lprev)
-                                             : build_tables_persistent(k, L, need, Lprev, C);
+            rc = o.sched == COLO_SCHED_PLAIN ? build_tables_plain(k, L, build, Lprev)
+                                             : build_tables_persistent(k, L, build, Lprev, C);
             if (rc) return rc;
             if (k == 2) st.l2_join_ms = level_count_ms + ms_since(tt);
             for (size_t i = 0; i < L.size(); i++) {
@@ -643,6 +706,15 @@ struct Miner {
                 rep.printf("\n");
             }
             for (Prev &p : L) p.slots.reset();
+            if (k == 2) {
+                l2_index_.assign((size_t)c->F * c->F, -1);
+                for (size_t i = 0; i < L.size(); i++) {
+                    l2_index_[(size_t)L[i].pat[0] * c->F + L[i].pat[1]] = (int64_t)l2_.size();
+                    l2_.push_back(L[i]);  // shares the table and offsets
+                }
+            } else {
+                for (Prev &p : L) p.offs.reset();
+            }
             Lprev = std::move(L);
             C = std::move(N);
             k++;

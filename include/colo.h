@@ -113,8 +113,9 @@ uint64_t colo_bits_words(const colo_ctx *ctx, uint32_t k, const uint32_t *cand);
 int colo_join2_count(colo_ctx *ctx, uint32_t a, uint32_t b, uint32_t *d_slots, uint32_t *d_bits,
                      uint64_t *rows, void *stream);
 
-/* Exclusive scan of n slot counts into 64-bit row offsets; *total receives
- * the sum (synchronises).
+/* Exclusive scan of n slot counts into 64-bit row offsets; d_offsets has
+ * n + 1 entries, the sum lands in d_offsets[n]; *total receives the sum if
+ * non-NULL (synchronises).
  * Replaces: the host thrust::exclusive_scan (cpp:391, 887), getTotalInstances
  * (DeviceProcess.h:13, cu:154-159) and scalebyConstant (DeviceProcess.h:15,
  * cu:187-196; the write passes scale by the degree themselves). */
@@ -122,10 +123,12 @@ int colo_scan_slots(colo_ctx *ctx, const uint32_t *d_slots, uint64_t n, uint64_t
                     uint64_t *total, void *stream);
 
 /* Level-2 write pass: rows (g, q) in a-instance order, neighbours in scan
- * order, into d_table (2 * rows u32).
+ * order, into d_table (2 * rows u32).  d_nbr (rows u32, may be NULL)
+ * receives the same neighbours as positions in the cell-sorted order of
+ * feature b: the anchors' neighbour lists of the level-k passes.
  * Replaces: degree2InstanceTableGenerator (DeviceProcess.h:6, cu:69-79). */
 int colo_join2_write(colo_ctx *ctx, uint32_t a, uint32_t b, const uint32_t *d_slots,
-                     const uint64_t *d_offsets, uint32_t *d_table, void *stream);
+                     const uint64_t *d_offsets, uint32_t *d_table, uint32_t *d_nbr, void *stream);
 
 /* Level-k count pass (k >= 3) over the R rows of the parent table d_parent
  * (R*(k-1) u32, instance table of cand[0..k-2]); cand is host, k ids.
@@ -139,6 +142,23 @@ int colo_joink_count(colo_ctx *ctx, const uint32_t *d_parent, uint64_t R, uint32
 int colo_joink_write(colo_ctx *ctx, const uint32_t *d_parent, uint64_t R, uint32_t k,
                      const uint32_t *cand, const uint32_t *d_slots, const uint64_t *d_offsets,
                      uint32_t *d_table, void *stream);
+
+/* Level-k passes that take the candidates of each parent row from the
+ * anchor's neighbour list: d_nbr2 / d_offsets2 are the neighbour lists
+ * written by colo_join2_write for (cand[0], cand[k-1]) and their n+1 row
+ * offsets per cand[0] instance (colo_scan_slots of that pair's slots).  Same results as
+ * colo_joink_count / colo_joink_write: the list holds exactly the
+ * last-feature instances in the anchor's 3x3 cells within r of the anchor,
+ * in scan order, so only the other members are tested.
+ * Replaces: generalInstanceTableSlotCounter / calcInstanceTableGeneral
+ * (DeviceProcess.h:8-9, cu:82-105). */
+int colo_joink_count_nbr(colo_ctx *ctx, const uint32_t *d_parent, uint64_t R, uint32_t k,
+                         const uint32_t *cand, const uint32_t *d_nbr2, const uint64_t *d_offsets2,
+                         uint32_t *d_slots, uint32_t *d_bits, uint64_t *rows, void *stream);
+int colo_joink_write_nbr(colo_ctx *ctx, const uint32_t *d_parent, uint64_t R, uint32_t k,
+                         const uint32_t *cand, const uint32_t *d_nbr2, const uint64_t *d_offsets2,
+                         const uint32_t *d_slots, const uint64_t *d_offsets, uint32_t *d_table,
+                         void *stream);
 
 /* Participation-index numerators and PI of one candidate from its bitmap
  * (synchronises).  num_compat: the reference's window sums (cu:113-124,

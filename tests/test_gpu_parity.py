@@ -157,6 +157,19 @@ def test_joink(c2small, colo_mod):
         t3, tot3 = ctx.joink_write(par_t, R, 3, list(cand), slots3)
         ot3 = orc.joink_write(opar, 3, list(cand), oslots3)
         np.testing.assert_array_equal(ctx._np_u32(t3, 3 * tot3), ot3)
+        # the same pass fed by the anchors' neighbour lists: level-2 table of
+        # (cand[0], cand[2]) and its row offsets
+        _say(f"joink k=3 neighbour lists {cand}")
+        _, s_ac, _ = ctx.join2_count(a, cc)
+        _, _, t_ac, off_ac = ctx.join2_write(a, cc, s_ac, lists=True)
+        rn, sn, bn = ctx.joink_count_nbr(par_t, R, 3, list(cand), t_ac, off_ac)
+        assert rn == orows3
+        np.testing.assert_array_equal(ctx._np_u32(sn, R), oslots3)
+        ncn, nen, pcn, pen = ctx.pi(bn, list(cand))
+        np.testing.assert_array_equal(ncn, onc)
+        np.testing.assert_array_equal(nen, one)
+        tn, totn = ctx.joink_write_nbr(par_t, R, 3, list(cand), sn, t_ac, off_ac)
+        np.testing.assert_array_equal(ctx._np_u32(tn, 3 * totn), ot3)
         if tot3 == 0:
             continue
         # one more level on top

@@ -85,3 +85,20 @@ def test_no_gpu_fails_loudly(colo_mod):
         pytest.skip("GPU present")
     with pytest.raises(colo_mod.ColoError):
         colo_mod.Context(np.zeros(4, np.float32), np.zeros(4, np.float32), [2, 2], [0, 0, 1, 1], 0.5)
+
+
+def test_compat_header_compiles(tmp_path):
+    """include/DeviceProcess_compat.h keeps the reference's entry-point names
+    (Codes/DeviceProcess.h:4-36) and must compile as a C++ header."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "DeviceProcess_compat.h"\nusing namespace colo_compat;\n'
+                   "int main() { Integer *s = nullptr; (void)degree2InstanceTableSlotCounter(0, s, 0, 0, 0, s, 0);\n"
+                   "  (void)scalebyConstant(2, nullptr, 0); return (int)getTotalInstances(s, 0); }\n")
+    r = subprocess.run([hipcc, "-std=c++17", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"), str(src)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout

@@ -11,9 +11,12 @@ reference CSV format before the timed region.
 
 Prints one JSON line.  value = neighbour pairs per second of the level-2
 join (pairs counted by the level-2 count pass / wall time of the level-2
-count + PI + scan + write); mining_s = wall time of a step.  The persistent
-(one launch per level phase) schedule is timed; the plain (per-candidate)
-schedule runs on the same data and both reports must be identical.
+count + PI + scan + write); mining_s = wall time of a step.  The timed
+schedule is the learned one (persistent launches whose work queue is ordered
+and partitioned, and whose table arenas are pre-sized, by the trained cost
+model in gpu-colocationmininig_amd/models/cost_mlp.bin) when that model is
+present, else the persistent one; the persistent and plain (per-candidate)
+schedules also run on the same data and every report must be identical.
 """
 import argparse
 import json
@@ -94,6 +97,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--model", default=os.path.join(ROOT, "gpu-colocationmininig_amd", "models", "cost_mlp.bin"))
+    ap.add_argument("--no-model", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is a design "
@@ -111,18 +116,31 @@ def main():
     ctx = colo.Context.from_host(hd, p["r"])
     torch.cuda.synchronize()
 
+    model = None
+    if not args.no_model and os.path.exists(args.model):
+        model = colo.CostModel.load(args.model)
+    main_sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
+    sched_name = {colo.COLO_SCHED_LEARNED: "learned", colo.COLO_SCHED_PERSISTENT: "persistent",
+                  colo.COLO_SCHED_PLAIN: "plain"}
+
+    bin_ms = []
+
     def step(sched):
+        t0 = time.perf_counter()
         ctx.rebin(p["r"])
-        return ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, hash_tables=False)
+        torch.cuda.synchronize()
+        bin_ms.append((time.perf_counter() - t0) * 1000.0)
+        return ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, hash_tables=False,
+                        model=model if sched == colo.COLO_SCHED_LEARNED else None)
 
     for i in range(args.warmup):
-        rep_w, _ = step(colo.COLO_SCHED_PERSISTENT)
+        rep_w, _ = step(main_sched)
         log(f"warmup {i + 1}/{args.warmup} done")
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        rep, st = step(colo.COLO_SCHED_PERSISTENT)
+        rep, st = step(main_sched)
         stats.append(st)
         log(f"step {i + 1}/{args.steps}: {st['total_ms']:.1f} ms, {st['patterns']} patterns")
     torch.cuda.synchronize()
@@ -138,8 +156,17 @@ def main():
            "vs_baseline": None, "dtype": "f32",
            "data": f"synthetic: seeded generator, reference CSV format ({p['name']})",
            "config": {"workload": describe(p), "instances": p["N"], "features": p["F"], "r": p["r"],
-                      "theta": p["theta"], "max_k": p["max_k"], "schedule": "persistent"}}
-    out["mining_s"] = {"persistent": ms_per_step / 1000.0, "parse_s": parse_s}
+                      "theta": p["theta"], "max_k": p["max_k"], "schedule": sched_name[main_sched]}}
+    out["mining_s"] = {sched_name[main_sched]: ms_per_step / 1000.0, "parse_s": parse_s}
+    if model is not None:
+        n_m = sum(s["model_n"] for s in stats)
+        out["cost_model"] = {"mean_abs_log1p_rows_err": sum(s["model_abs_err"] for s in stats) / n_m if n_m else None,
+                             "overflow_fallbacks": sum(s["overflows"] for s in stats),
+                             "arenas": sum(s["arenas"] for s in stats), "predictions": n_m}
+    out["phases_ms_per_step"] = {
+        "bin": sum(bin_ms[args.warmup:args.warmup + args.steps]) / args.steps,
+        **{k: sum(s[k] for s in stats) / args.steps for k in ("filter_ms", "count_ms", "scan_ms", "write_ms",
+                                                               "host_ms", "total_ms")}}
     out["patterns"] = stats[-1]["patterns"]
     out["levels"] = stats[-1]["levels"]
 
@@ -167,16 +194,26 @@ def main():
                                   "pi": sum(s["pi_kernel_ms"] for s in stats) / args.steps}
 
     if not args.no_plain:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rep_p, st_p = step(colo.COLO_SCHED_PLAIN)
-        torch.cuda.synchronize()
-        plain_s = time.perf_counter() - t0
-        out["mining_s"]["plain"] = plain_s
-        out["mining_s"]["persistent_speedup_vs_plain"] = plain_s / (ms_per_step / 1000.0)
-        out["plain_report_identical"] = (rep_p == rep)
-        if rep_p != rep:
-            log("ERROR: plain and persistent reports differ")
+        identical = True
+        others = [colo.COLO_SCHED_PLAIN] + ([colo.COLO_SCHED_PERSISTENT] if main_sched != colo.COLO_SCHED_PERSISTENT
+                                            else [])
+        for sc in others:
+            step(sc)  # warm
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 1 if sc == colo.COLO_SCHED_PLAIN else args.steps
+            for _ in range(reps):
+                rep_o, _ = step(sc)
+            torch.cuda.synchronize()
+            out["mining_s"][sched_name[sc]] = (time.perf_counter() - t0) / reps
+            identical &= rep_o == rep
+        main_s = ms_per_step / 1000.0
+        out["mining_s"][f"{sched_name[main_sched]}_speedup_vs_plain"] = out["mining_s"]["plain"] / main_s
+        if "persistent" in out["mining_s"] and main_sched != colo.COLO_SCHED_PERSISTENT:
+            out["mining_s"]["learned_speedup_vs_persistent"] = out["mining_s"]["persistent"] / main_s
+        out["reports_identical"] = identical
+        if not identical:
+            log("ERROR: schedules produced different reports")
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(csv, p, args.cpu_budget, threads)

@@ -42,7 +42,8 @@ class Grid(C.Structure):
 
 class MineOpts(C.Structure):
     _fields_ = [("theta", C.c_float), ("max_k", C.c_uint32), ("pi_mode", C.c_int32), ("filter_on", C.c_int32),
-                ("sched", C.c_int32), ("hash_tables", C.c_int32), ("out_prefix", C.c_char_p)]
+                ("sched", C.c_int32), ("hash_tables", C.c_int32), ("out_prefix", C.c_char_p),
+                ("model", C.c_void_p), ("log_path", C.c_char_p)]
 
 
 class MineStats(C.Structure):
@@ -53,7 +54,8 @@ class MineStats(C.Structure):
                                           "count_bytes", "write_bytes")] + \
                [(n, C.c_uint64) for n in ("count_launches", "write_launches")] + \
                [(n, C.c_double) for n in ("l2_count_kernel_ms", "l2_count_bytes", "l2_write_kernel_ms",
-                                          "l2_write_bytes")]
+                                          "l2_write_bytes", "model_abs_err", "model_n", "arena_bytes")] + \
+               [(n, C.c_uint64) for n in ("overflows", "arenas")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -66,7 +68,8 @@ SYMBOLS = [
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
-    "colo_candidates",
+    "colo_candidates", "colo_mlp_create", "colo_mlp_destroy", "colo_mlp_set_params", "colo_mlp_get_params",
+    "colo_mlp_set_norm", "colo_mlp_predict", "colo_mlp_grad", "colo_mlp_train", "colo_mlp_save", "colo_mlp_load",
 ]
 
 
@@ -107,6 +110,16 @@ def lib():
         L.colo_host_free.argtypes = [vp]
         L.colo_candidates.restype = u64
         L.colo_candidates.argtypes = [vp, u64, u32, vp, u64]
+        L.colo_mlp_create.argtypes = [u32, C.POINTER(vp)]
+        L.colo_mlp_destroy.argtypes = [vp]
+        L.colo_mlp_set_params.argtypes = [vp, vp, C.c_size_t]
+        L.colo_mlp_get_params.argtypes = [vp, vp, C.c_size_t]
+        L.colo_mlp_set_norm.argtypes = [vp, vp, vp]
+        L.colo_mlp_predict.argtypes = [vp, vp, u32, vp]
+        L.colo_mlp_grad.argtypes = [vp, vp, vp, u32, vp, C.POINTER(f32)]
+        L.colo_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, f32, u32, C.POINTER(f32)]
+        L.colo_mlp_save.argtypes = [vp, C.c_char_p]
+        L.colo_mlp_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         for name in SYMBOLS:
             if name in ("colo_free_str", "colo_host_free"):
                 getattr(L, name).restype = None
@@ -363,9 +376,10 @@ class Context:
 
     @_on_stream
     def mine(self, theta, max_k=0, pi_mode=0, filter_on=True, sched=COLO_SCHED_PERSISTENT, hash_tables=True,
-             out_prefix=None):
+             out_prefix=None, model=None, log_path=None):
         o = MineOpts(C.c_float(theta), max_k, pi_mode, int(filter_on), sched, int(hash_tables),
-                     out_prefix.encode() if out_prefix else None)
+                     out_prefix.encode() if out_prefix else None, model.h if model is not None else None,
+                     log_path.encode() if log_path else None)
         rep = C.c_void_p()
         st = MineStats()
         rc = lib().colo_mine(self.h, C.byref(o), C.byref(rep), C.byref(st))
@@ -375,6 +389,107 @@ class Context:
             lib().colo_free_str(rep)
         self._check(rc)
         return text, st.as_dict()
+
+
+MLP_IN, MLP_HIDDEN = 16, 64
+MLP_NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+FEATURE_NAMES = ["k", "log1p N_first", "log1p N_last", "log1p N_min", "log1p N_max", "log1p anchors",
+                 "log1p filter_first", "log1p filter_last", "filter bound", "log1p cells_first", "log1p cells_last",
+                 "log1p 3x3 work(first,last)", "log1p R2(first,last)", "log1p R2(prev,last)",
+                 "log occupancy_first", "log occupancy_last"]
+RECORD_FLOATS = MLP_IN + 2  # features, log1p(rows), k
+
+
+def read_log(path):
+    """Training records written by the miner's log_path hook."""
+    a = np.fromfile(path, dtype=np.float32)
+    a = a[: len(a) // RECORD_FLOATS * RECORD_FLOATS].reshape(-1, RECORD_FLOATS)
+    return a[:, :MLP_IN].copy(), a[:, MLP_IN].copy()
+
+
+class CostModel:
+    """FP32 MLP 16 -> 64 -> 64 -> 64 -> 1 on the GPU (HIP forward, backward
+    and Adam kernels in colo_mlp.hip) predicting log1p(table rows)."""
+
+    def __init__(self, seed=0, h=None):
+        _torch()
+        if h is None:
+            h = C.c_void_p()
+            rc = lib().colo_mlp_create(seed, C.byref(h))
+            if rc:
+                raise ColoError(rc, "colo_mlp_create")
+        self.h = h
+
+    @classmethod
+    def load(cls, path):
+        _torch()
+        h = C.c_void_p()
+        rc = lib().colo_mlp_load(path.encode(), C.byref(h))
+        if rc:
+            raise ColoError(rc, f"cannot load {path}")
+        return cls(h=h)
+
+    def save(self, path):
+        rc = lib().colo_mlp_save(self.h, path.encode())
+        if rc:
+            raise ColoError(rc, f"cannot save {path}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().colo_mlp_destroy(self.h)
+            self.h = None
+
+    @property
+    def params(self):
+        p = np.zeros(MLP_NPARAMS, np.float32)
+        rc = lib().colo_mlp_get_params(self.h, p.ctypes.data, MLP_NPARAMS)
+        if rc:
+            raise ColoError(rc, "get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        rc = lib().colo_mlp_set_params(self.h, p.ctypes.data, len(p))
+        if rc:
+            raise ColoError(rc, "set_params")
+
+    def set_norm(self, mean, std):
+        mean = np.ascontiguousarray(mean, np.float32)
+        std = np.ascontiguousarray(std, np.float32)
+        rc = lib().colo_mlp_set_norm(self.h, mean.ctypes.data, std.ctypes.data)
+        if rc:
+            raise ColoError(rc, "set_norm")
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.zeros(len(X), np.float32)
+        rc = lib().colo_mlp_predict(self.h, X.ctypes.data, len(X), y.ctypes.data)
+        if rc:
+            raise ColoError(rc, "predict")
+        return y
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        loss = C.c_float()
+        rc = lib().colo_mlp_grad(self.h, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data, C.byref(loss))
+        if rc:
+            raise ColoError(rc, "grad")
+        return g, loss.value
+
+    def train(self, X, T, steps=2000, batch=256, lr=1e-3, seed=0, normalise=True):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        if normalise:
+            self.set_norm(X.mean(0), X.std(0) + 1e-6)
+        loss = C.c_float()
+        rc = lib().colo_mlp_train(self.h, X.ctypes.data, T.ctypes.data, len(X), steps, batch, C.c_float(lr), seed,
+                                  C.byref(loss))
+        if rc:
+            raise ColoError(rc, "train")
+        return loss.value
 
 
 def candidates(L, km1):

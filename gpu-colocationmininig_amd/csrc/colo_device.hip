@@ -574,6 +574,79 @@ __device__ inline void set_bit(uint32_t *bits, uint64_t i) {
     if (!(__atomic_load_n(w, __ATOMIC_RELAXED) & m)) atomicOr(w, m);
 }
 
+// Level-2 work estimate of a (first, last) feature pair: over the non-empty
+// cells c of `first`, count(first, c) x (instances of `last` in the 3x3
+// cells around c) -- the number of distance tests of the 3x3 join.
+__global__ __launch_bounds__(kBlock) void k_pairwork(const FilterItem *items, const uint64_t *chunk_prefix,
+                                                     uint64_t n_items, const uint32_t *pairs, const uint32_t *run_cell,
+                                                     const uint32_t *dir, DevGrid g, unsigned long long *out) {
+    uint64_t b = blockIdx.x;
+    uint64_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (chunk_prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    const FilterItem it = items[lo];
+    uint64_t r0 = it.run_begin + (b - chunk_prefix[lo]) * kFilterRuns;
+    uint64_t r1 = r0 + kFilterRuns < it.run_end ? r0 + kFilterRuns : it.run_end;
+    const uint32_t f0 = pairs[2 * it.cand], fl = pairs[2 * it.cand + 1];
+    const uint32_t *dir0 = dir + (uint64_t)f0 * (g.cells + 1);
+    const uint32_t *dirl = dir + (uint64_t)fl * (g.cells + 1);
+    uint64_t sum = 0;
+    for (uint64_t h = r0 + threadIdx.x; h < r1; h += kBlock) {
+        uint32_t c = run_cell[h];
+        RowSpan rs;
+        int m = row_spans(g, dirl, c, rs);
+        uint64_t nb = 0;
+        for (int r = 0; r < m; r++) nb += rs.hi[r] - rs.lo[r];
+        sum += (uint64_t)(dir0[c + 1] - dir0[c]) * nb;
+    }
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(sum, ex);
+    if (threadIdx.x == 0 && tot) atomicAdd(&out[it.cand], (unsigned long long)tot);
+}
+
+int colo::pairwork_batch(colo_ctx *c, const uint32_t *pairs, uint32_t P, uint64_t *out, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (!P) return COLO_OK;
+    std::vector<FilterItem> items;
+    std::vector<uint64_t> prefix;
+    uint64_t chunks = 0;
+    for (uint32_t i = 0; i < P; i++) {
+        uint32_t f = pairs[2 * i];
+        uint64_t rb = c->run_off[f], re = c->run_off[f + 1];
+        if (re == rb) continue;
+        items.push_back(FilterItem{i, 0, rb, re});
+        prefix.push_back(chunks);
+        chunks += (re - rb + kFilterRuns - 1) / kFilterRuns;
+    }
+    size_t bytes = items.size() * sizeof(FilterItem) + prefix.size() * 8 + (size_t)P * 8 + (size_t)P * 8 + 64;
+    void *tmp;
+    int rc = scratch(c, bytes, &tmp);
+    if (rc) return rc;
+    char *q = (char *)tmp;
+    FilterItem *d_items = (FilterItem *)q;
+    q += items.size() * sizeof(FilterItem);
+    uint64_t *d_prefix = (uint64_t *)q;
+    q += prefix.size() * 8;
+    uint32_t *d_pairs = (uint32_t *)q;
+    q += (size_t)P * 8;
+    uint64_t *d_out = (uint64_t *)(((uintptr_t)q + 7) & ~(uintptr_t)7);
+    if ((rc = fill_u32((uint32_t *)d_out, 2ull * P, 0, s))) return rc;
+    COLO_HIP(hipMemcpyAsync(d_pairs, pairs, (size_t)P * 8, hipMemcpyHostToDevice, s));
+    if (!items.empty()) {
+        COLO_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(FilterItem), hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_pairwork, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_items, d_prefix,
+                           (uint64_t)items.size(), d_pairs, c->d_run_cell, c->d_dir, c->dev().g,
+                           (unsigned long long *)d_out);
+        COLO_HIP(hipGetLastError());
+    }
+    COLO_HIP(hipMemcpyAsync(out, d_out, (size_t)P * 8, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    return COLO_OK;
+}
+
 // Level-2 count for one anchor at sorted position p of feature a
 // (degree2TableSlotCounter_Kernel, K:12-55).
 __device__ inline void join2_count_anchor(const DevData &D, uint32_t a, uint32_t b, uint32_t p, uint32_t *slots,
@@ -1228,6 +1301,7 @@ const char *colo_last_error(const colo_ctx *ctx) { return ctx ? ctx->err.c_str()
 int colo_bin(colo_ctx *c, float r, void *stream) {
     if (!c) return COLO_EARG;
     hipStream_t s = pick_stream(c, stream);
+    if (r != c->r) c->pairwork.clear();  // same data and r: same binning
     int rc = set_grid(c, r);
     if (!rc) rc = fill_u32(c->d_flag, 4, 0, s);
     if (!rc) rc = (c->cellbits + c->fidbits <= 32) ? bin_impl<uint32_t>(c, s) : bin_impl<uint64_t>(c, s);

@@ -107,6 +107,7 @@ struct colo_ctx {
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
     uint64_t *h_pinned = nullptr;   // small pinned host staging
+    std::vector<uint64_t> pairwork; // F x F 3x3 join work per (first, last), filled on first use
     std::string err;
     colo::DevData dev() const;
 };
@@ -138,6 +139,9 @@ void seg_descs(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bit
 void pi_from_segs(const colo_ctx *c, uint32_t k, const uint32_t *cand, const SegOut *so,
                   uint64_t *num_compat, uint64_t *num_exact, float *pi_compat, float *pi_exact);
 int num_cus();
+// 3x3 join work of (first, last) pairs (P pairs, host in/out).
+int pairwork_batch(colo_ctx *c, const uint32_t *pairs, uint32_t P, uint64_t *out, hipStream_t s);
+int mlp_predict_host(colo_mlp *m, const float *X, uint32_t n, float *y);
 }  // namespace colo
 
 #define COLO_HIP(call)                                                                  \

@@ -150,9 +150,11 @@ struct Prev {
     uint64_t nslots = 0;
     uint64_t rows = 0;
     int64_t parent = -1;            // index into the previous level's list
-    std::shared_ptr<DevBuf> table;  // instance table, rows x k
+    std::shared_ptr<DevBuf> table;  // memory holding the instance table (own buffer or a level arena)
+    uint32_t *tptr = nullptr;       // instance table, rows x k
     std::shared_ptr<DevBuf> offs;   // row offsets of the table (nslots + 1 entries from offs_at)
-    std::shared_ptr<DevBuf> nbr;    // level 2: second column as sorted positions
+    std::shared_ptr<DevBuf> nbr;    // level 2: memory of the neighbour list
+    uint32_t *nptr = nullptr;       // level 2: second column as sorted positions
     uint64_t offs_at = 0;           // index of the first offset inside offs
     uint64_t off_base = 0;          // value of the first offset
 };
@@ -286,13 +288,13 @@ struct Miner {
                 rc = colo_join2_count(c, cand[0], cand[1], slots->as<uint32_t>(), bits.as<uint32_t>(), &r.rows, s);
             } else {
                 parent = find(Lprev, cand, k - 1);
-                if (parent < 0 || !Lprev[parent].table) return set_error(c, COLO_EARG, "missing parent table");
+                if (parent < 0 || !Lprev[parent].tptr) return set_error(c, COLO_EARG, "missing parent table");
                 nslots = Lprev[parent].rows;
                 if (!slots->ensure(nslots * 4)) return set_error(c, COLO_ENOMEM, "slots");
                 const Prev *nb = nbr_list(cand[0], cand[k - 1]);
                 if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
-                rc = colo_joink_count_nbr(c, Lprev[parent].table->as<uint32_t>(), nslots, k, cand,
-                                          nb->nbr->as<uint32_t>(), nb->offs->as<uint64_t>() + nb->offs_at,
+                rc = colo_joink_count_nbr(c, Lprev[parent].tptr, nslots, k, cand,
+                                          nb->nptr, nb->offs->as<uint64_t>() + nb->offs_at,
                                           slots->as<uint32_t>(), bits.as<uint32_t>(), &r.rows, s);
             }
             if (rc) return rc;
@@ -336,18 +338,20 @@ struct Miner {
             auto t1 = Clock::now();
             p.table = std::make_shared<DevBuf>();
             if (!p.table->ensure(std::max<uint64_t>(total * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
+            p.tptr = p.table->as<uint32_t>();
             if (k == 2) {
                 p.nbr = std::make_shared<DevBuf>();
                 if (!p.nbr->ensure(std::max<uint64_t>(total, 1) * 4)) return set_error(c, COLO_ENOMEM, "neighbour list");
+                p.nptr = p.nbr->as<uint32_t>();
                 rc = colo_join2_write(c, p.pat[0], p.pat[1], p.slots->as<uint32_t>() + p.slots_off, p.offs->as<uint64_t>(),
-                                      p.table->as<uint32_t>(), p.nbr->as<uint32_t>(), s);
+                                      p.tptr, p.nptr, s);
             } else {
                 const Prev *nb = nbr_list(p.pat[0], p.pat[k - 1]);
                 if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
-                rc = colo_joink_write_nbr(c, Lprev[p.parent].table->as<uint32_t>(), p.nslots, k, p.pat.data(),
-                                          nb->nbr->as<uint32_t>(), nb->offs->as<uint64_t>() + nb->offs_at,
+                rc = colo_joink_write_nbr(c, Lprev[p.parent].tptr, p.nslots, k, p.pat.data(),
+                                          nb->nptr, nb->offs->as<uint64_t>() + nb->offs_at,
                                           p.slots->as<uint32_t>() + p.slots_off, p.offs->as<uint64_t>(),
-                                          p.table->as<uint32_t>(), s);
+                                          p.tptr, s);
             }
             if (rc) return rc;
             if ((rc = sync())) return rc;
@@ -365,7 +369,7 @@ struct Miner {
     const Prev *nbr_list(uint32_t a, uint32_t b) const {
         if (l2_index_.empty()) return nullptr;
         int64_t i = l2_index_[(size_t)a * c->F + b];
-        return (i >= 0 && l2_[i].nbr && l2_[i].offs) ? &l2_[i] : nullptr;
+        return (i >= 0 && l2_[i].nptr && l2_[i].offs) ? &l2_[i] : nullptr;
     }
 
     // PERSISTENT: one count launch, one PI launch, one scan for the level.
@@ -392,7 +396,7 @@ struct Miner {
 
     int run_level_persistent(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
                              const std::vector<Prev> &Lprev, std::vector<CandRes> &res, std::vector<Prev> &L,
-                             std::shared_ptr<DevBuf> &slots_arena) {
+                             std::shared_ptr<DevBuf> &slots_arena, const std::vector<float> *pred) {
         const uint64_t nC = C.size() / k;
         std::vector<uint64_t> idx;  // passing candidates
         for (uint64_t i = 0; i < nC; i++)
@@ -412,12 +416,12 @@ struct Miner {
                 d.n_anchor = c->fcount[cand[0]];
             } else {
                 parent[u] = find(Lprev, cand, k - 1);
-                if (parent[u] < 0 || !Lprev[parent[u]].table) return set_error(c, COLO_EARG, "missing parent table");
+                if (parent[u] < 0 || !Lprev[parent[u]].tptr) return set_error(c, COLO_EARG, "missing parent table");
                 d.n_anchor = Lprev[parent[u]].rows;
-                d.parent = Lprev[parent[u]].table->as<uint32_t>();
+                d.parent = Lprev[parent[u]].tptr;
                 const Prev *nb = nbr_list(cand[0], cand[k - 1]);
                 if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
-                d.nb_table = nb->nbr->as<uint32_t>();
+                d.nb_table = nb->nptr;
                 d.nb_off = nb->offs->as<uint64_t>() + nb->offs_at;
                 d.nb_base = nb->off_base;
             }
@@ -429,14 +433,39 @@ struct Miner {
             bits_tot += words;
         }
         // work queue: candidates sharing a parent table form one item stream;
-        // largest runs first so the tail of the launch is short
+        // costliest runs first so the tail of the launch is short.  Without
+        // a model a run costs its rows x candidates and is cut into
+        // 1024-row items; with the model (learned schedule) a run costs its
+        // rows plus the predicted rows of its tables, and items are cut to
+        // an equal predicted cost.
         auto runs = parent_runs(desc, k);
+        std::vector<double> cost(m, 0.0);
+        for (auto &r : runs) {
+            double cst = (double)desc[r.first].n_anchor * r.second;
+            if (pred) {
+                cst = (double)desc[r.first].n_anchor;
+                for (uint32_t j = 0; j < r.second; j++) cst += std::expm1((double)(*pred)[r.first + j]);
+            }
+            cost[r.first] = cst;
+        }
         std::stable_sort(runs.begin(), runs.end(), [&](const std::pair<uint64_t, uint32_t> &a,
                                                        const std::pair<uint64_t, uint32_t> &b) {
-            return desc[a.first].n_anchor * a.second > desc[b.first].n_anchor * b.second;
+            return cost[a.first] > cost[b.first];
         });
         std::vector<WorkItem> items;
-        for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
+        if (pred) {
+            double total = 0;
+            for (auto &r : runs) total += cost[r.first];
+            double target = std::max(1.0, total / (double)(num_cus() * 8 * 4));
+            for (auto &r : runs) {
+                uint64_t n = desc[r.first].n_anchor;
+                double per_row = n ? cost[r.first] / (double)n : 1.0;
+                uint64_t chunk = (uint64_t)std::min(16384.0, std::max(256.0, target / std::max(per_row, 1e-9)));
+                make_items(r.first, n, chunk, items, r.second);
+            }
+        } else {
+            for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
+        }
         slots_arena = std::make_shared<DevBuf>();
         if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "slots arena");
         if (!arena_bits.ensure(std::max<uint64_t>(bits_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "bits arena");
@@ -453,6 +482,18 @@ struct Miner {
                          arena_bits.as<uint32_t>(), d_queue.as<uint32_t>(), s);
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Count : kCount);
+        if (pred) {
+            // pre-size the level's table arena from the predicted rows while
+            // the count pass runs (the exact sizes are known only after it)
+            double words = 0;
+            for (uint64_t u = 0; u < m; u++) words += std::expm1((double)std::max(0.f, (*pred)[u])) * (k + (k == 2 ? 1 : 0));
+            words *= 1.10;
+            if (words < 4e10) {
+                arena_reserved_ = std::make_shared<DevBuf>();
+                if (!arena_reserved_->ensure((size_t)std::max(words, 1.0) * 4)) arena_reserved_.reset();
+                else st.arena_bytes += (double)arena_reserved_->bytes;
+            }
+        }
         {
             double by = 0;
             for (uint64_t u = 0; u < m; u++) by += count_bytes(k, &C[idx[u] * k], desc[u].n_anchor);
@@ -536,13 +577,30 @@ struct Miner {
         std::vector<uint64_t> cand_of_prev(L.size(), UINT64_MAX);
         for (uint64_t u = 0, pi = 0; u < level_idx_.size() && pi < L.size(); u++)
             if (std::equal(L[pi].pat.begin(), L[pi].pat.end(), &C[level_idx_[u] * k])) cand_of_prev[pi++] = u;
+        // one table arena for the level: tables (k words per row) and, at
+        // level 2, the neighbour lists (1 word per row)
+        uint64_t words = 0;
+        for (size_t i = 0; i < L.size(); i++)
+            if (build[i]) words += L[i].rows * (k + (k == 2 ? 1 : 0));
+        std::shared_ptr<DevBuf> arena;
+        if (arena_reserved_ && arena_reserved_->bytes >= words * 4) {
+            arena = arena_reserved_;      // pre-sized before the count pass
+        } else {
+            if (arena_reserved_) st.overflows++;  // prediction too small: exact size
+            arena = std::make_shared<DevBuf>();
+            if (!arena->ensure(std::max<uint64_t>(words, 1) * 4)) return set_error(c, COLO_ENOMEM, "table arena");
+        }
+        arena_reserved_.reset();
+        st.arenas++;
+        uint64_t at = 0;
         for (size_t i = 0; i < L.size(); i++) {
             if (!build[i]) continue;
             Prev &p = L[i];
             uint64_t u = cand_of_prev[i];
             if (u == UINT64_MAX) return set_error(c, COLO_EARG, "internal: prevalent pattern not in level");
-            p.table = std::make_shared<DevBuf>();
-            if (!p.table->ensure(std::max<uint64_t>(p.rows * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
+            p.table = arena;
+            p.tptr = arena->as<uint32_t>() + at;
+            at += p.rows * k;
             CandDesc d = level_desc_[u];
             d.offsets = arena_offs->as<uint64_t>() + d.slots_off;
             d.off_base = level_starts_[u];
@@ -550,11 +608,12 @@ struct Miner {
             p.offs_at = d.slots_off;
             p.off_base = d.off_base;
             if (k == 2) {
-                p.nbr = std::make_shared<DevBuf>();
-                if (!p.nbr->ensure(std::max<uint64_t>(p.rows, 1) * 4)) return set_error(c, COLO_ENOMEM, "neighbour list");
-                d.nbr_out = p.nbr->as<uint32_t>();
+                p.nbr = arena;
+                p.nptr = arena->as<uint32_t>() + at;
+                at += p.rows;
+                d.nbr_out = p.nptr;
             }
-            d.table = p.table->as<uint32_t>();
+            d.table = p.tptr;
             desc.push_back(d);
             if (k == 2) st.l2_rows += p.rows;
         }
@@ -571,12 +630,8 @@ struct Miner {
         span(e0, mark(), k == 2 ? kL2Write : kWrite);
         {
             double by = 0;
-            for (const CandDesc &d : desc) {
-                uint64_t rows = 0;
-                for (const Prev &p : L)
-                    if (p.table && p.table->p == (void *)d.table) rows = p.rows;
-                by += write_bytes(k, d.f, d.n_anchor, rows);
-            }
+            for (size_t i = 0; i < L.size(); i++)
+                if (build[i]) by += write_bytes(k, L[i].pat.data(), L[i].nslots, L[i].rows);
             st.write_bytes += by;
             if (k == 2) st.l2_write_bytes += by;
             st.write_launches++;
@@ -586,6 +641,102 @@ struct Miner {
         st.launches += 2;
         (void)Lprev;
         return COLO_OK;
+    }
+
+    // ---------------------------------------------------- cost model
+    std::shared_ptr<DevBuf> arena_reserved_;
+
+    // Features of a candidate (COLO_MLP_IN floats): size, instance counts,
+    // anchors / parent rows, filter counts and bound, non-empty cells, the
+    // 3x3 work of (first, last), level-2 rows of (first, last) and
+    // (second-to-last, last), mean cell occupancy of first and last.
+    void cand_features(uint32_t k, const uint32_t *cand, const uint32_t *fcnt, uint64_t n_anchor, uint64_t pairwork,
+                       float *f) const {
+        auto L1 = [](double v) { return (float)std::log1p(v > 0 ? v : 0); };
+        uint32_t a = cand[0], z = cand[k - 1];
+        double Nmin = 1e30, Nmax = 0, ub = 1e30;
+        for (uint32_t j = 0; j < k; j++) {
+            double N = c->fcount[cand[j]];
+            Nmin = std::min(Nmin, N);
+            Nmax = std::max(Nmax, N);
+            double u = fcnt ? (float)((double)fcnt[j] / (double)(float)N) : 1.0;
+            ub = std::min(ub, u);
+        }
+        auto H = [&](uint32_t ff) { return (double)(c->run_off[ff + 1] - c->run_off[ff]); };
+        auto R2 = [&](uint32_t x, uint32_t y) -> double {
+            if (l2_index_.empty()) return 0;
+            int64_t i = l2_index_[(size_t)x * c->F + y];
+            return i >= 0 ? (double)l2_[i].rows : 0;
+        };
+        f[0] = (float)k;
+        f[1] = L1(c->fcount[a]);
+        f[2] = L1(c->fcount[z]);
+        f[3] = L1(Nmin);
+        f[4] = L1(Nmax);
+        f[5] = L1((double)n_anchor);
+        f[6] = L1(fcnt ? fcnt[0] : 0);
+        f[7] = L1(fcnt ? fcnt[k - 1] : 0);
+        f[8] = (float)ub;
+        f[9] = L1(H(a));
+        f[10] = L1(H(z));
+        f[11] = L1((double)pairwork);
+        f[12] = k > 2 ? L1(R2(a, z)) : 0.f;
+        f[13] = k > 2 ? L1(R2(cand[k - 2], z)) : 0.f;
+        f[14] = (float)std::log(c->fcount[a] / std::max(1.0, H(a)));
+        f[15] = (float)std::log(c->fcount[z] / std::max(1.0, H(z)));
+    }
+
+    // Features of the level's passing candidates (rows of COLO_MLP_IN).
+    int level_features(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
+                       const std::vector<uint32_t> &fcnt, const std::vector<Prev> &Lprev,
+                       std::vector<uint64_t> &idx, std::vector<float> &X) {
+        const uint64_t nC = C.size() / k;
+        idx.clear();
+        for (uint64_t i = 0; i < nC; i++)
+            if (pass[i]) idx.push_back(i);
+        // 3x3 join work of every (first, last) feature pair, computed once
+        // per binning in one launch
+        if (c->pairwork.empty() && c->F) {
+            std::vector<uint32_t> pairs;
+            for (uint32_t a = 0; a < c->F; a++)
+                for (uint32_t b = 0; b < c->F; b++) {
+                    pairs.push_back(a);
+                    pairs.push_back(b);
+                }
+            std::vector<uint64_t> pw((size_t)c->F * c->F, 0);
+            int rc = pairwork_batch(c, pairs.data(), (uint32_t)(pairs.size() / 2), pw.data(), s);
+            if (rc) return rc;
+            c->pairwork = std::move(pw);
+        }
+        std::vector<uint64_t> pw(idx.size(), 0);
+        for (size_t u = 0; u < idx.size(); u++)
+            pw[u] = c->pairwork[(size_t)C[idx[u] * k] * c->F + C[idx[u] * k + k - 1]];
+        X.assign(idx.size() * COLO_MLP_IN, 0.f);
+        for (size_t u = 0; u < idx.size(); u++) {
+            const uint32_t *cand = &C[idx[u] * k];
+            uint64_t na = c->fcount[cand[0]];
+            if (k > 2) {
+                int64_t pi = find(Lprev, cand, k - 1);
+                na = pi >= 0 ? Lprev[pi].rows : 0;
+            }
+            cand_features(k, cand, &fcnt[idx[u] * k], na, pw[u], &X[u * COLO_MLP_IN]);
+        }
+        return COLO_OK;
+    }
+
+    void log_records(uint32_t k, const std::vector<uint64_t> &idx, const std::vector<float> &X,
+                     const std::vector<CandRes> &res) {
+        if (!o.log_path || idx.empty()) return;
+        FILE *fp = fopen(o.log_path, "ab");
+        if (!fp) return;
+        std::vector<float> rec(COLO_MLP_IN + 2);
+        for (size_t u = 0; u < idx.size(); u++) {
+            memcpy(rec.data(), &X[u * COLO_MLP_IN], COLO_MLP_IN * 4);
+            rec[COLO_MLP_IN] = (float)std::log1p((double)res[idx[u]].rows);
+            rec[COLO_MLP_IN + 1] = (float)k;
+            fwrite(rec.data(), 4, rec.size(), fp);
+        }
+        fclose(fp);
     }
 
     int run(char **report) {
@@ -634,10 +785,32 @@ struct Miner {
             std::vector<Prev> L;
             std::shared_ptr<DevBuf> arena;
             auto tl = Clock::now();
-            int rc = o.sched == COLO_SCHED_PLAIN ? run_level_plain(k, C, pass, Lprev, res, L)
-                                                 : run_level_persistent(k, C, pass, Lprev, res, L, arena);
+            // cost-model features of the passing candidates (logging and the
+            // learned schedule); predictions of log1p(rows)
+            const bool learned = o.sched == COLO_SCHED_LEARNED && o.model;
+            std::vector<uint64_t> fidx;
+            std::vector<float> FX, pred;
+            if (learned || o.log_path) {
+                int rc0 = level_features(k, C, pass, fcnt, Lprev, fidx, FX);
+                if (rc0) return rc0;
+            }
+            if (learned && !fidx.empty()) {
+                pred.resize(fidx.size());
+                int rc0 = mlp_predict_host(o.model, FX.data(), (uint32_t)fidx.size(), pred.data());
+                if (rc0) return set_error(c, rc0, "cost model prediction");
+
+            }
+            int rc = o.sched == COLO_SCHED_PLAIN
+                         ? run_level_plain(k, C, pass, Lprev, res, L)
+                         : run_level_persistent(k, C, pass, Lprev, res, L, arena, learned ? &pred : nullptr);
             if (rc) return rc;
             double level_count_ms = ms_since(tl);
+            if (learned)
+                for (size_t u = 0; u < fidx.size(); u++) {
+                    st.model_abs_err += std::fabs((double)pred[u] - std::log1p((double)res[fidx[u]].rows));
+                    st.model_n += 1;
+                }
+            log_records(k, fidx, FX, res);
             rep.printf("level %u cand %llu pass %llu prev %llu\n", k, (unsigned long long)nC,
                        (unsigned long long)npass, (unsigned long long)L.size());
             for (uint64_t i = 0; i < nC; i++) {
@@ -700,7 +873,7 @@ struct Miner {
                 rep.printf(" rows %llu", (unsigned long long)p.rows);
                 if (o.hash_tables) {
                     uint64_t h = 0;
-                    if ((rc = table_hash(c, p.table->as<uint32_t>(), p.rows * k, &h, s))) return rc;
+                    if ((rc = table_hash(c, p.tptr, p.rows * k, &h, s))) return rc;
                     rep.printf(" h %016llx", (unsigned long long)h);
                 }
                 rep.printf("\n");

@@ -181,6 +181,8 @@ int colo_participation(colo_ctx *ctx, const uint32_t *d_bits, uint32_t k, const 
 #define COLO_SCHED_PERSISTENT 1 /* one persistent launch per level phase       */
 #define COLO_SCHED_LEARNED 2    /* persistent, queue ordered/pre-sized by MLP  */
 
+typedef struct colo_mlp colo_mlp;
+
 typedef struct colo_mine_opts {
     float theta;            /* prevalence threshold (parameter.txt line 2)   */
     uint32_t max_k;         /* largest pattern size, 0 = no cap               */
@@ -189,6 +191,9 @@ typedef struct colo_mine_opts {
     int32_t sched;          /* COLO_SCHED_*                                   */
     int32_t hash_tables;    /* 1 = report a position-sensitive table checksum */
     const char *out_prefix; /* ColocationDegree_<k>.txt prefix, NULL = none   */
+    colo_mlp *model;        /* cost model of COLO_SCHED_LEARNED              */
+    const char *log_path;   /* append (features, rows) records of every
+                               counted candidate (training data), or NULL   */
 } colo_mine_opts;
 
 /* Per-run measurements (milliseconds unless stated).  The *_kernel_ms
@@ -206,7 +211,38 @@ typedef struct colo_mine_stats {
     uint64_t count_launches, write_launches;
     double l2_count_kernel_ms, l2_count_bytes;
     double l2_write_kernel_ms, l2_write_bytes;
+    /* learned schedule: predictions vs counted rows (|log1p| error summed
+     * over candidates), table arenas that the prediction under-sized (the
+     * exact allocation was taken instead), and arena bytes reserved */
+    double model_abs_err, model_n, arena_bytes;
+    uint64_t overflows, arenas;
 } colo_mine_stats;
+
+/* ------------------------------------------------------- cost model */
+
+/* Candidate features the miner logs and the cost model reads (16 floats per
+ * candidate, see COLO_FEATURE_NAMES in colo.py) and the model's shape:
+ * 16 -> 64 -> 64 -> 64 -> 1, ReLU, FP32, predicting log1p(table rows).
+ * No counterpart in the reference: these serve the learned schedule. */
+#define COLO_MLP_IN 16
+#define COLO_MLP_HIDDEN 64
+#define COLO_MLP_NPARAMS (64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+
+int colo_mlp_create(uint32_t seed, colo_mlp **out);
+int colo_mlp_destroy(colo_mlp *m);
+int colo_mlp_set_params(colo_mlp *m, const float *params, size_t n);
+int colo_mlp_get_params(colo_mlp *m, float *params, size_t n);
+int colo_mlp_set_norm(colo_mlp *m, const float *mean, const float *stdv);
+/* y[i] = model(X[i]) for n rows of COLO_MLP_IN host floats. */
+int colo_mlp_predict(colo_mlp *m, const float *X, uint32_t n, float *y);
+/* Gradient of the mean squared error over the n rows (HIP forward and
+ * backward kernels); grad has COLO_MLP_NPARAMS floats. */
+int colo_mlp_grad(colo_mlp *m, const float *X, const float *T, uint32_t n, float *grad, float *loss);
+/* Adam training on the GPU: steps minibatches of `batch` rows. */
+int colo_mlp_train(colo_mlp *m, const float *X, const float *T, uint32_t n, uint32_t steps, uint32_t batch,
+                   float lr, uint32_t seed, float *final_loss);
+int colo_mlp_save(colo_mlp *m, const char *path);
+int colo_mlp_load(const char *path, colo_mlp **out);
 
 /* Runs the Apriori level loop (colocationFinder::Begin cpp:1104-1137 with
  * degree2Processing cpp:256-425, generatePrevalentPatternsGeneral

@@ -165,8 +165,8 @@ def main():
                              "arenas": sum(s["arenas"] for s in stats), "predictions": n_m}
     out["phases_ms_per_step"] = {
         "bin": sum(bin_ms[args.warmup:args.warmup + args.steps]) / args.steps,
-        **{k: sum(s[k] for s in stats) / args.steps for k in ("filter_ms", "count_ms", "scan_ms", "write_ms",
-                                                               "host_ms", "total_ms")}}
+        **{k: sum(s[k] for s in stats) / args.steps for k in ("filter_ms", "model_ms", "count_ms", "scan_ms",
+                                                               "write_ms", "host_ms", "total_ms")}}
     out["patterns"] = stats[-1]["patterns"]
     out["levels"] = stats[-1]["levels"]
 

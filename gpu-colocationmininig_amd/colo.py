@@ -55,7 +55,7 @@ class MineStats(C.Structure):
                [(n, C.c_uint64) for n in ("count_launches", "write_launches")] + \
                [(n, C.c_double) for n in ("l2_count_kernel_ms", "l2_count_bytes", "l2_write_kernel_ms",
                                           "l2_write_bytes", "model_abs_err", "model_n", "arena_bytes")] + \
-               [(n, C.c_uint64) for n in ("overflows", "arenas")]
+               [(n, C.c_uint64) for n in ("overflows", "arenas")] + [("model_ms", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -574,6 +574,30 @@ __device__ inline void set_bit(uint32_t *bits, uint64_t i) {
     if (!(__atomic_load_n(w, __ATOMIC_RELAXED) & m)) atomicOr(w, m);
 }
 
+// Collects the bits a lane sets in ascending index order (candidates come
+// in sorted position order) and issues one atomic per 32-bit word.
+struct BitAcc {
+    uint32_t *bits;
+    uint64_t word = ~0ull;
+    uint32_t mask = 0;
+    __device__ explicit BitAcc(uint32_t *b) : bits(b) {}
+    __device__ inline void flush() {
+        if (mask) {
+            uint32_t *w = bits + word;
+            if ((__atomic_load_n(w, __ATOMIC_RELAXED) & mask) != mask) atomicOr(w, mask);
+        }
+        mask = 0;
+    }
+    __device__ inline void set(uint64_t i) {
+        uint64_t w = i >> 5;
+        if (w != word) {
+            flush();
+            word = w;
+        }
+        mask |= 1u << (i & 31);
+    }
+};
+
 // Level-2 work estimate of a (first, last) feature pair: over the non-empty
 // cells c of `first`, count(first, c) x (instances of `last` in the 3x3
 // cells around c) -- the number of distance tests of the 3x3 join.
@@ -659,14 +683,16 @@ __device__ inline void join2_count_anchor(const DevData &D, uint32_t a, uint32_t
     RowSpan rs;
     int m = row_spans(D.g, dirb, D.scell[p], rs);
     uint32_t cnt = 0;
+    BitAcc acc(bits_b);
     for (int r = 0; r < m; r++) {
         for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
             if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
                 cnt++;
-                set_bit(bits_b, q - sb);
+                acc.set(q - sb);
             }
         }
     }
+    acc.flush();
     slots[t] = cnt;
     if (cnt) set_bit(bits_a, t);
 }
@@ -685,6 +711,7 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
     RowSpan rs;
     int nr = row_spans(D.g, dirl, D.cell_of[row[0]], rs);
     uint32_t cnt = 0;
+    BitAcc acc(bits + seg_off[km1]);
     for (int r = 0; r < nr; r++) {
         for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
             const float qx = D.sx[q], qy = D.sy[q];
@@ -693,10 +720,11 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
                 if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
             if (ok) {
                 cnt++;
-                set_bit(bits + seg_off[km1], q - sl);
+                acc.set(q - sl);
             }
         }
     }
+    acc.flush();
     *slot = cnt;
     if (cnt)
         for (uint32_t m = 0; m < km1; m++) set_bit(bits + seg_off[m], row[m] - D.fstart[feat[m]]);
@@ -776,6 +804,7 @@ __device__ inline void joink_count_row_nbr(const DevData &D, const uint32_t *row
     const uint64_t b0 = nb_off[t] - nb_base, b1 = nb_off[t + 1] - nb_base;
     const uint32_t sl = D.fstart[fl];
     uint32_t cnt = 0;
+    BitAcc acc(bits + seg_off[km1]);
     for (uint64_t i = b0; i < b1; i++) {
         const uint32_t q = nbr[i];  // sorted position
         const float qx = D.sx[q], qy = D.sy[q];
@@ -784,9 +813,10 @@ __device__ inline void joink_count_row_nbr(const DevData &D, const uint32_t *row
             if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
         if (ok) {
             cnt++;
-            set_bit(bits + seg_off[km1], q - sl);
+            acc.set(q - sl);
         }
     }
+    acc.flush();
     *slot = cnt;
     if (cnt)
         for (uint32_t m = 0; m < km1; m++) set_bit(bits + seg_off[m], row[m] - D.fstart[feat[m]]);
@@ -888,13 +918,32 @@ __global__ void k_joink_write_nbr(DevData D, const uint32_t *parent, uint64_t R,
 
 constexpr int kItemAnchors = kBlock * 4;
 
-__device__ inline bool next_item(uint32_t *queue, uint64_t n_items, uint64_t &item) {
+// Work queue with one segment per XCD.  Workgroups are dispatched to the 8
+// XCDs round-robin by workgroup id, so workgroup b drains segment b % 8
+// first -- consecutive items (adjacent anchors of one candidate, which read
+// the same neighbourhoods) then share that XCD's L2 -- and steals from the
+// other segments when its own is empty.  seg holds 9 item boundaries; ctr
+// holds 8 counters (zeroed before the launch).
+constexpr int kXcds = 8;
+
+__device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &item) {
     __shared__ uint64_t s_item;
-    if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long *)queue, 1ull);
+    if (threadIdx.x == 0) {
+        uint64_t got = ~0ull;
+        const uint32_t home = blockIdx.x % kXcds;
+        for (uint32_t d = 0; d < kXcds && got == ~0ull; d++) {
+            const uint32_t x = (home + d) % kXcds;
+            const uint64_t b = seg[x], e = seg[x + 1];
+            if (b >= e || __atomic_load_n(&ctr[x], __ATOMIC_RELAXED) >= e - b) continue;
+            const uint32_t i = atomicAdd(&ctr[x], 1u);
+            if (b + i < e) got = b + i;
+        }
+        s_item = got;
+    }
     __syncthreads();
     item = s_item;
     __syncthreads();
-    return item < n_items;
+    return item != ~0ull;
 }
 
 // Level-k count over the rows of one parent table for ncand candidates that
@@ -920,6 +969,7 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         const uint32_t sl = D.fstart[cd.f[km1]];
         uint32_t *bt = bits + cd.bits_off;
         uint32_t cnt = 0;
+        BitAcc acc(bt + cd.seg_off[km1]);
         for (uint64_t e = b0; e < b1; e++) {
             const uint32_t q = cd.nb_table[e];
             const float qx = D.sx[q], qy = D.sy[q];
@@ -928,9 +978,10 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
                 if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
             if (ok) {
                 cnt++;
-                set_bit(bt + cd.seg_off[km1], q - sl);
+                acc.set(q - sl);
             }
         }
+        acc.flush();
         slots[cd.slots_off + i] = cnt;
         if (cnt)
             for (uint32_t m = 0; m < km1; m++) set_bit(bt + cd.seg_off[m], ids[m] - D.fstart[cd.f[m]]);
@@ -938,10 +989,10 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
 }
 
 __global__ __launch_bounds__(kBlock) void k_count_batch(DevData D, const CandDesc *desc, const WorkItem *items,
-                                                        uint64_t n_items, uint32_t *slots, uint32_t *bits,
+                                                        const uint64_t *seg, uint32_t *slots, uint32_t *bits,
                                                         uint32_t *queue) {
     uint64_t it;
-    while (next_item(queue, n_items, it)) {
+    while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         const CandDesc &cd = desc[wi.cand];
         uint32_t *sl = slots + cd.slots_off;
@@ -1003,9 +1054,9 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
 }
 
 __global__ __launch_bounds__(kBlock) void k_write_batch(DevData D, const CandDesc *desc, const WorkItem *items,
-                                                        uint64_t n_items, const uint32_t *slots, uint32_t *queue) {
+                                                        const uint64_t *seg, const uint32_t *slots, uint32_t *queue) {
     uint64_t it;
-    while (next_item(queue, n_items, it)) {
+    while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         const CandDesc &cd = desc[wi.cand];
         const uint32_t *sl = slots + cd.slots_off;
@@ -1029,29 +1080,27 @@ __global__ __launch_bounds__(kBlock) void k_write_batch(DevData D, const CandDes
 
 static unsigned persistent_grid(uint64_t n_items) {
     uint64_t g = (uint64_t)num_cus() * 8;
-    if (g > n_items) g = n_items;
-    return (unsigned)(g ? g : 1);
+    if (g > n_items) g = (n_items + kXcds - 1) / kXcds * kXcds;
+    return (unsigned)(g ? g : kXcds);
 }
 
-int colo::count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items, uint32_t k,
-                      uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s) {
-    (void)k;
+int colo::count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+                      uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s) {
     if (!n_items) return COLO_OK;
-    int rc = fill_u32(d_queue, 2, 0, s);
+    int rc = fill_u32(d_queue, kXcds, 0, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_count_batch, dim3(persistent_grid(n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
-                       d_items, n_items, d_slots, d_bits, d_queue);
+                       d_items, d_seg, d_slots, d_bits, d_queue);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "count_batch launch");
 }
 
-int colo::write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items, uint32_t k,
-                      const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s) {
-    (void)k;
+int colo::write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+                      uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s) {
     if (!n_items) return COLO_OK;
-    int rc = fill_u32(d_queue, 2, 0, s);
+    int rc = fill_u32(d_queue, kXcds, 0, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_write_batch, dim3(persistent_grid(n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
-                       d_items, n_items, d_slots, d_queue);
+                       d_items, d_seg, d_slots, d_queue);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "write_batch launch");
 }
 
@@ -1378,6 +1427,7 @@ int colo_ctx_destroy(colo_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    c->pool.release();
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return COLO_OK;

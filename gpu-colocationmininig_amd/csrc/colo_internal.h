@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,56 @@ struct SegOut {
     uint32_t first, last;
 };
 
+// Caching device allocator of one context: freed blocks are kept by size and
+// handed out again (a mining run allocates and frees multi-GB table arenas
+// per level; hipMalloc/hipFree of such sizes cost milliseconds).  Blocks are
+// rounded up to 2 MiB (or a power of two below that) and reused when at most
+// 1.5x the request.  Everything is returned to HIP on release() and when the
+// context is destroyed.
+struct DevPool {
+    std::multimap<size_t, void *> free_blocks;
+    size_t cached = 0;
+    static size_t round(size_t b) {
+        if (b >= (2u << 20)) return (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    void *alloc(size_t bytes, size_t *got) {
+        size_t want = round(bytes ? bytes : 1);
+        auto it = free_blocks.lower_bound(want);
+        if (it != free_blocks.end() && it->first <= want + want / 2) {
+            void *p = it->second;
+            *got = it->first;
+            cached -= it->first;
+            free_blocks.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            release();  // give the cache back and try once more
+            if (hipMalloc(&p, want) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+        }
+        *got = want;
+        return p;
+    }
+    void free(void *p, size_t bytes) {
+        if (!p) return;
+        free_blocks.emplace(bytes, p);
+        cached += bytes;
+    }
+    void release() {
+        (void)hipDeviceSynchronize();
+        for (auto &kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+        cached = 0;
+    }
+};
+
 }  // namespace colo
 
 struct colo_ctx {
@@ -108,6 +159,7 @@ struct colo_ctx {
     size_t scratch_bytes = 0;
     uint64_t *h_pinned = nullptr;   // small pinned host staging
     std::vector<uint64_t> pairwork; // F x F 3x3 join work per (first, last), filled on first use
+    colo::DevPool pool;             // miner buffers (tables, arenas, slots, offsets)
     std::string err;
     colo::DevData dev() const;
 };
@@ -123,10 +175,12 @@ int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 int filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
                  uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
 // Batched passes over many candidates (one launch each).
-int count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items,
-                uint32_t k, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s);
-int write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, uint64_t n_items,
-                uint32_t k, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
+// d_seg: 9 item boundaries, one segment per XCD (see xcd_segments);
+// d_queue: 8 counters.
+int count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+                uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s);
+int write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+                uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
 int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
                  hipStream_t s);
 int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t n, uint64_t *h, hipStream_t s);

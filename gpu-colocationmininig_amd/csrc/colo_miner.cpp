@@ -116,21 +116,32 @@ std::vector<uint32_t> gen_candidates(const std::vector<uint32_t> &L, uint32_t km
     return out;
 }
 
+// Device buffer from the context's caching pool (or plain hipMalloc when
+// made without one).  Users synchronise before a buffer is released.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DevPool *pool = nullptr;
     DevBuf() = default;
+    explicit DevBuf(DevPool *pl) : pool(pl) {}
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (pool) pool->free(p, bytes);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
     }
     bool ensure(size_t b) {
         if (b <= bytes && p) return true;
         release();
+        if (pool) {
+            p = pool->alloc(b, &bytes);
+            return p != nullptr;
+        }
         if (hipMalloc(&p, b ? b : 8) != hipSuccess) {
             p = nullptr;
             return false;
@@ -167,7 +178,10 @@ struct Miner {
     Str rep;
     colo_ctx *ctx_for_err;
 
-    Miner(colo_ctx *c_, const colo_mine_opts &o_) : c(c_), o(o_), s(c_->stream), ctx_for_err(c_) {}
+    Miner(colo_ctx *c_, const colo_mine_opts &o_) : c(c_), o(o_), s(c_->stream), ctx_for_err(c_) {
+        for (DevBuf *b : {&arena_slots, &arena_bits, &d_desc, &d_items, &d_segs, &d_segout, &d_queue, &d_idx, &d_rowsel})
+            b->pool = &c->pool;
+    }
     ~Miner() {
         for (auto &e : pool_) (void)hipEventDestroy(e);
     }
@@ -267,7 +281,7 @@ struct Miner {
     int run_level_plain(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
                         const std::vector<Prev> &Lprev, std::vector<CandRes> &res, std::vector<Prev> &L) {
         const uint64_t nC = C.size() / k;
-        DevBuf bits;
+        DevBuf bits(&c->pool);
         uint64_t max_words = 0;
         for (uint64_t i = 0; i < nC; i++)
             if (pass[i]) max_words = std::max(max_words, colo_bits_words(c, k, &C[i * k]));
@@ -277,7 +291,7 @@ struct Miner {
             const uint32_t *cand = &C[i * k];
             CandRes &r = res[i];
             r.pass = true;
-            auto slots = std::make_shared<DevBuf>();
+            auto slots = std::make_shared<DevBuf>(&c->pool);
             uint64_t nslots;
             int64_t parent = -1;
             auto t0 = Clock::now();
@@ -327,7 +341,7 @@ struct Miner {
             if (!build[i]) continue;
             Prev &p = L[i];
             auto t0 = Clock::now();
-            p.offs = std::make_shared<DevBuf>();
+            p.offs = std::make_shared<DevBuf>(&c->pool);
             if (!p.offs->ensure((p.nslots + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets");
             p.offs_at = 0;
             p.off_base = 0;
@@ -336,11 +350,11 @@ struct Miner {
             if (rc) return rc;
             st.scan_ms += ms_since(t0);
             auto t1 = Clock::now();
-            p.table = std::make_shared<DevBuf>();
+            p.table = std::make_shared<DevBuf>(&c->pool);
             if (!p.table->ensure(std::max<uint64_t>(total * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
             p.tptr = p.table->as<uint32_t>();
             if (k == 2) {
-                p.nbr = std::make_shared<DevBuf>();
+                p.nbr = std::make_shared<DevBuf>(&c->pool);
                 if (!p.nbr->ensure(std::max<uint64_t>(total, 1) * 4)) return set_error(c, COLO_ENOMEM, "neighbour list");
                 p.nptr = p.nbr->as<uint32_t>();
                 rc = colo_join2_write(c, p.pat[0], p.pat[1], p.slots->as<uint32_t>() + p.slots_off, p.offs->as<uint64_t>(),
@@ -379,6 +393,38 @@ struct Miner {
     static void make_items(uint64_t cand, uint64_t n, uint64_t chunk, std::vector<WorkItem> &items, uint32_t ncand = 1) {
         for (uint64_t b = 0; b < n; b += chunk) items.push_back(WorkItem{(uint32_t)cand, ncand, b, std::min(n, b + chunk)});
     }
+
+    // 8 contiguous item segments of about equal weight (rows x candidates,
+    // or predicted cost), one per XCD, appended to the item upload.
+    static std::vector<uint64_t> xcd_segments(const std::vector<WorkItem> &items, const std::vector<double> *w) {
+        std::vector<uint64_t> seg(9, items.size());
+        double tot = 0;
+        std::vector<double> wt(items.size());
+        for (size_t i = 0; i < items.size(); i++) {
+            wt[i] = w ? (*w)[i] : (double)(items[i].end - items[i].begin) * items[i].ncand;
+            tot += wt[i];
+        }
+        seg[0] = 0;
+        double acc = 0;
+        size_t x = 1;
+        for (size_t i = 0; i < items.size() && x < 8; i++) {
+            acc += wt[i];
+            while (x < 8 && acc >= tot * x / 8.0) seg[x++] = i + 1;
+        }
+        for (; x < 8; x++) seg[x] = items.size();
+        seg[8] = items.size();
+        return seg;
+    }
+    int upload_items(const std::vector<WorkItem> &items, const std::vector<double> *w) {
+        std::vector<uint64_t> seg = xcd_segments(items, w);
+        size_t ib = items.size() * sizeof(WorkItem);
+        if (!d_items.ensure(ib + 9 * 8)) return set_error(c, COLO_ENOMEM, "work items");
+        COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), ib, hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync((char *)d_items.p + ib, seg.data(), 9 * 8, hipMemcpyHostToDevice, s));
+        d_seg_ = (const uint64_t *)((const char *)d_items.p + ib);
+        return COLO_OK;
+    }
+    const uint64_t *d_seg_ = nullptr;
 
     // Runs of consecutive descriptors that share a parent table (level k);
     // level 2 descriptors are runs of one.
@@ -453,6 +499,7 @@ struct Miner {
             return cost[a.first] > cost[b.first];
         });
         std::vector<WorkItem> items;
+        std::vector<double> iw;  // predicted cost of each item (learned schedule)
         if (pred) {
             double total = 0;
             for (auto &r : runs) total += cost[r.first];
@@ -461,25 +508,29 @@ struct Miner {
                 uint64_t n = desc[r.first].n_anchor;
                 double per_row = n ? cost[r.first] / (double)n : 1.0;
                 uint64_t chunk = (uint64_t)std::min(16384.0, std::max(256.0, target / std::max(per_row, 1e-9)));
+                size_t before = items.size();
                 make_items(r.first, n, chunk, items, r.second);
+                for (size_t i = before; i < items.size(); i++) iw.push_back(per_row * (items[i].end - items[i].begin));
             }
         } else {
             for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
         }
-        slots_arena = std::make_shared<DevBuf>();
+        slots_arena = std::make_shared<DevBuf>(&c->pool);
         if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "slots arena");
         if (!arena_bits.ensure(std::max<uint64_t>(bits_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "bits arena");
-        if (!d_desc.ensure(m * sizeof(CandDesc)) || !d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(WorkItem)) ||
-            !d_queue.ensure(16))
+        if (!d_desc.ensure(m * sizeof(CandDesc)) || !d_queue.ensure(64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), m * sizeof(CandDesc), hipMemcpyHostToDevice, s));
-        COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+        {
+            int rc0 = upload_items(items, pred ? &iw : nullptr);
+            if (rc0) return rc0;
+        }
         auto t0 = Clock::now();
         int rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
         if (rc) return rc;
         size_t e0 = mark();
-        rc = count_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), items.size(), k, slots_arena->as<uint32_t>(),
-                         arena_bits.as<uint32_t>(), d_queue.as<uint32_t>(), s);
+        rc = count_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
+                         slots_arena->as<uint32_t>(), arena_bits.as<uint32_t>(), d_queue.as<uint32_t>(), s);
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Count : kCount);
         if (pred) {
@@ -489,7 +540,7 @@ struct Miner {
             for (uint64_t u = 0; u < m; u++) words += std::expm1((double)std::max(0.f, (*pred)[u])) * (k + (k == 2 ? 1 : 0));
             words *= 1.10;
             if (words < 4e10) {
-                arena_reserved_ = std::make_shared<DevBuf>();
+                arena_reserved_ = std::make_shared<DevBuf>(&c->pool);
                 if (!arena_reserved_->ensure((size_t)std::max(words, 1.0) * 4)) arena_reserved_.reset();
                 else st.arena_bytes += (double)arena_reserved_->bytes;
             }
@@ -503,7 +554,7 @@ struct Miner {
         }
         st.launches += 2;
         // rows per candidate: one scan over the whole slots arena
-        arena_offs = std::make_shared<DevBuf>();
+        arena_offs = std::make_shared<DevBuf>(&c->pool);
         if (!arena_offs->ensure((slots_tot + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
         rc = scan_batch(c, slots_arena->as<uint32_t>(), slots_tot, arena_offs->as<uint64_t>(), s);
         if (rc) return rc;
@@ -587,7 +638,7 @@ struct Miner {
             arena = arena_reserved_;      // pre-sized before the count pass
         } else {
             if (arena_reserved_) st.overflows++;  // prediction too small: exact size
-            arena = std::make_shared<DevBuf>();
+            arena = std::make_shared<DevBuf>(&c->pool);
             if (!arena->ensure(std::max<uint64_t>(words, 1) * 4)) return set_error(c, COLO_ENOMEM, "table arena");
         }
         arena_reserved_.reset();
@@ -619,12 +670,15 @@ struct Miner {
         }
         if (desc.empty()) return COLO_OK;
         for (auto &r : parent_runs(desc, k)) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
-        if (!d_desc.ensure(desc.size() * sizeof(CandDesc)) || !d_items.ensure(items.size() * sizeof(WorkItem)))
+        if (!d_desc.ensure(desc.size() * sizeof(CandDesc)) || !d_queue.ensure(64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
-        COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+        {
+            int rc0 = upload_items(items, nullptr);
+            if (rc0) return rc0;
+        }
         size_t e0 = mark();
-        int rc = write_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), items.size(), k,
+        int rc = write_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
                              L[0].slots ? L[0].slots->as<uint32_t>() : nullptr, d_queue.as<uint32_t>(), s);
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Write : kWrite);
@@ -755,7 +809,7 @@ struct Miner {
         uint32_t k = 2;
         uint64_t total_prev = 0;
         uint32_t levels = 0;
-        DevBuf d_cands, d_counts, d_pass;
+        DevBuf d_cands(&c->pool), d_counts(&c->pool), d_pass(&c->pool);
         while (!C.empty() && k <= F && (o.max_k == 0 || k <= o.max_k)) {
             levels++;
             const uint64_t nC = C.size() / k;
@@ -790,6 +844,7 @@ struct Miner {
             const bool learned = o.sched == COLO_SCHED_LEARNED && o.model;
             std::vector<uint64_t> fidx;
             std::vector<float> FX, pred;
+            auto tm = Clock::now();
             if (learned || o.log_path) {
                 int rc0 = level_features(k, C, pass, fcnt, Lprev, fidx, FX);
                 if (rc0) return rc0;
@@ -798,8 +853,8 @@ struct Miner {
                 pred.resize(fidx.size());
                 int rc0 = mlp_predict_host(o.model, FX.data(), (uint32_t)fidx.size(), pred.data());
                 if (rc0) return set_error(c, rc0, "cost model prediction");
-
             }
+            st.model_ms += ms_since(tm);
             int rc = o.sched == COLO_SCHED_PLAIN
                          ? run_level_plain(k, C, pass, Lprev, res, L)
                          : run_level_persistent(k, C, pass, Lprev, res, L, arena, learned ? &pred : nullptr);
@@ -1053,6 +1108,7 @@ int colo_mine(colo_ctx *c, const colo_mine_opts *opts, char **report, colo_mine_
     if (report) *report = nullptr;
     Miner M(c, *opts);
     int rc = M.run(report);
+    (void)hipStreamSynchronize(c->stream);  // before buffers return to the pool
     if (stats) *stats = M.st;
     return rc;
 }

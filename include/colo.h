@@ -216,6 +216,7 @@ typedef struct colo_mine_stats {
      * exact allocation was taken instead), and arena bytes reserved */
     double model_abs_err, model_n, arena_bytes;
     uint64_t overflows, arenas;
+    double model_ms;        /* features + prediction (learned schedule, logging) */
 } colo_mine_stats;
 
 /* ------------------------------------------------------- cost model */

@@ -1104,6 +1104,57 @@ int colo::write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_ite
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "write_batch launch");
 }
 
+// Per-candidate row totals: sums of the slot segments, one chunk of
+// kSumChunk slots per workgroup, one 64-bit atomic per workgroup.
+constexpr uint64_t kSumChunk = 1u << 16;
+
+__global__ __launch_bounds__(kBlock) void k_seg_sum(const uint32_t *slots, const uint64_t *seg_off,
+                                                    const uint64_t *chunk_prefix, uint64_t nseg,
+                                                    unsigned long long *out) {
+    uint64_t b = blockIdx.x;
+    uint64_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (chunk_prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    const uint64_t s0 = seg_off[2 * lo], len = seg_off[2 * lo + 1];
+    const uint64_t c0 = (b - chunk_prefix[lo]) * kSumChunk;
+    const uint64_t c1 = c0 + kSumChunk < len ? c0 + kSumChunk : len;
+    uint64_t acc = 0;
+    for (uint64_t i = c0 + threadIdx.x; i < c1; i += kBlock) acc += slots[s0 + i];
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(acc, ex);
+    if (threadIdx.x == 0 && tot) atomicAdd(&out[lo], (unsigned long long)tot);
+}
+
+// Sums segments (offset, length) of the slots arena into out (device, nseg
+// u64).  seg: 2*nseg host values.
+int colo::seg_sums(colo_ctx *c, const uint32_t *d_slots, const uint64_t *seg, uint64_t nseg, uint64_t *d_out,
+                   hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (!nseg) return COLO_OK;
+    std::vector<uint64_t> prefix(nseg);
+    uint64_t chunks = 0;
+    for (uint64_t i = 0; i < nseg; i++) {
+        prefix[i] = chunks;
+        chunks += (seg[2 * i + 1] + kSumChunk - 1) / kSumChunk;
+    }
+    int rc = fill_u32((uint32_t *)d_out, 2 * nseg, 0, s);
+    if (rc) return rc;
+    if (!chunks) return COLO_OK;
+    void *tmp;
+    if ((rc = scratch(c, nseg * 24 + 64, &tmp))) return rc;
+    uint64_t *d_seg = (uint64_t *)tmp, *d_pre = d_seg + 2 * nseg;
+    COLO_HIP(hipMemcpyAsync(d_seg, seg, nseg * 16, hipMemcpyHostToDevice, s));
+    COLO_HIP(hipMemcpyAsync(d_pre, prefix.data(), nseg * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_seg_sum, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_slots, d_seg, d_pre, nseg,
+                       (unsigned long long *)d_out);
+    COLO_HIP(hipGetLastError());
+    // the segment arrays live in scratch: finish before its next use
+    COLO_HIP(hipStreamSynchronize(s));
+    return COLO_OK;
+}
+
 // -------------------------------------------------------------------- PI
 // One workgroup per bitmap segment: popcount reduction plus the two
 // boundary bits the reference's off-by-one windows need (cu:113-124).

@@ -89,6 +89,7 @@ struct SegOut {
 struct DevPool {
     std::multimap<size_t, void *> free_blocks;
     size_t cached = 0;
+    static constexpr size_t kCap = 48ull << 30;  // bytes kept cached at most
     static size_t round(size_t b) {
         if (b >= (2u << 20)) return (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
         size_t r = 256;
@@ -119,6 +120,10 @@ struct DevPool {
     }
     void free(void *p, size_t bytes) {
         if (!p) return;
+        if (cached + bytes > kCap) {
+            (void)hipFree(p);
+            return;
+        }
         free_blocks.emplace(bytes, p);
         cached += bytes;
     }
@@ -186,6 +191,8 @@ int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uin
 int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t n, uint64_t *h, hipStream_t s);
 int gather_u64(colo_ctx *c, const uint64_t *d_a, const uint64_t *d_idx, uint64_t m, uint64_t *d_out, hipStream_t s);
 int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s);
+int seg_sums(colo_ctx *c, const uint32_t *d_slots, const uint64_t *seg, uint64_t nseg, uint64_t *d_out,
+             hipStream_t s);
 // Segment descriptors of one candidate's bitmap (bits_off = base word).
 void seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off,
                 uint64_t *seg_off, uint64_t *words_total);

@@ -553,27 +553,18 @@ struct Miner {
             st.count_launches++;
         }
         st.launches += 2;
-        // rows per candidate: one scan over the whole slots arena
-        arena_offs = std::make_shared<DevBuf>(&c->pool);
-        if (!arena_offs->ensure((slots_tot + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
-        rc = scan_batch(c, slots_arena->as<uint32_t>(), slots_tot, arena_offs->as<uint64_t>(), s);
-        if (rc) return rc;
-        std::vector<uint64_t> bounds(m + 1);
-        for (uint64_t u = 0; u < m; u++) bounds[u] = desc[u].slots_off;
-        bounds[m] = slots_tot;
-        // offset at position slots_tot = total: computed as the last offset + last slot
-        if (!d_idx.ensure((m + 1) * 8) || !d_rowsel.ensure((m + 2) * 8)) return set_error(c, COLO_ENOMEM, "idx");
-        COLO_HIP(hipMemcpyAsync(d_idx.p, bounds.data(), m * 8, hipMemcpyHostToDevice, s));
-        rc = gather_u64(c, arena_offs->as<uint64_t>(), d_idx.as<uint64_t>(), m, d_rowsel.as<uint64_t>(), s);
-        if (rc) return rc;
-        std::vector<uint64_t> starts(m + 1, 0);
-        uint32_t last_slot = 0;
-        uint64_t last_off = 0;
-        COLO_HIP(hipMemcpyAsync(starts.data(), d_rowsel.p, m * 8, hipMemcpyDeviceToHost, s));
-        if (slots_tot) {
-            COLO_HIP(hipMemcpyAsync(&last_slot, slots_arena->as<uint32_t>() + slots_tot - 1, 4, hipMemcpyDeviceToHost, s));
-            COLO_HIP(hipMemcpyAsync(&last_off, arena_offs->as<uint64_t>() + slots_tot - 1, 8, hipMemcpyDeviceToHost, s));
+        // rows per candidate: segmented sums of the slots arena (offsets are
+        // built later, only for the tables that are written)
+        std::vector<uint64_t> sseg(2 * m);
+        for (uint64_t u = 0; u < m; u++) {
+            sseg[2 * u] = desc[u].slots_off;
+            sseg[2 * u + 1] = desc[u].n_anchor;
         }
+        if (!d_rowsel.ensure((m + 1) * 8)) return set_error(c, COLO_ENOMEM, "row totals");
+        rc = seg_sums(c, slots_arena->as<uint32_t>(), sseg.data(), m, d_rowsel.as<uint64_t>(), s);
+        if (rc) return rc;
+        std::vector<uint64_t> rows_of(m, 0);
+        COLO_HIP(hipMemcpyAsync(rows_of.data(), d_rowsel.p, m * 8, hipMemcpyDeviceToHost, s));
         // PI segments of every candidate, one launch
         std::vector<SegDesc> segs(m * k);
         for (uint64_t u = 0; u < m; u++) seg_descs(c, k, &C[idx[u] * k], desc[u].bits_off, &segs[u * k]);
@@ -589,12 +580,11 @@ struct Miner {
         if ((rc = sync())) return rc;
         st.count_ms += ms_since(t0);
         st.launches += 4;
-        starts[m] = slots_tot ? last_off + last_slot : 0;
         for (uint64_t u = 0; u < m; u++) {
             const uint32_t *cand = &C[idx[u] * k];
             CandRes &r = res[idx[u]];
             r.pass = true;
-            r.rows = starts[u + 1] - starts[u];
+            r.rows = rows_of[u];
             pi_from_segs(c, k, cand, &so[u * k], r.numc, r.nume, &r.pic, &r.pie);
             if (k == 2) st.l2_pairs += r.rows;
             float pim = o.pi_mode ? r.pie : r.pic;
@@ -610,12 +600,12 @@ struct Miner {
                 L.push_back(std::move(p));
             }
         }
-        level_starts_ = std::move(starts);
+
         level_desc_ = std::move(desc);
         level_idx_ = std::move(idx);
         return COLO_OK;
     }
-    std::vector<uint64_t> level_starts_;
+
     std::vector<CandDesc> level_desc_;
     std::vector<uint64_t> level_idx_;
 
@@ -633,6 +623,32 @@ struct Miner {
         uint64_t words = 0;
         for (size_t i = 0; i < L.size(); i++)
             if (build[i]) words += L[i].rows * (k + (k == 2 ? 1 : 0));
+        // row offsets of the written tables: one exclusive scan per table
+        // into one offsets arena (n_anchor + 1 entries each, the total last)
+        uint64_t owords = 0;
+        for (size_t i = 0; i < L.size(); i++)
+            if (build[i]) owords += L[i].nslots + 1;
+        arena_offs = std::make_shared<DevBuf>(&c->pool);
+        if (!arena_offs->ensure(std::max<uint64_t>(owords, 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
+        std::vector<uint64_t> offs_at(L.size(), 0);
+        {
+            // size the scan temporaries once for the largest table
+            uint64_t mx = 0;
+            for (size_t i = 0; i < L.size(); i++)
+                if (build[i]) mx = std::max(mx, L[i].nslots);
+            void *tmp;
+            int rcs = scratch(c, 8 * (mx / 1024 + 64) + 4096, &tmp);
+            if (rcs) return rcs;
+            uint64_t oat = 0;
+            for (size_t i = 0; i < L.size(); i++) {
+                if (!build[i]) continue;
+                offs_at[i] = oat;
+                int rc0 = scan_batch(c, L[i].slots->as<uint32_t>() + L[i].slots_off, L[i].nslots,
+                                     arena_offs->as<uint64_t>() + oat, s);
+                if (rc0) return rc0;
+                oat += L[i].nslots + 1;
+            }
+        }
         std::shared_ptr<DevBuf> arena;
         if (arena_reserved_ && arena_reserved_->bytes >= words * 4) {
             arena = arena_reserved_;      // pre-sized before the count pass
@@ -653,11 +669,11 @@ struct Miner {
             p.tptr = arena->as<uint32_t>() + at;
             at += p.rows * k;
             CandDesc d = level_desc_[u];
-            d.offsets = arena_offs->as<uint64_t>() + d.slots_off;
-            d.off_base = level_starts_[u];
+            d.offsets = arena_offs->as<uint64_t>() + offs_at[i];
+            d.off_base = 0;
             p.offs = arena_offs;
-            p.offs_at = d.slots_off;
-            p.off_base = d.off_base;
+            p.offs_at = offs_at[i];
+            p.off_base = 0;
             if (k == 2) {
                 p.nbr = arena;
                 p.nptr = arena->as<uint32_t>() + at;

@@ -66,7 +66,7 @@ _lib = None
 SYMBOLS = [
     "colo_ctx_create", "colo_ctx_destroy", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
-    "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_participation", "colo_mine", "colo_free_str",
+    "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_mine_sweep", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
     "colo_candidates", "colo_mlp_create", "colo_mlp_destroy", "colo_mlp_set_params", "colo_mlp_get_params",
     "colo_mlp_set_norm", "colo_mlp_predict", "colo_mlp_grad", "colo_mlp_train", "colo_mlp_save", "colo_mlp_load",
@@ -101,6 +101,7 @@ def lib():
         L.colo_pi.argtypes = [vp, vp, u32, vp, vp, vp, C.POINTER(f32), C.POINTER(f32), vp]
         L.colo_participation.argtypes = [vp, vp, u32, vp, vp, vp]
         L.colo_mine.argtypes = [vp, C.POINTER(MineOpts), C.POINTER(vp), C.POINTER(MineStats)]
+        L.colo_mine_sweep.argtypes = [vp, C.POINTER(MineOpts), vp, u32, vp, vp]
         L.colo_free_str.argtypes = [vp]
         L.colo_run_config.argtypes = [C.c_char_p, C.POINTER(MineOpts), C.POINTER(vp)]
         L.colo_load_csv.argtypes = [C.c_char_p, C.POINTER(vp)]
@@ -390,6 +391,32 @@ class Context:
         self._check(rc)
         return text, st.as_dict()
 
+    @_on_stream
+    def mine_sweep(self, thetas, max_k=0, pi_mode=0, filter_on=True, sched=COLO_SCHED_PERSISTENT, hash_tables=True,
+                   out_prefix=None, model=None):
+        """Mines every theta in one call, reusing theta-independent results
+        (colo_mine_sweep); returns [(report, stats)] in the given order."""
+        o = _opts(0.0, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, None)
+        n = len(thetas)
+        th = np.ascontiguousarray(thetas, np.float32)
+        reps = (C.c_void_p * n)()
+        sts = (MineStats * n)()
+        rc = lib().colo_mine_sweep(self.h, C.byref(o), th.ctypes.data, n, C.cast(reps, C.c_void_p),
+                                   C.cast(sts, C.c_void_p))
+        out = []
+        for i in range(n):
+            text = None
+            if reps[i]:
+                text = C.string_at(reps[i]).decode()
+                lib().colo_free_str(reps[i])
+            out.append((text, sts[i].as_dict()))
+        try:
+            self._check(rc)
+        except ColoError as e:
+            e.results = out
+            raise
+        return out
+
 
 MLP_IN, MLP_HIDDEN = 16, 64
 MLP_NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
@@ -398,6 +425,12 @@ FEATURE_NAMES = ["k", "log1p N_first", "log1p N_last", "log1p N_min", "log1p N_m
                  "log1p 3x3 work(first,last)", "log1p R2(first,last)", "log1p R2(prev,last)",
                  "log occupancy_first", "log occupancy_last"]
 RECORD_FLOATS = MLP_IN + 2  # features, log1p(rows), k
+
+
+def _opts(theta, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, log_path):
+    return MineOpts(C.c_float(theta), max_k, pi_mode, int(filter_on), sched, int(hash_tables),
+                    out_prefix.encode() if out_prefix else None, model.h if model is not None else None,
+                    log_path.encode() if log_path else None)
 
 
 def read_log(path):

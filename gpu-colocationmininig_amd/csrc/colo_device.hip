@@ -1020,8 +1020,9 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
     const uint32_t *row = c0.parent + i * km1;
     uint32_t ids[kMaxK];
     float mx[kMaxK], my[kMaxK];
+    auto slot_of = [&](const CandDesc &cd) { return cd.wslots ? cd.wslots[i] : slots[cd.slots_off + i]; };
     bool any = false;
-    for (uint32_t j = 0; j < ncand; j++) any |= slots[descs[j].slots_off + i] != 0;
+    for (uint32_t j = 0; j < ncand; j++) any |= slot_of(descs[j]) != 0;
     if (!any) return;
     for (uint32_t m = 0; m < km1; m++) {
         ids[m] = row[m];
@@ -1033,7 +1034,7 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
     const uint32_t t = ids[0] - D.fstart[c0.f[0]];
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
-        const uint32_t nrows = slots[cd.slots_off + i];
+        const uint32_t nrows = slot_of(cd);
         if (!nrows) continue;
         const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
         uint32_t *out = cd.table + (cd.offsets[i] - cd.off_base) * k;
@@ -1059,7 +1060,7 @@ __global__ __launch_bounds__(kBlock) void k_write_batch(DevData D, const CandDes
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         const CandDesc &cd = desc[wi.cand];
-        const uint32_t *sl = slots + cd.slots_off;
+        const uint32_t *sl = cd.wslots ? cd.wslots : slots + cd.slots_off;
         if (cd.k == 2) {
             for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
                 uint32_t nr = sl[t];

@@ -58,6 +58,7 @@ struct CandDesc {
     const uint64_t *nb_off;
     uint64_t nb_base;
     uint32_t *nbr_out;             // level-2 write pass: neighbour-list output (optional)
+    const uint32_t *wslots;        // write pass: this table's slot counts (n_anchor)
 };
 
 // One work item of a batched launch: anchors / parent rows [begin, end) of

@@ -170,6 +170,21 @@ struct Prev {
     uint64_t off_base = 0;          // value of the first offset
 };
 
+// Results that do not depend on theta, kept across the runs of a threshold
+// sweep (ascending theta): every candidate's counted rows, numerators and
+// PIs, and the level-2 tables (the prevalent pairs at a higher theta are a
+// subset of those at a lower one).
+struct SweepMemo {
+    struct Res {
+        uint64_t rows;
+        uint64_t numc[kMaxK], nume[kMaxK];
+        float pic, pie;
+    };
+    std::map<std::vector<uint32_t>, Res> res;
+    std::map<std::pair<uint32_t, uint32_t>, Prev> l2;
+    uint64_t hits = 0, misses = 0;
+};
+
 struct Miner {
     colo_ctx *c;
     const colo_mine_opts &o;
@@ -177,6 +192,8 @@ struct Miner {
     hipStream_t s;
     Str rep;
     colo_ctx *ctx_for_err;
+
+    SweepMemo *memo = nullptr;
 
     Miner(colo_ctx *c_, const colo_mine_opts &o_) : c(c_), o(o_), s(c_->stream), ctx_for_err(c_) {
         for (DevBuf *b : {&arena_slots, &arena_bits, &d_desc, &d_items, &d_segs, &d_segout, &d_queue, &d_idx, &d_rowsel})
@@ -239,6 +256,19 @@ struct Miner {
         return 4.0 * (k - 1) * n_anchor + 12.0 * n_anchor + 8.0 * c->fcount[cand[k - 1]] + 4.0 * k * rows;
     }
 
+    // Out-of-memory error naming the buffer, the request and the device's
+    // free memory.
+    uint32_t cur_k_ = 0;
+    int oom(const char *what, double bytes) {
+        size_t fr = 0, tot = 0;
+        (void)hipMemGetInfo(&fr, &tot);
+        (void)hipGetLastError();
+        char b[256];
+        snprintf(b, sizeof b, "%s: %.3f GB requested at level %u; device free %.3f of %.3f GB, pool cache %.3f GB", what,
+                 bytes / 1e9, cur_k_, fr / 1e9, tot / 1e9, c->pool.cached / 1e9);
+        return set_error(c, COLO_ENOMEM, b);
+    }
+
     int sync() {
         COLO_HIP(hipStreamSynchronize(s));
         harvest();
@@ -285,7 +315,7 @@ struct Miner {
         uint64_t max_words = 0;
         for (uint64_t i = 0; i < nC; i++)
             if (pass[i]) max_words = std::max(max_words, colo_bits_words(c, k, &C[i * k]));
-        if (!bits.ensure(max_words * 4)) return set_error(c, COLO_ENOMEM, "bits buffer");
+        if (!bits.ensure(max_words * 4)) return oom("bits buffer", max_words * 4);
         for (uint64_t i = 0; i < nC; i++) {
             if (!pass[i]) continue;
             const uint32_t *cand = &C[i * k];
@@ -298,13 +328,13 @@ struct Miner {
             int rc;
             if (k == 2) {
                 nslots = c->fcount[cand[0]];
-                if (!slots->ensure(nslots * 4)) return set_error(c, COLO_ENOMEM, "slots");
+                if (!slots->ensure(nslots * 4)) return oom("slots", nslots * 4);
                 rc = colo_join2_count(c, cand[0], cand[1], slots->as<uint32_t>(), bits.as<uint32_t>(), &r.rows, s);
             } else {
                 parent = find(Lprev, cand, k - 1);
                 if (parent < 0 || !Lprev[parent].tptr) return set_error(c, COLO_EARG, "missing parent table");
                 nslots = Lprev[parent].rows;
-                if (!slots->ensure(nslots * 4)) return set_error(c, COLO_ENOMEM, "slots");
+                if (!slots->ensure(nslots * 4)) return oom("slots", nslots * 4);
                 const Prev *nb = nbr_list(cand[0], cand[k - 1]);
                 if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
                 rc = colo_joink_count_nbr(c, Lprev[parent].tptr, nslots, k, cand,
@@ -342,7 +372,7 @@ struct Miner {
             Prev &p = L[i];
             auto t0 = Clock::now();
             p.offs = std::make_shared<DevBuf>(&c->pool);
-            if (!p.offs->ensure((p.nslots + 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets");
+            if (!p.offs->ensure((p.nslots + 1) * 8)) return oom("offsets", (p.nslots + 1) * 8);
             p.offs_at = 0;
             p.off_base = 0;
             uint64_t total = 0;
@@ -351,11 +381,11 @@ struct Miner {
             st.scan_ms += ms_since(t0);
             auto t1 = Clock::now();
             p.table = std::make_shared<DevBuf>(&c->pool);
-            if (!p.table->ensure(std::max<uint64_t>(total * k, 1) * 4)) return set_error(c, COLO_ENOMEM, "table");
+            if (!p.table->ensure(std::max<uint64_t>(total * k, 1) * 4)) return oom("table", std::max<uint64_t>(total * k, 1) * 4);
             p.tptr = p.table->as<uint32_t>();
             if (k == 2) {
                 p.nbr = std::make_shared<DevBuf>(&c->pool);
-                if (!p.nbr->ensure(std::max<uint64_t>(total, 1) * 4)) return set_error(c, COLO_ENOMEM, "neighbour list");
+                if (!p.nbr->ensure(std::max<uint64_t>(total, 1) * 4)) return oom("neighbour list", std::max<uint64_t>(total, 1) * 4);
                 p.nptr = p.nbr->as<uint32_t>();
                 rc = colo_join2_write(c, p.pat[0], p.pat[1], p.slots->as<uint32_t>() + p.slots_off, p.offs->as<uint64_t>(),
                                       p.tptr, p.nptr, s);
@@ -418,7 +448,7 @@ struct Miner {
     int upload_items(const std::vector<WorkItem> &items, const std::vector<double> *w) {
         std::vector<uint64_t> seg = xcd_segments(items, w);
         size_t ib = items.size() * sizeof(WorkItem);
-        if (!d_items.ensure(ib + 9 * 8)) return set_error(c, COLO_ENOMEM, "work items");
+        if (!d_items.ensure(ib + 9 * 8)) return oom("work items", ib + 9 * 8);
         COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), ib, hipMemcpyHostToDevice, s));
         COLO_HIP(hipMemcpyAsync((char *)d_items.p + ib, seg.data(), 9 * 8, hipMemcpyHostToDevice, s));
         d_seg_ = (const uint64_t *)((const char *)d_items.p + ib);
@@ -516,8 +546,8 @@ struct Miner {
             for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
         }
         slots_arena = std::make_shared<DevBuf>(&c->pool);
-        if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "slots arena");
-        if (!arena_bits.ensure(std::max<uint64_t>(bits_tot, 1) * 4)) return set_error(c, COLO_ENOMEM, "bits arena");
+        if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return oom("slots arena", std::max<uint64_t>(slots_tot, 1) * 4);
+        if (!arena_bits.ensure(std::max<uint64_t>(bits_tot, 1) * 4)) return oom("bits arena", std::max<uint64_t>(bits_tot, 1) * 4);
         if (!d_desc.ensure(m * sizeof(CandDesc)) || !d_queue.ensure(64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), m * sizeof(CandDesc), hipMemcpyHostToDevice, s));
@@ -560,7 +590,7 @@ struct Miner {
             sseg[2 * u] = desc[u].slots_off;
             sseg[2 * u + 1] = desc[u].n_anchor;
         }
-        if (!d_rowsel.ensure((m + 1) * 8)) return set_error(c, COLO_ENOMEM, "row totals");
+        if (!d_rowsel.ensure((m + 1) * 8)) return oom("row totals", (m + 1) * 8);
         rc = seg_sums(c, slots_arena->as<uint32_t>(), sseg.data(), m, d_rowsel.as<uint64_t>(), s);
         if (rc) return rc;
         std::vector<uint64_t> rows_of(m, 0);
@@ -615,9 +645,7 @@ struct Miner {
         std::vector<CandDesc> desc;
         std::vector<WorkItem> items;
         auto t0 = Clock::now();
-        std::vector<uint64_t> cand_of_prev(L.size(), UINT64_MAX);
-        for (uint64_t u = 0, pi = 0; u < level_idx_.size() && pi < L.size(); u++)
-            if (std::equal(L[pi].pat.begin(), L[pi].pat.end(), &C[level_idx_[u] * k])) cand_of_prev[pi++] = u;
+        (void)C;
         // one table arena for the level: tables (k words per row) and, at
         // level 2, the neighbour lists (1 word per row)
         uint64_t words = 0;
@@ -629,7 +657,7 @@ struct Miner {
         for (size_t i = 0; i < L.size(); i++)
             if (build[i]) owords += L[i].nslots + 1;
         arena_offs = std::make_shared<DevBuf>(&c->pool);
-        if (!arena_offs->ensure(std::max<uint64_t>(owords, 1) * 8)) return set_error(c, COLO_ENOMEM, "offsets arena");
+        if (!arena_offs->ensure(std::max<uint64_t>(owords, 1) * 8)) return oom("offsets arena", std::max<uint64_t>(owords, 1) * 8);
         std::vector<uint64_t> offs_at(L.size(), 0);
         {
             // size the scan temporaries once for the largest table
@@ -655,7 +683,7 @@ struct Miner {
         } else {
             if (arena_reserved_) st.overflows++;  // prediction too small: exact size
             arena = std::make_shared<DevBuf>(&c->pool);
-            if (!arena->ensure(std::max<uint64_t>(words, 1) * 4)) return set_error(c, COLO_ENOMEM, "table arena");
+            if (!arena->ensure(std::max<uint64_t>(words, 1) * 4)) return oom("table arena", std::max<uint64_t>(words, 1) * 4);
         }
         arena_reserved_.reset();
         st.arenas++;
@@ -663,12 +691,23 @@ struct Miner {
         for (size_t i = 0; i < L.size(); i++) {
             if (!build[i]) continue;
             Prev &p = L[i];
-            uint64_t u = cand_of_prev[i];
-            if (u == UINT64_MAX) return set_error(c, COLO_EARG, "internal: prevalent pattern not in level");
             p.table = arena;
             p.tptr = arena->as<uint32_t>() + at;
             at += p.rows * k;
-            CandDesc d = level_desc_[u];
+            CandDesc d;
+            memset(&d, 0, sizeof d);
+            d.k = k;
+            for (uint32_t j = 0; j < k; j++) d.f[j] = p.pat[j];
+            d.n_anchor = p.nslots;
+            d.wslots = p.slots->as<uint32_t>() + p.slots_off;
+            if (k > 2) {
+                d.parent = Lprev[p.parent].tptr;
+                const Prev *nb = nbr_list(p.pat[0], p.pat[k - 1]);
+                if (!nb) return set_error(c, COLO_EARG, "internal: missing neighbour list");
+                d.nb_table = nb->nptr;
+                d.nb_off = nb->offs->as<uint64_t>() + nb->offs_at;
+                d.nb_base = nb->off_base;
+            }
             d.offsets = arena_offs->as<uint64_t>() + offs_at[i];
             d.off_base = 0;
             p.offs = arena_offs;
@@ -694,8 +733,8 @@ struct Miner {
             if (rc0) return rc0;
         }
         size_t e0 = mark();
-        int rc = write_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
-                             L[0].slots ? L[0].slots->as<uint32_t>() : nullptr, d_queue.as<uint32_t>(), s);
+        int rc = write_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(), nullptr,
+                             d_queue.as<uint32_t>(), s);
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Write : kWrite);
         {
@@ -828,6 +867,7 @@ struct Miner {
         DevBuf d_cands(&c->pool), d_counts(&c->pool), d_pass(&c->pool);
         while (!C.empty() && k <= F && (o.max_k == 0 || k <= o.max_k)) {
             levels++;
+            cur_k_ = k;
             const uint64_t nC = C.size() / k;
             st.candidates += nC;
             // filter (cpp:297, 760)
@@ -855,6 +895,28 @@ struct Miner {
             std::vector<Prev> L;
             std::shared_ptr<DevBuf> arena;
             auto tl = Clock::now();
+            // candidates to count: those passing the filter whose results are
+            // not known from an earlier run of a sweep
+            std::vector<uint8_t> todo = pass;
+            if (memo)
+                for (uint64_t i = 0; i < nC; i++) {
+                    if (!pass[i]) continue;
+                    auto it = memo->res.find(std::vector<uint32_t>(&C[i * k], &C[i * k] + k));
+                    if (it == memo->res.end()) {
+                        memo->misses++;
+                        continue;
+                    }
+                    memo->hits++;
+                    todo[i] = 0;
+                    CandRes &r = res[i];
+                    r.pass = true;
+                    r.rows = it->second.rows;
+                    memcpy(r.numc, it->second.numc, sizeof r.numc);
+                    memcpy(r.nume, it->second.nume, sizeof r.nume);
+                    r.pic = it->second.pic;
+                    r.pie = it->second.pie;
+                    r.prevalent = !((o.pi_mode ? r.pie : r.pic) < o.theta);
+                }
             // cost-model features of the passing candidates (logging and the
             // learned schedule); predictions of log1p(rows)
             const bool learned = o.sched == COLO_SCHED_LEARNED && o.model;
@@ -862,7 +924,7 @@ struct Miner {
             std::vector<float> FX, pred;
             auto tm = Clock::now();
             if (learned || o.log_path) {
-                int rc0 = level_features(k, C, pass, fcnt, Lprev, fidx, FX);
+                int rc0 = level_features(k, C, todo, fcnt, Lprev, fidx, FX);
                 if (rc0) return rc0;
             }
             if (learned && !fidx.empty()) {
@@ -872,9 +934,44 @@ struct Miner {
             }
             st.model_ms += ms_since(tm);
             int rc = o.sched == COLO_SCHED_PLAIN
-                         ? run_level_plain(k, C, pass, Lprev, res, L)
-                         : run_level_persistent(k, C, pass, Lprev, res, L, arena, learned ? &pred : nullptr);
+                         ? run_level_plain(k, C, todo, Lprev, res, L)
+                         : run_level_persistent(k, C, todo, Lprev, res, L, arena, learned ? &pred : nullptr);
             if (rc) return rc;
+            if (memo) {
+                for (uint64_t i = 0; i < nC; i++)
+                    if (todo[i]) {
+                        SweepMemo::Res m{};
+                        m.rows = res[i].rows;
+                        memcpy(m.numc, res[i].numc, sizeof m.numc);
+                        memcpy(m.nume, res[i].nume, sizeof m.nume);
+                        m.pic = res[i].pic;
+                        m.pie = res[i].pie;
+                        memo->res.emplace(std::vector<uint32_t>(&C[i * k], &C[i * k] + k), m);
+                    }
+                // the prevalent list in candidate order: counted ones carry
+                // their slots, remembered ones are added without
+                std::vector<Prev> all;
+                size_t j = 0;
+                for (uint64_t i = 0; i < nC; i++) {
+                    if (!res[i].prevalent) continue;
+                    if (todo[i]) {
+                        all.push_back(std::move(L[j++]));
+                        continue;
+                    }
+                    Prev p;
+                    p.pat.assign(&C[i * k], &C[i * k] + k);
+                    p.rows = res[i].rows;
+                    if (k == 2) {
+                        p.nslots = c->fcount[C[i * k]];
+                    } else {
+                        p.parent = find(Lprev, &C[i * k], k - 1);
+                        if (p.parent < 0) return set_error(c, COLO_EARG, "internal: missing parent");
+                        p.nslots = Lprev[p.parent].rows;
+                    }
+                    all.push_back(std::move(p));
+                }
+                L = std::move(all);
+            }
             double level_count_ms = ms_since(tl);
             if (learned)
                 for (size_t u = 0; u < fidx.size(); u++) {
@@ -932,6 +1029,46 @@ struct Miner {
                 for (auto &b : build) b = 1;
             st.host_ms += ms_since(th);
             auto tt = Clock::now();
+            if (memo) {
+                // level-2 tables from earlier runs
+                if (k == 2)
+                    for (size_t i = 0; i < L.size(); i++) {
+                        if (!build[i]) continue;
+                        auto it = memo->l2.find({L[i].pat[0], L[i].pat[1]});
+                        if (it == memo->l2.end()) continue;
+                        L[i] = it->second;
+                        build[i] = 0;
+                    }
+                // tables to build whose slots were not counted in this run
+                std::vector<uint8_t> again(nC, 0);
+                bool any = false;
+                for (size_t i = 0; i < L.size(); i++)
+                    if (build[i] && !L[i].slots) {
+                        for (uint64_t ci = 0; ci < nC; ci++)
+                            if (std::equal(L[i].pat.begin(), L[i].pat.end(), &C[ci * k])) {
+                                again[ci] = 1;
+                                any = true;
+                                break;
+                            }
+                    }
+                if (any) {
+                    std::vector<CandRes> res2(nC);
+                    std::vector<Prev> L2;
+                    std::shared_ptr<DevBuf> arena2;
+                    rc = o.sched == COLO_SCHED_PLAIN ? run_level_plain(k, C, again, Lprev, res2, L2)
+                                                     : run_level_persistent(k, C, again, Lprev, res2, L2, arena2, nullptr);
+                    if (rc) return rc;
+                    for (Prev &q : L2)
+                        for (size_t i = 0; i < L.size(); i++)
+                            if (L[i].pat == q.pat) {
+                                L[i].slots = q.slots;
+                                L[i].slots_off = q.slots_off;
+                                L[i].nslots = q.nslots;
+                                L[i].parent = q.parent;
+                                break;
+                            }
+                }
+            }
             rc = o.sched == COLO_SCHED_PLAIN ? build_tables_plain(k, L, build, Lprev)
                                              : build_tables_persistent(k, L, build, Lprev, C);
             if (rc) return rc;
@@ -950,6 +1087,9 @@ struct Miner {
                 rep.printf("\n");
             }
             for (Prev &p : L) p.slots.reset();
+            if (memo && k == 2)
+                for (const Prev &p : L)
+                    if (p.tptr && p.nptr) memo->l2.emplace(std::make_pair(p.pat[0], p.pat[1]), p);
             if (k == 2) {
                 l2_index_.assign((size_t)c->F * c->F, -1);
                 for (size_t i = 0; i < L.size(); i++) {
@@ -1126,6 +1266,35 @@ int colo_mine(colo_ctx *c, const colo_mine_opts *opts, char **report, colo_mine_
     int rc = M.run(report);
     (void)hipStreamSynchronize(c->stream);  // before buffers return to the pool
     if (stats) *stats = M.st;
+    return rc;
+}
+
+int colo_mine_sweep(colo_ctx *c, const colo_mine_opts *opts, const float *thetas, uint32_t n, char **reports,
+                    colo_mine_stats *stats) {
+    if (!c || !opts || (!thetas && n)) return COLO_EARG;
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return thetas[a] < thetas[b]; });
+    SweepMemo memo;
+    int rc = COLO_OK;
+    for (uint32_t i : order) {
+        colo_mine_opts o = *opts;
+        o.theta = thetas[i];
+        std::string prefix;
+        if (opts->out_prefix) {
+            char b[64];
+            snprintf(b, sizeof b, "theta_%g_", (double)thetas[i]);
+            prefix = std::string(opts->out_prefix) + (n > 1 ? b : "");
+            o.out_prefix = prefix.c_str();
+        }
+        Miner M(c, o);
+        M.memo = &memo;
+        if (reports) reports[i] = nullptr;
+        rc = M.run(reports ? &reports[i] : nullptr);
+        (void)hipStreamSynchronize(c->stream);
+        if (stats) stats[i] = M.st;
+        if (rc) break;
+    }
     return rc;
 }
 

@@ -254,6 +254,16 @@ int colo_mlp_load(const char *path, colo_mlp **out);
 int colo_mine(colo_ctx *ctx, const colo_mine_opts *opts, char **report, colo_mine_stats *stats);
 void colo_free_str(char *s);
 
+/* Threshold sweep in one call (the c5 workload's theta in {0.1, 0.2, 0.3,
+ * 0.5}): runs colo_mine for each theta in ascending order, reusing what
+ * does not depend on theta -- every counted candidate's rows, numerators
+ * and PIs, and the level-2 tables -- across the runs.  Each report equals
+ * the one colo_mine would give for that theta.  With n > 1 the pattern
+ * files go to <out_prefix>theta_<theta>_ColocationDegree_<k>.txt.
+ * No counterpart in the reference (which mines one theta per run). */
+int colo_mine_sweep(colo_ctx *ctx, const colo_mine_opts *opts, const float *thetas, uint32_t n, char **reports,
+                    colo_mine_stats *stats);
+
 /* End-to-end drop-in for colocationFinder::Begin(argc, argv): reads
  * config.txt (cpp:1077-1093), parameter.txt (cpp:57-74) and the dataset CSV
  * (cpp:139-254), mines, and writes the pattern files under the config's

@@ -261,3 +261,17 @@ def test_seattle_subset(colo_mod, oracle_mod):
     ctx = colo_mod.Context.from_host(hd, 10.0)
     rep, _ = ctx.mine(0.3, max_k=3)
     assert rep == orc.mine(10.0, 0.3, max_k=3)
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+def test_threshold_sweep(c2small, sched):
+    """One sweep over several thresholds equals one mining run per threshold
+    (theta-independent results are reused across the sweep)."""
+    ctx, orc, p = c2small
+    _say(f"threshold sweep sched={sched}")
+    thetas = [0.25, 0.1, 0.15]
+    out = ctx.mine_sweep(thetas, max_k=4, sched=sched)
+    for th, (rep, st) in zip(thetas, out):
+        single, _ = ctx.mine(th, max_k=4, sched=sched)
+        assert rep == single, th
+    assert out[0][0] == orc.mine(p["r"], 0.25, max_k=4)

@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the colocation-mining hot path on one MI355X.
 
-A step is one full mining run of the workload held in HBM: grid binning
-(colo_bin) and the Apriori level loop (colo_mine: filter, neighbour-cell
-joins, PI numerators, instance tables) up to the workload's largest pattern
-size.  The default workload is c2 (500k instances, 20 features with Zipf
-frequencies, 30% of points in 200 Gaussian hotspots, r = 200, theta = 0.25,
-patterns up to size 5), generated with a fixed seed and parsed from the
-reference CSV format before the timed region.
+A step is one full mining job on the workload held in HBM: grid binning
+(colo_bin) and the Apriori level loop (filter, neighbour-cell joins, PI
+numerators, instance tables) up to the workload's largest pattern size, at
+every prevalence threshold of the workload in one colo_mine_sweep call
+(theta-independent results are computed once per step and reused).
+
+The default workload is c5: 50M instances, 60 features with Zipf(1.2)
+frequencies, 60% of the points in 2,000 Gaussian hotspots (sigma log-uniform
+in 100-2000 on a 1e5 square, 12 features per hotspot), 20 planted patterns of
+size 3-6, r for ~64 mean neighbours, patterns up to size 6, thresholds 0.1,
+0.2, 0.3 and 0.5.  The data is generated in memory with a fixed seed
+(float64 coordinates converted to float32, the reference's coordinate type)
+before the timed region.  c1/c2/c3 are available with --workload.
 
 Prints one JSON line.  value = neighbour pairs per second of the level-2
 join (pairs counted by the level-2 count pass / wall time of the level-2
@@ -22,7 +28,6 @@ import argparse
 import json
 import os
 import sys
-import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -37,32 +42,29 @@ def log(msg):
 
 
 def describe(p):
-    return (f"{p['name']}: {p['N']} instances, {p['F']} features, r={p['r']:g}, theta={p['theta']:g}, "
+    th = ", ".join(f"{t:g}" for t in p["thetas"])
+    return (f"{p['name']}: {p['N']} instances, {p['F']} features, r={p['r']:g}, theta in {{{th}}}, "
             f"patterns up to size {p['max_k']}, seed {p['seed']}")
 
 
 def prepare(args):
     import datagen
-    cache = args.cache or os.path.join(tempfile.gettempdir(), "colo_bench")
-    d = os.path.join(cache, f"{args.workload}_{args.N or 'default'}")
-    meta = os.path.join(d, "params.json")
-    if not os.path.exists(meta):
-        log(f"generating {args.workload} into {d}")
-        w = datagen.write_workload(d, args.workload, N=args.N)
-        with open(meta, "w") as fh:
-            json.dump(w["params"], fh)
-    with open(meta) as fh:
-        p = json.load(fh)
-    return os.path.join(d, "data.csv"), p
+    import numpy as np
+    t0 = time.perf_counter()
+    labels, x, y, S, p = datagen.generate(args.workload, N=args.N)
+    fc = np.bincount(labels, minlength=p["F"]).astype(np.uint32)
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    log(f"generated {describe(p)} in {time.perf_counter() - t0:.1f} s")
+    return x32, y32, fc, np.array([0, 0, S, S], np.float32), p
 
 
-def cpu_baseline(csv, p, budget_s, threads):
+def cpu_baseline(x, y, fc, bounds, p, budget_s, threads):
     """The oracle's level-2 joins (count + write) of the same workload on
     host cores, over as many candidate pairs as fit the time budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.set_threads(threads)
-    o = oracle.Oracle.from_csv(csv)
+    o = oracle.Oracle.from_arrays(x, y, fc, bounds)
     o.bin(p["r"])
     F = o.F
     pairs = [(a, b) for a in range(F) for b in range(a + 1, F)]
@@ -91,9 +93,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c5")
     ap.add_argument("--N", type=int, default=None)
-    ap.add_argument("--cache", default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -105,15 +106,12 @@ def main():
                                    "note only)", "n_gpus": args.gpus}))
         return 2
 
+    x, y, fc, bounds, p = prepare(args)
+
     import colo
     import torch
 
-    csv, p = prepare(args)
-    t0 = time.perf_counter()
-    hd = colo.HostData(csv)
-    parse_s = time.perf_counter() - t0
-    log(f"parsed {hd.n} instances / {hd.F} features in {parse_s:.2f} s")
-    ctx = colo.Context.from_host(hd, p["r"])
+    ctx = colo.Context(x, y, fc, bounds, p["r"])
     torch.cuda.synchronize()
 
     model = None
@@ -122,6 +120,7 @@ def main():
     main_sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
     sched_name = {colo.COLO_SCHED_LEARNED: "learned", colo.COLO_SCHED_PERSISTENT: "persistent",
                   colo.COLO_SCHED_PLAIN: "plain"}
+    thetas = p["thetas"]
 
     bin_ms = []
 
@@ -130,19 +129,21 @@ def main():
         ctx.rebin(p["r"])
         torch.cuda.synchronize()
         bin_ms.append((time.perf_counter() - t0) * 1000.0)
-        return ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, hash_tables=False,
-                        model=model if sched == colo.COLO_SCHED_LEARNED else None)
+        out = ctx.mine_sweep(thetas, max_k=p["max_k"], sched=sched, hash_tables=False,
+                             model=model if sched == colo.COLO_SCHED_LEARNED else None)
+        return [o[0] for o in out], [o[1] for o in out]
 
     for i in range(args.warmup):
-        rep_w, _ = step(main_sched)
+        step(main_sched)
         log(f"warmup {i + 1}/{args.warmup} done")
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        rep, st = step(main_sched)
-        stats.append(st)
-        log(f"step {i + 1}/{args.steps}: {st['total_ms']:.1f} ms, {st['patterns']} patterns")
+        rep, sts = step(main_sched)
+        stats.extend(sts)
+        log(f"step {i + 1}/{args.steps}: " + ", ".join(f"theta {t:g}: {s['total_ms']:.0f} ms / {s['patterns']} patterns"
+                                                      for t, s in zip(thetas, sts)))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
@@ -154,10 +155,10 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "neighbor pairs/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32",
-           "data": f"synthetic: seeded generator, reference CSV format ({p['name']})",
+           "data": f"synthetic: seeded generator ({p['name']}), generated in memory",
            "config": {"workload": describe(p), "instances": p["N"], "features": p["F"], "r": p["r"],
-                      "theta": p["theta"], "max_k": p["max_k"], "schedule": sched_name[main_sched]}}
-    out["mining_s"] = {sched_name[main_sched]: ms_per_step / 1000.0, "parse_s": parse_s}
+                      "thetas": thetas, "max_k": p["max_k"], "schedule": sched_name[main_sched]}}
+    out["mining_s"] = {sched_name[main_sched]: ms_per_step / 1000.0}
     if model is not None:
         n_m = sum(s["model_n"] for s in stats)
         out["cost_model"] = {"mean_abs_log1p_rows_err": sum(s["model_abs_err"] for s in stats) / n_m if n_m else None,
@@ -167,8 +168,8 @@ def main():
         "bin": sum(bin_ms[args.warmup:args.warmup + args.steps]) / args.steps,
         **{k: sum(s[k] for s in stats) / args.steps for k in ("filter_ms", "model_ms", "count_ms", "scan_ms",
                                                                "write_ms", "host_ms", "total_ms")}}
-    out["patterns"] = stats[-1]["patterns"]
-    out["levels"] = stats[-1]["levels"]
+    out["patterns"] = {f"{t:g}": s["patterns"] for t, s in zip(thetas, sts)}
+    out["levels"] = {f"{t:g}": s["levels"] for t, s in zip(thetas, sts)}
 
     # dominant kernel by HIP-event time: batched count or write launch
     cnt_ms = sum(s["count_kernel_ms"] for s in stats)
@@ -182,7 +183,7 @@ def main():
     achieved = (by / n) / ((ms / n) / 1000.0) / 1e9 if n and ms else 0.0
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{name}_{args.workload}.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and args.N is None:
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -207,6 +208,7 @@ def main():
             torch.cuda.synchronize()
             out["mining_s"][sched_name[sc]] = (time.perf_counter() - t0) / reps
             identical &= rep_o == rep
+            log(f"{sched_name[sc]}: {out['mining_s'][sched_name[sc]]:.3f} s per step")
         main_s = ms_per_step / 1000.0
         out["mining_s"][f"{sched_name[main_sched]}_speedup_vs_plain"] = out["mining_s"]["plain"] / main_s
         if "persistent" in out["mining_s"] and main_sched != colo.COLO_SCHED_PERSISTENT:
@@ -216,7 +218,8 @@ def main():
             log("ERROR: schedules produced different reports")
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(csv, p, args.cpu_budget, threads)
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(x, y, fc, bounds, p, args.cpu_budget, threads)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -55,16 +55,19 @@ class MineStats(C.Structure):
                [(n, C.c_uint64) for n in ("count_launches", "write_launches")] + \
                [(n, C.c_double) for n in ("l2_count_kernel_ms", "l2_count_bytes", "l2_write_kernel_ms",
                                           "l2_write_bytes", "model_abs_err", "model_n", "arena_bytes")] + \
-               [(n, C.c_uint64) for n in ("overflows", "arenas")] + [("model_ms", C.c_double)]
+               [(n, C.c_uint64) for n in ("overflows", "arenas")] + \
+               [(n, C.c_double) for n in ("model_ms", "alloc_ms", "alloc_bytes")] + [("level_ms", C.c_double * 16)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["level_ms"] = list(self.level_ms)
+        return d
 
 
 _lib = None
 
 SYMBOLS = [
-    "colo_ctx_create", "colo_ctx_destroy", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
+    "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_mine_sweep", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
@@ -83,6 +86,7 @@ def lib():
         vp, u32, u64, f32, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_int
         L.colo_ctx_create.argtypes = [C.POINTER(Dataset), f32, vp, C.POINTER(vp)]
         L.colo_ctx_destroy.argtypes = [vp]
+        L.colo_trim.argtypes = [vp]
         L.colo_bin.argtypes = [vp, f32, vp]
         L.colo_get_grid.argtypes = [vp, C.POINTER(Grid)]
         L.colo_last_error.restype = C.c_char_p
@@ -223,6 +227,10 @@ class Context:
         if getattr(self, "h", None):
             lib().colo_ctx_destroy(self.h)
             self.h = None
+
+    def trim(self):
+        """Returns the device memory cached by earlier mining runs (colo_trim)."""
+        self._check(lib().colo_trim(self.h))
 
     def __del__(self):
         self.close()

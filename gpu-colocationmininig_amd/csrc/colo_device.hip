@@ -1485,6 +1485,13 @@ int colo_ctx_destroy(colo_ctx *c) {
     return COLO_OK;
 }
 
+int colo_trim(colo_ctx *c) {
+    if (!c) return COLO_EARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->pool.release();
+    return COLO_OK;
+}
+
 int colo_get_grid(const colo_ctx *c, colo_grid *o) {
     if (!c || !o) return COLO_EARG;
     o->minX = c->minX; o->minY = c->minY; o->cellWidth = c->r; o->r2 = c->r2;

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <chrono>
+#include <iterator>
 #include <map>
 #include <string>
 #include <vector>
@@ -85,12 +87,32 @@ struct SegOut {
 // handed out again (a mining run allocates and frees multi-GB table arenas
 // per level; hipMalloc/hipFree of such sizes cost milliseconds).  Blocks are
 // rounded up to 2 MiB (or a power of two below that) and reused when at most
-// 1.5x the request.  Everything is returned to HIP on release() and when the
-// context is destroyed.
+// 1.5x the request.  Up to 3/4 of the device's memory stays cached; when
+// hipMalloc fails, cached blocks are returned largest first until it
+// succeeds.  Everything is returned to HIP on release() (colo_trim) and when
+// the context is destroyed.
 struct DevPool {
     std::multimap<size_t, void *> free_blocks;
     size_t cached = 0;
-    static constexpr size_t kCap = 48ull << 30;  // bytes kept cached at most
+    double sys_ms = 0, sys_bytes = 0;  // time in hipMalloc/hipFree, bytes allocated
+    struct Timer {
+        double &acc;
+        std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+        explicit Timer(double &a) : acc(a) {}
+        ~Timer() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+    };
+    size_t cap = 0;  // bytes kept cached at most: 3/4 of the device's memory
+    size_t cap_bytes() {
+        if (!cap) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess || !tot) {
+                (void)hipGetLastError();
+                tot = 64ull << 30;
+            }
+            cap = tot / 4 * 3;
+        }
+        return cap;
+    }
     static size_t round(size_t b) {
         if (b >= (2u << 20)) return (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
         size_t r = 256;
@@ -108,20 +130,30 @@ struct DevPool {
             return p;
         }
         void *p = nullptr;
-        if (hipMalloc(&p, want) != hipSuccess) {
+        Timer tm(sys_ms);
+        bool synced = false;
+        while (hipMalloc(&p, want) != hipSuccess) {
             (void)hipGetLastError();
-            release();  // give the cache back and try once more
-            if (hipMalloc(&p, want) != hipSuccess) {
-                (void)hipGetLastError();
-                return nullptr;
+            p = nullptr;
+            if (free_blocks.empty()) return nullptr;
+            // hand cached blocks back, largest first, until the request fits
+            if (!synced) {
+                (void)hipDeviceSynchronize();
+                synced = true;
             }
+            auto last = std::prev(free_blocks.end());
+            (void)hipFree(last->second);
+            cached -= last->first;
+            free_blocks.erase(last);
         }
+        sys_bytes += (double)want;
         *got = want;
         return p;
     }
     void free(void *p, size_t bytes) {
         if (!p) return;
-        if (cached + bytes > kCap) {
+        if (cached + bytes > cap_bytes()) {
+            Timer tm(sys_ms);
             (void)hipFree(p);
             return;
         }
@@ -129,6 +161,7 @@ struct DevPool {
         cached += bytes;
     }
     void release() {
+        Timer tm(sys_ms);
         (void)hipDeviceSynchronize();
         for (auto &kv : free_blocks) (void)hipFree(kv.second);
         free_blocks.clear();

@@ -850,6 +850,7 @@ struct Miner {
 
     int run(char **report) {
         auto T0 = Clock::now();
+        const double sys_ms0 = c->pool.sys_ms, sys_bytes0 = c->pool.sys_bytes;
         const uint32_t F = c->F;
         rep.printf("colo-report v1\n");
         rep.printf("grid gx=%u gy=%u cells=%u n=%u F=%u r2=%08x theta=%08x\n", c->gx, c->gy, c->cells, c->n, F,
@@ -868,6 +869,7 @@ struct Miner {
         while (!C.empty() && k <= F && (o.max_k == 0 || k <= o.max_k)) {
             levels++;
             cur_k_ = k;
+            auto TL = Clock::now();
             const uint64_t nC = C.size() / k;
             st.candidates += nC;
             // filter (cpp:297, 760)
@@ -1101,12 +1103,15 @@ struct Miner {
             }
             Lprev = std::move(L);
             C = std::move(N);
+            if (k < 16) st.level_ms[k] = ms_since(TL);
             k++;
         }
         rep.printf("end levels %u patterns %llu\n", levels, (unsigned long long)total_prev);
         st.levels = levels;
         st.patterns = total_prev;
         st.total_ms = ms_since(T0);
+        st.alloc_ms = c->pool.sys_ms - sys_ms0;
+        st.alloc_bytes = c->pool.sys_bytes - sys_bytes0;
         if (report) {
             *report = (char *)malloc(rep.s.size() + 1);
             memcpy(*report, rep.s.c_str(), rep.s.size() + 1);

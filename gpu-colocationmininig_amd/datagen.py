@@ -86,13 +86,16 @@ def training_config(rng, n_range=(100_000, 2_000_000)):
     """Randomised parameters of the c4 training datasets (drawn around the
     c1/c2 generator): features 10-40, instances in n_range, hotspot
     fraction 0-0.6, sigma 0.1-2% of the side, r for 8-64 mean neighbours,
-    theta 0.05-0.3.  Returns keyword overrides for generate("c2", ...)."""
+    theta 0.05-0.5, hotspots shared by all features (40%) or hosting 2..F
+    features each.  Returns keyword overrides for generate("c2", ...)."""
     F = int(rng.integers(10, 41))
     lo = float(rng.uniform(100, 2000))
+    hf = None if rng.random() < 0.4 else int(rng.integers(2, F + 1))
     return dict(F=F, N=int(rng.integers(n_range[0], n_range[1] + 1)), hot_frac=float(rng.uniform(0.0, 0.6)),
-                hotspots=int(rng.integers(50, 1000)), sigma=(lo, lo), zipf=float(rng.uniform(0.5, 1.3)),
+                hotspots=int(rng.integers(50, 2001)), sigma=(lo, lo), zipf=float(rng.uniform(0.5, 1.3)),
                 planted=[int(v) for v in rng.integers(3, 6, size=int(rng.integers(2, 8)))],
-                r=None, mean_nbrs=float(rng.uniform(8, 64)), theta=float(rng.uniform(0.05, 0.3)), max_k=4)
+                r=None, mean_nbrs=float(rng.uniform(8, 64)), theta=float(rng.uniform(0.05, 0.5)), max_k=4,
+                hot_feats=hf)
 
 
 def generate(name, N=None, seed=None, **over):

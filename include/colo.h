@@ -77,6 +77,10 @@ typedef struct colo_grid {
  * colocationFinder::setGrid (cpp:102-126). */
 int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **out);
 int colo_ctx_destroy(colo_ctx *ctx);
+/* Returns the device memory the context keeps cached between calls (table
+ * arenas and scratch of earlier mining runs) to the HIP runtime.  No
+ * counterpart in the reference, which allocates and frees per call. */
+int colo_trim(colo_ctx *ctx);
 /* Re-grids and re-bins the resident dataset for distance r (the same work
  * colo_ctx_create does after its upload).
  * Replaces: colocationFinder::setGrid (cpp:102-126) -> CountCellFeatures
@@ -217,6 +221,9 @@ typedef struct colo_mine_stats {
     double model_abs_err, model_n, arena_bytes;
     uint64_t overflows, arenas;
     double model_ms;        /* features + prediction (learned schedule, logging) */
+    double alloc_ms;        /* host time in hipMalloc / hipFree (pool misses) */
+    double alloc_bytes;     /* bytes obtained from hipMalloc during the run   */
+    double level_ms[16];    /* wall time of level k (index k)                 */
 } colo_mine_stats;
 
 /* ------------------------------------------------------- cost model */

@@ -100,7 +100,18 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--model", default=os.path.join(ROOT, "gpu-colocationmininig_amd", "models", "cost_mlp.bin"))
     ap.add_argument("--no-model", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy 3 times (traffic-counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is a design "
                                    "note only)", "n_gpus": args.gpus}))

@@ -46,6 +46,13 @@ class MineOpts(C.Structure):
                 ("model", C.c_void_p), ("log_path", C.c_char_p)]
 
 
+class Tune(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("count_var", "write_var", "blocks_per_cu", "item_rows")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class MineStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("filter_ms", "count_ms", "pi_ms", "scan_ms", "write_ms", "host_ms",
                                           "total_ms", "l2_join_ms")] + \
@@ -67,7 +74,7 @@ class MineStats(C.Structure):
 _lib = None
 
 SYMBOLS = [
-    "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
+    "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_get_tune", "colo_set_tune", "colo_autotune", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_mine_sweep", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
@@ -87,6 +94,9 @@ def lib():
         L.colo_ctx_create.argtypes = [C.POINTER(Dataset), f32, vp, C.POINTER(vp)]
         L.colo_ctx_destroy.argtypes = [vp]
         L.colo_trim.argtypes = [vp]
+        L.colo_get_tune.argtypes = [vp, C.POINTER(Tune)]
+        L.colo_set_tune.argtypes = [vp, C.POINTER(Tune)]
+        L.colo_autotune.argtypes = [vp, C.POINTER(MineOpts), C.c_char_p, C.POINTER(Tune)]
         L.colo_bin.argtypes = [vp, f32, vp]
         L.colo_get_grid.argtypes = [vp, C.POINTER(Grid)]
         L.colo_last_error.restype = C.c_char_p
@@ -227,6 +237,28 @@ class Context:
         if getattr(self, "h", None):
             lib().colo_ctx_destroy(self.h)
             self.h = None
+
+    @property
+    def tune(self):
+        """Launch shapes of the batched passes (colo_get_tune) as a dict."""
+        t = Tune()
+        self._check(lib().colo_get_tune(self.h, C.byref(t)))
+        return t.as_dict()
+
+    @tune.setter
+    def tune(self, d):
+        cur = self.tune
+        cur.update(d)
+        self._check(lib().colo_set_tune(self.h, C.byref(Tune(**cur))))
+
+    @_on_stream
+    def autotune(self, theta, max_k=0, sched=COLO_SCHED_PERSISTENT, model=None, cache=None, pi_mode=0,
+                 filter_on=True):
+        """Chooses the launch shapes on this context's data (colo_autotune)."""
+        o = _opts(theta, max_k, pi_mode, filter_on, sched, False, None, model, None)
+        t = Tune()
+        self._check(lib().colo_autotune(self.h, C.byref(o), cache.encode() if cache else None, C.byref(t)))
+        return t.as_dict()
 
     def trim(self):
         """Returns the device memory cached by earlier mining runs (colo_trim)."""

@@ -948,14 +948,18 @@ __device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &i
 
 // Level-k count over the rows of one parent table for ncand candidates that
 // extend it: the row and its members' coordinates are loaded once.
+template <int KM1>
 __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
                                              uint32_t *slots, uint32_t *bits) {
     const CandDesc &c0 = descs[0];
-    const uint32_t km1 = c0.k - 1;
+    const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
+    constexpr int kArr = KM1 ? KM1 : kMaxK;
     const uint32_t *row = c0.parent + i * km1;
-    uint32_t ids[kMaxK];
-    float mx[kMaxK], my[kMaxK];
-    for (uint32_t m = 0; m < km1; m++) {
+    uint32_t ids[kArr];
+    float mx[kArr], my[kArr];
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
         ids[m] = row[m];
         if (m) {
             mx[m] = D.x[ids[m]];
@@ -974,8 +978,11 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
             const uint32_t q = cd.nb_table[e];
             const float qx = D.sx[q], qy = D.sy[q];
             bool ok = true;
-            for (uint32_t m = 1; m < km1; m++)
-                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+#pragma unroll
+            for (uint32_t m = 1; m < (uint32_t)kArr; m++) {
+                if (m >= km1) break;
+                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) ok = false;
+            }
             if (ok) {
                 cnt++;
                 acc.set(q - sl);
@@ -984,47 +991,275 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         acc.flush();
         slots[cd.slots_off + i] = cnt;
         if (cnt)
-            for (uint32_t m = 0; m < km1; m++) set_bit(bt + cd.seg_off[m], ids[m] - D.fstart[cd.f[m]]);
+#pragma unroll
+            for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+                if (m >= km1) break;
+                set_bit(bt + cd.seg_off[m], ids[m] - D.fstart[cd.f[m]]);
+            }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_count_batch(DevData D, const CandDesc *desc, const WorkItem *items,
-                                                        const uint64_t *seg, uint32_t *slots, uint32_t *bits,
-                                                        uint32_t *queue) {
+// Sets bit `rank` of `base` for every lane with `hit`, one atomic per
+// distinct 32-bit word.  Ranks must be ascending over the participating
+// lanes (list entries are sorted positions), so lanes of one word are
+// adjacent: a segmented OR over each run of equal words leaves the run's
+// mask in its first lane.
+__device__ inline void wave_set_bits(uint32_t *base, uint32_t rank, bool act, bool hit) {
+    const uint32_t lane = __lane_id();
+    const uint32_t w = act ? (rank >> 5) : 0xffffffffu;
+    uint32_t msk = act && hit ? (1u << (rank & 31)) : 0u;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t ow = __shfl_down(w, off, 64);
+        const uint32_t om = __shfl_down(msk, off, 64);
+        if (lane + off < 64 && ow == w) msk |= om;
+    }
+    const uint32_t pw = __shfl_up(w, 1, 64);
+    if (act && msk && (lane == 0 || pw != w)) {
+        uint32_t *p = base + w;
+        if ((__atomic_load_n(p, __ATOMIC_RELAXED) & msk) != msk) atomicOr(p, msk);
+    }
+}
+
+// Wave-cooperative level-k join over parent rows [r0, r0 + n), n <= 64, of
+// the parent table shared by the ncand candidates descs[0..ncand).  Rows
+// with the same anchor (adjacent in the parent table) form a group; the
+// candidates for the last feature of every row of a group are the anchor's
+// neighbour list of that feature.  The wave spreads the list over its lanes
+// and steps through the group's rows, so each lane tests one (row, entry)
+// pair per step against the row's other members; lists of at most 32
+// entries are packed 64 / m rows per step.  Results equal the per-row
+// passes (joink_count_row_multi / joink_write_row_multi): COUNT writes
+// each row's slot and the participation bits, WRITE emits each row's
+// extensions in list order at the row's offset.
+constexpr uint32_t kRegChunks = 4;  // list chunks of 64 kept in registers
+
+template <bool WRITE, int KM1>
+__device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t r0, uint32_t n,
+                                uint32_t *slots, uint32_t *bits) {
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1;
+    const CandDesc &c0 = descs[0];
+    const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1, k = km1 + 1;
+    constexpr int kArr = KM1 ? KM1 : kMaxK;
+    const uint32_t *parent = c0.parent;
+    const uint32_t fa0 = D.fstart[c0.f[0]];
+    const float r2 = D.g.r2;
+    const uint32_t anc = lane < n ? parent[(r0 + lane) * km1] : 0xffffffffu;
+    const uint32_t prv = __shfl_up(anc, 1, 64);
+    uint64_t starts = __ballot(lane < n && (lane == 0 || anc != prv));
+    while (starts) {
+        const uint32_t gs = (uint32_t)__builtin_ctzll(starts);
+        starts &= starts - 1;
+        const uint32_t ge = starts ? (uint32_t)__builtin_ctzll(starts) : n;
+        const uint32_t nrows = ge - gs;
+        const uint32_t t = __shfl(anc, gs, 64) - fa0;
+        for (uint32_t j = 0; j < ncand; j++) {
+            const CandDesc &cd = descs[j];
+            const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
+            const uint32_t m = (uint32_t)(b1 - b0);
+            const uint32_t sl = D.fstart[cd.f[km1]];
+            uint32_t *bt = WRITE ? nullptr : bits + cd.bits_off;
+            const uint32_t *wsl = WRITE ? (cd.wslots ? cd.wslots : slots + cd.slots_off) : nullptr;
+            if (m == 0) {
+                if (!WRITE && lane < nrows) slots[cd.slots_off + r0 + gs + lane] = 0;
+                continue;
+            }
+            if (m <= 32) {
+                // packed: lane = (row offset ro, list entry e)
+                const uint32_t R = 64 / m;
+                const uint32_t ro = lane / m, e = lane - ro * m;
+                const bool act = ro < R;
+                const uint64_t segm = ((1ull << m) - 1) << (act ? ro * m : 0);
+                uint32_t q = 0;
+                float qx = 0.f, qy = 0.f;
+                if (act) {
+                    q = cd.nb_table[b0 + e];
+                    qx = D.sx[q];
+                    qy = D.sy[q];
+                }
+                bool hit = false;
+                for (uint32_t s0 = 0; s0 < nrows; s0 += R) {
+                    const uint32_t rl = s0 + ro;
+                    const bool valid = act && rl < nrows;
+                    const uint64_t row = r0 + gs + rl;
+                    const uint32_t *rp = parent + (valid ? row : r0) * km1;
+                    bool ok = valid;
+                    for (uint32_t mm = 1; mm < km1 && ok; mm++) {
+                        const uint32_t id = rp[mm];
+                        if (sqdist(D.x[id], D.y[id], qx, qy) > r2) ok = false;
+                    }
+                    const uint64_t bal = __ballot(ok);
+                    if (!WRITE) {
+                        const uint32_t cnt = (uint32_t)__popcll(bal & segm);
+                        if (valid && e == 0) {
+                            slots[cd.slots_off + row] = cnt;
+                            if (cnt)
+                                for (uint32_t mm = 0; mm < km1; mm++)
+                                    set_bit(bt + cd.seg_off[mm], rp[mm] - D.fstart[cd.f[mm]]);
+                        }
+                        hit |= ok;
+                    } else if (ok) {
+                        const uint32_t rank = (uint32_t)__popcll(bal & segm & lt);
+                        if (rank < wsl[row]) {
+                            uint32_t *out = cd.table + (cd.offsets[row] - cd.off_base + rank) * k;
+                            for (uint32_t mm = 0; mm < km1; mm++) out[mm] = rp[mm];
+                            out[km1] = D.sidx[q];
+                        }
+                    }
+                }
+                if (!WRITE) {
+                    // fold the row offsets' hits onto entry e, then one
+                    // participation bit per entry
+                    const uint64_t hb = __ballot(hit);
+                    uint64_t fold = 0;
+                    for (uint32_t x = 0; x < R; x++) fold |= (hb >> (x * m)) & ((1ull << m) - 1);
+                    wave_set_bits(bt + cd.seg_off[km1], q - sl, lane < m, (fold >> lane) & 1);
+                }
+                continue;
+            }
+            // chunked: the list in chunks of 64 lanes, the group's rows in turn
+            const uint32_t nch = (m + 63) / 64;
+            uint32_t qr[kRegChunks];
+            float xr[kRegChunks], yr[kRegChunks];
+            bool hr[kRegChunks];
+#pragma unroll
+            for (uint32_t c = 0; c < kRegChunks; c++) {
+                const uint32_t e = c * 64 + lane;
+                qr[c] = 0;
+                xr[c] = yr[c] = 0.f;
+                hr[c] = false;
+                if (e < m) {
+                    qr[c] = cd.nb_table[b0 + e];
+                    xr[c] = D.sx[qr[c]];
+                    yr[c] = D.sy[qr[c]];
+                }
+            }
+            for (uint32_t rl = 0; rl < nrows; rl++) {
+                const uint64_t row = r0 + gs + rl;
+                const uint32_t *rp = parent + row * km1;
+                uint32_t nr = 0;
+                uint64_t base = 0;
+                if (WRITE) {
+                    nr = wsl[row];
+                    if (!nr) continue;
+                    base = cd.offsets[row] - cd.off_base;
+                }
+                float mx[kArr], my[kArr];
+#pragma unroll
+                for (uint32_t mm = 1; mm < (uint32_t)kArr; mm++) {
+                    if (mm >= km1) break;
+                    const uint32_t id = rp[mm];
+                    mx[mm] = D.x[id];
+                    my[mm] = D.y[id];
+                }
+                uint32_t cnt = 0;
+                auto step = [&](uint32_t e, uint32_t q, float qx, float qy, bool &h) {
+                    bool ok = e < m;
+#pragma unroll
+                    for (uint32_t mm = 1; mm < (uint32_t)kArr; mm++) {
+                        if (mm >= km1) break;
+                        if (sqdist(mx[mm], my[mm], qx, qy) > r2) ok = false;
+                    }
+                    const uint64_t bal = __ballot(ok);
+                    if (!WRITE) {
+                        h |= ok;
+                    } else if (ok) {
+                        const uint32_t rank = cnt + (uint32_t)__popcll(bal & lt);
+                        if (rank < nr) {
+                            uint32_t *out = cd.table + (base + rank) * k;
+                            for (uint32_t mm = 0; mm < km1; mm++) out[mm] = rp[mm];
+                            out[km1] = D.sidx[q];
+                        }
+                    }
+                    cnt += (uint32_t)__popcll(bal);
+                };
+#pragma unroll
+                for (uint32_t c = 0; c < kRegChunks; c++)
+                    if (c < nch) step(c * 64 + lane, qr[c], xr[c], yr[c], hr[c]);
+                for (uint32_t c = kRegChunks; c < nch; c++) {
+                    const uint32_t e = c * 64 + lane;
+                    uint32_t q = 0;
+                    float qx = 0.f, qy = 0.f;
+                    if (e < m) {
+                        q = cd.nb_table[b0 + e];
+                        qx = D.sx[q];
+                        qy = D.sy[q];
+                    }
+                    bool h = false;
+                    step(e, q, qx, qy, h);
+                    if (!WRITE) wave_set_bits(bt + cd.seg_off[km1], q - sl, e < m, h);
+                }
+                if (!WRITE) {
+                    if (lane == 0) slots[cd.slots_off + row] = cnt;
+                    if (cnt && lane < km1) set_bit(bt + cd.seg_off[lane], rp[lane] - D.fstart[cd.f[lane]]);
+                }
+            }
+            if (!WRITE) {
+#pragma unroll
+                for (uint32_t c = 0; c < kRegChunks; c++)
+                    if (c < nch) wave_set_bits(bt + cd.seg_off[km1], qr[c] - sl, c * 64 + lane < m, hr[c]);
+            }
+        }
+    }
+}
+
+// Level 2: one anchor per thread over the item's anchor range.
+__global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDesc *desc, const WorkItem *items,
+                                                         const uint64_t *seg, uint32_t *slots, uint32_t *bits,
+                                                         uint32_t *queue) {
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         const CandDesc &cd = desc[wi.cand];
         uint32_t *sl = slots + cd.slots_off;
         uint32_t *bt = bits + cd.bits_off;
-        if (cd.k == 2) {
-            const uint32_t a = cd.f[0], b = cd.f[1];
+        const uint32_t a = cd.f[0], b = cd.f[1];
+        for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+            join2_count_anchor(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+    }
+}
+
+// Level k >= 3 (KM1 = k - 1, 0 for any k): the item's parent rows in chunks
+// of 64, taken by the block's waves in turn.
+// Level k >= 3 (KM1 = k - 1; 0 for any k).  VAR 0: one parent row per
+// thread; VAR 1: wave-cooperative (wave_rows_multi) over 64-row chunks
+// taken by the block's waves in turn.
+template <int KM1, int VAR>
+__global__ __launch_bounds__(kBlock) void k_countk_batch(DevData D, const CandDesc *desc, const WorkItem *items,
+                                                         const uint64_t *seg, uint32_t *slots, uint32_t *bits,
+                                                         uint32_t *queue) {
+    uint64_t it;
+    while (next_item(queue, seg, it)) {
+        const WorkItem wi = items[it];
+        if (VAR == 0) {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                join2_count_anchor(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
-        } else if (cd.nb_table) {
-            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                joink_count_row_multi(D, desc + wi.cand, wi.ncand, i, slots, bits);
+                joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits);
         } else {
-            const uint32_t km1 = cd.k - 1;
-            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                joink_count_row(D, cd.parent + i * km1, km1, cd.f, cd.f[km1], sl + i, bt, cd.seg_off);
+            for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
+                const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
+                wave_rows_multi<false, KM1>(D, desc + wi.cand, wi.ncand, c0, n, slots, bits);
+            }
         }
     }
 }
 
 // Level-k write over the rows of one parent table for ncand tables.
+template <int KM1>
 __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
                                              const uint32_t *slots) {
     const CandDesc &c0 = descs[0];
-    const uint32_t km1 = c0.k - 1, k = c0.k;
+    const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1, k = km1 + 1;
+    constexpr int kArr = KM1 ? KM1 : kMaxK;
     const uint32_t *row = c0.parent + i * km1;
-    uint32_t ids[kMaxK];
-    float mx[kMaxK], my[kMaxK];
+    uint32_t ids[kArr];
+    float mx[kArr], my[kArr];
     auto slot_of = [&](const CandDesc &cd) { return cd.wslots ? cd.wslots[i] : slots[cd.slots_off + i]; };
     bool any = false;
     for (uint32_t j = 0; j < ncand; j++) any |= slot_of(descs[j]) != 0;
     if (!any) return;
-    for (uint32_t m = 0; m < km1; m++) {
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
         ids[m] = row[m];
         if (m) {
             mx[m] = D.x[ids[m]];
@@ -1043,10 +1278,17 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
             const uint32_t q = cd.nb_table[e];
             const float qx = D.sx[q], qy = D.sy[q];
             bool ok = true;
-            for (uint32_t m = 1; m < km1; m++)
-                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
+#pragma unroll
+            for (uint32_t m = 1; m < (uint32_t)kArr; m++) {
+                if (m >= km1) break;
+                if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) ok = false;
+            }
             if (ok) {
-                for (uint32_t m = 0; m < km1; m++) out[(uint64_t)w * k + m] = ids[m];
+#pragma unroll
+                for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+                    if (m >= km1) break;
+                    out[(uint64_t)w * k + m] = ids[m];
+                }
                 out[(uint64_t)w * k + km1] = D.sidx[q];
                 w++;
             }
@@ -1054,54 +1296,90 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_write_batch(DevData D, const CandDesc *desc, const WorkItem *items,
-                                                        const uint64_t *seg, const uint32_t *slots, uint32_t *queue) {
+__global__ __launch_bounds__(kBlock) void k_write2_batch(DevData D, const CandDesc *desc, const WorkItem *items,
+                                                         const uint64_t *seg, const uint32_t *slots, uint32_t *queue) {
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         const CandDesc &cd = desc[wi.cand];
         const uint32_t *sl = cd.wslots ? cd.wslots : slots + cd.slots_off;
-        if (cd.k == 2) {
-            for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
-                uint32_t nr = sl[t];
-                if (nr) join2_write_anchor(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
-            }
-        } else if (cd.nb_table) {
+        for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
+            uint32_t nr = sl[t];
+            if (nr) join2_write_anchor(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
+        }
+    }
+}
+
+template <int KM1, int VAR>
+__global__ __launch_bounds__(kBlock) void k_writek_batch(DevData D, const CandDesc *desc, const WorkItem *items,
+                                                         const uint64_t *seg, const uint32_t *slots, uint32_t *queue) {
+    uint64_t it;
+    while (next_item(queue, seg, it)) {
+        const WorkItem wi = items[it];
+        if (VAR == 0) {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                joink_write_row_multi(D, desc + wi.cand, wi.ncand, i, slots);
+                joink_write_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots);
         } else {
-            const uint32_t km1 = cd.k - 1;
-            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock) {
-                uint32_t nr = sl[i];
-                if (nr) joink_write_row(D, cd.parent + i * km1, km1, cd.f[km1], nr, cd.offsets[i] - cd.off_base, cd.table);
+            for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
+                const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
+                wave_rows_multi<true, KM1>(D, desc + wi.cand, wi.ncand, c0, n, const_cast<uint32_t *>(slots),
+                                           nullptr);
             }
         }
     }
 }
 
-static unsigned persistent_grid(uint64_t n_items) {
-    uint64_t g = (uint64_t)num_cus() * 8;
+static unsigned persistent_grid(const colo_ctx *c, uint64_t n_items) {
+    uint64_t g = (uint64_t)num_cus() * c->tune.blocks_per_cu;
     if (g > n_items) g = (n_items + kXcds - 1) / kXcds * kXcds;
     return (unsigned)(g ? g : kXcds);
 }
 
-int colo::count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+// Kernel of a level: k = 2, a specialisation for k = 3..8, any k above.
+template <template <int> class Pick>
+static auto pick_k(uint32_t k) -> decltype(Pick<0>::get()) {
+    switch (k) {
+        case 3: return Pick<2>::get();
+        case 4: return Pick<3>::get();
+        case 5: return Pick<4>::get();
+        case 6: return Pick<5>::get();
+        case 7: return Pick<6>::get();
+        case 8: return Pick<7>::get();
+        default: return Pick<0>::get();
+    }
+}
+template <int KM1> struct PickCount0 { static auto get() { return &k_countk_batch<KM1, 0>; } };
+template <int KM1> struct PickCount1 { static auto get() { return &k_countk_batch<KM1, 1>; } };
+template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
+template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
+
+int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                       uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s) {
     if (!n_items) return COLO_OK;
     int rc = fill_u32(d_queue, kXcds, 0, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_count_batch, dim3(persistent_grid(n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
-                       d_items, d_seg, d_slots, d_bits, d_queue);
+    if (k == 2)
+        hipLaunchKernelGGL(k_count2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
+                           d_items, d_seg, d_slots, d_bits, d_queue);
+    else
+        hipLaunchKernelGGL(c->tune.count_var ? pick_k<PickCount1>(k) : pick_k<PickCount0>(k),
+                           dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
+                           d_slots, d_bits, d_queue);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "count_batch launch");
 }
 
-int colo::write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+int colo::write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                       uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s) {
     if (!n_items) return COLO_OK;
     int rc = fill_u32(d_queue, kXcds, 0, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_write_batch, dim3(persistent_grid(n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
-                       d_items, d_seg, d_slots, d_queue);
+    if (k == 2)
+        hipLaunchKernelGGL(k_write2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
+                           d_items, d_seg, d_slots, d_queue);
+    else
+        hipLaunchKernelGGL(c->tune.write_var ? pick_k<PickWrite1>(k) : pick_k<PickWrite0>(k),
+                           dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
+                           d_slots, d_queue);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "write_batch launch");
 }
 
@@ -1482,6 +1760,27 @@ int colo_ctx_destroy(colo_ctx *c) {
     c->pool.release();
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return COLO_OK;
+}
+
+int colo_get_tune(const colo_ctx *c, colo_tune *o) {
+    if (!c || !o) return COLO_EARG;
+    o->count_var = c->tune.count_var;
+    o->write_var = c->tune.write_var;
+    o->blocks_per_cu = c->tune.blocks_per_cu;
+    o->item_rows = c->tune.item_rows;
+    return COLO_OK;
+}
+
+int colo_set_tune(colo_ctx *c, const colo_tune *t) {
+    if (!c || !t) return COLO_EARG;
+    if (t->count_var > 1 || t->write_var > 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+        t->item_rows > 65536)
+        return set_error(c, COLO_EARG, "launch shape out of range");
+    c->tune.count_var = t->count_var;
+    c->tune.write_var = t->write_var;
+    c->tune.blocks_per_cu = t->blocks_per_cu;
+    c->tune.item_rows = t->item_rows;
     return COLO_OK;
 }
 

@@ -171,8 +171,17 @@ struct DevPool {
 
 }  // namespace colo
 
+// Launch shapes of the batched passes (chosen by the autotuner).
+struct TuneParams {
+    uint32_t count_var = 0;       // level-k count: 0 one row per thread, 1 wave-cooperative
+    uint32_t write_var = 0;       // level-k write: same choice
+    uint32_t blocks_per_cu = 8;   // persistent grid: workgroups per CU
+    uint32_t item_rows = 1024;    // rows (anchors) per work item without the cost model
+};
+
 struct colo_ctx {
     hipStream_t stream = nullptr;
+    TuneParams tune;
     bool own_stream = false;
     uint32_t n = 0, F = 0;
     std::vector<uint32_t> fstart, fcount;
@@ -216,9 +225,9 @@ int filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, f
 // Batched passes over many candidates (one launch each).
 // d_seg: 9 item boundaries, one segment per XCD (see xcd_segments);
 // d_queue: 8 counters.
-int count_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+int count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                 uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s);
-int write_batch(colo_ctx *c, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
+int write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                 uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
 int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
                  hipStream_t s);

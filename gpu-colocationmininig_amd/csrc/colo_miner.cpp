@@ -533,7 +533,7 @@ struct Miner {
         if (pred) {
             double total = 0;
             for (auto &r : runs) total += cost[r.first];
-            double target = std::max(1.0, total / (double)(num_cus() * 8 * 4));
+            double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 4));
             for (auto &r : runs) {
                 uint64_t n = desc[r.first].n_anchor;
                 double per_row = n ? cost[r.first] / (double)n : 1.0;
@@ -543,7 +543,7 @@ struct Miner {
                 for (size_t i = before; i < items.size(); i++) iw.push_back(per_row * (items[i].end - items[i].begin));
             }
         } else {
-            for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
+            for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows, items, r.second);
         }
         slots_arena = std::make_shared<DevBuf>(&c->pool);
         if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return oom("slots arena", std::max<uint64_t>(slots_tot, 1) * 4);
@@ -559,7 +559,7 @@ struct Miner {
         int rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
         if (rc) return rc;
         size_t e0 = mark();
-        rc = count_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
+        rc = count_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
                          slots_arena->as<uint32_t>(), arena_bits.as<uint32_t>(), d_queue.as<uint32_t>(), s);
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Count : kCount);
@@ -724,7 +724,7 @@ struct Miner {
             if (k == 2) st.l2_rows += p.rows;
         }
         if (desc.empty()) return COLO_OK;
-        for (auto &r : parent_runs(desc, k)) make_items(r.first, desc[r.first].n_anchor, 1024, items, r.second);
+        for (auto &r : parent_runs(desc, k)) make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows, items, r.second);
         if (!d_desc.ensure(desc.size() * sizeof(CandDesc)) || !d_queue.ensure(64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
@@ -733,7 +733,7 @@ struct Miner {
             if (rc0) return rc0;
         }
         size_t e0 = mark();
-        int rc = write_batch(c, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(), nullptr,
+        int rc = write_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(), nullptr,
                              d_queue.as<uint32_t>(), s);
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Write : kWrite);
@@ -1301,6 +1301,115 @@ int colo_mine_sweep(colo_ctx *c, const colo_mine_opts *opts, const float *thetas
         if (rc) break;
     }
     return rc;
+}
+
+// ------------------------------------------------------------ autotuner
+
+static std::string tune_key(const colo_ctx *c, const colo_mine_opts &o) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t p;
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+        arch = p.gcnArchName;
+        cus = p.multiProcessorCount;
+    } else {
+        (void)hipGetLastError();
+    }
+    for (char &ch : arch)
+        if (ch == ' ') ch = '_';
+    char b[256];
+    snprintf(b, sizeof b, "%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
+             fbits(o.theta), o.max_k, o.sched);
+    return b;
+}
+
+static bool tune_load(const char *path, const std::string &key, TuneParams *t) {
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return false;
+    char k[256];
+    unsigned a, b, cc, d;
+    bool found = false;
+    while (fscanf(fp, "%255s %u %u %u %u", k, &a, &b, &cc, &d) == 5)
+        if (key == k) {  // the last entry of a key wins
+            t->count_var = a;
+            t->write_var = b;
+            t->blocks_per_cu = cc;
+            t->item_rows = d;
+            found = true;
+        }
+    fclose(fp);
+    return found;
+}
+
+int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_path, colo_tune *out) {
+    if (!c || !opts) return COLO_EARG;
+    colo_mine_opts o = *opts;
+    o.out_prefix = nullptr;
+    o.log_path = nullptr;
+    o.hash_tables = 0;
+    if (o.sched == COLO_SCHED_PLAIN) o.sched = COLO_SCHED_PERSISTENT;
+    const std::string key = tune_key(c, o);
+    TuneParams best = c->tune;
+    if (!(cache_path && tune_load(cache_path, key, &best))) {
+        // kernel time of one run: {level-k count, level-k write, all batched passes}
+        auto measure = [&](const TuneParams &t, double *m3) -> int {
+            c->tune = t;
+            Miner M(c, o);
+            int rc = M.run(nullptr);
+            (void)hipStreamSynchronize(c->stream);
+            if (rc) return rc;
+            m3[0] = M.st.count_kernel_ms - M.st.l2_count_kernel_ms;
+            m3[1] = M.st.write_kernel_ms - M.st.l2_write_kernel_ms;
+            m3[2] = M.st.count_kernel_ms + M.st.write_kernel_ms;
+            return COLO_OK;
+        };
+        double cur[3], m[3];
+        int rc = measure(best, cur);  // warm-up (pool, cached features)
+        if (!rc) rc = measure(best, cur);
+        if (rc) {
+            c->tune = best;
+            return rc;
+        }
+        struct Axis {
+            uint32_t TuneParams::*field;
+            std::vector<uint32_t> values;
+            int metric;
+        };
+        std::vector<Axis> axes = {{&TuneParams::count_var, {0, 1}, 0},
+                                  {&TuneParams::write_var, {0, 1}, 1},
+                                  {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
+        if (o.sched != COLO_SCHED_LEARNED || !o.model)  // the cost model cuts its own items
+            axes.push_back({&TuneParams::item_rows, {256, 1024, 4096}, 2});
+        for (const Axis &ax : axes) {
+            const uint32_t start = best.*(ax.field);
+            for (uint32_t v : ax.values) {
+                if (v == start) continue;
+                TuneParams t = best;
+                t.*(ax.field) = v;
+                if ((rc = measure(t, m))) {
+                    c->tune = best;
+                    return rc;
+                }
+                if (m[ax.metric] < cur[ax.metric]) {
+                    best = t;
+                    memcpy(cur, m, sizeof cur);
+                }
+            }
+        }
+        if (cache_path) {
+            FILE *fp = fopen(cache_path, "ab");
+            if (fp) {
+                fprintf(fp, "%s %u %u %u %u\n", key.c_str(), best.count_var, best.write_var, best.blocks_per_cu,
+                        best.item_rows);
+                fclose(fp);
+            }
+        }
+    }
+    c->tune = best;
+    if (out) colo_get_tune(c, out);
+    return COLO_OK;
 }
 
 int colo_run_config(const char *config_path, const colo_mine_opts *opts, char **timing_line) {

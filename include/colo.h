@@ -226,6 +226,32 @@ typedef struct colo_mine_stats {
     double level_ms[16];    /* wall time of level k (index k)                 */
 } colo_mine_stats;
 
+/* ------------------------------------------------------- launch shapes */
+
+/* Launch shapes of the batched (persistent) passes.  No counterpart in the
+ * reference, whose kernels have fixed 1-D launches.
+ *   count_var, write_var  level-k count / write pass: 0 = one parent row
+ *                         per thread, 1 = wave-cooperative over 64-row
+ *                         chunks (a row group's neighbour list spread over
+ *                         the lanes)
+ *   blocks_per_cu         persistent workgroups per compute unit (1..32)
+ *   item_rows             rows per work-queue item when no cost model
+ *                         partitions the queue (64..65536) */
+typedef struct colo_tune {
+    uint32_t count_var, write_var, blocks_per_cu, item_rows;
+} colo_tune;
+
+int colo_get_tune(const colo_ctx *ctx, colo_tune *out);
+int colo_set_tune(colo_ctx *ctx, const colo_tune *t);
+/* Autotuner: mines the context's data with opts under candidate launch
+ * shapes, one parameter at a time (coordinate descent), and keeps the one
+ * with the least HIP-event kernel time in the passes the parameter
+ * affects; the context then uses the result.  With cache_path, results are
+ * stored under a key of device and workload (instances, features, r, theta,
+ * max_k, schedule) and a cached entry is taken without measuring.  Every
+ * shape gives the same results; only time differs. */
+int colo_autotune(colo_ctx *ctx, const colo_mine_opts *opts, const char *cache_path, colo_tune *out);
+
 /* ------------------------------------------------------- cost model */
 
 /* Candidate features the miner logs and the cost model reads (16 floats per

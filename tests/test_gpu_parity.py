@@ -275,3 +275,40 @@ def test_threshold_sweep(c2small, sched):
         single, _ = ctx.mine(th, max_k=4, sched=sched)
         assert rep == single, th
     assert out[0][0] == orc.mine(p["r"], 0.25, max_k=4)
+
+
+@pytest.mark.parametrize("tune", [dict(count_var=1, write_var=1), dict(blocks_per_cu=4, item_rows=64),
+                                  dict(count_var=1, write_var=0, blocks_per_cu=16, item_rows=4096)])
+def test_launch_shapes_identical(c2small, tune):
+    """Every launch shape of the batched passes gives the oracle's report."""
+    ctx, orc, p = c2small
+    _say(f"launch shape {tune}")
+    old = ctx.tune
+    try:
+        ctx.tune = tune
+        for sched in (1, 2):
+            rep, _ = ctx.mine(0.15, max_k=4, sched=sched)
+            assert rep == orc.mine(p["r"], 0.15, max_k=4), (tune, sched)
+    finally:
+        ctx.tune = old
+
+
+def test_autotune_and_cache(c2small, tmp_path):
+    """The autotuner picks valid shapes, stores them, and takes them from the
+    cache on the next call; results stay identical."""
+    ctx, orc, p = c2small
+    _say("autotune")
+    old = ctx.tune
+    cache = str(tmp_path / "tune.txt")
+    try:
+        t1 = ctx.autotune(0.15, max_k=4, cache=cache)
+        assert os.path.getsize(cache) > 0
+        ctx.tune = old
+        t2 = ctx.autotune(0.15, max_k=4, cache=cache)
+        assert t1 == t2 and ctx.tune == t1
+        with open(cache) as fh:
+            assert len(fh.read().splitlines()) == 1
+        rep, _ = ctx.mine(0.15, max_k=4)
+        assert rep == orc.mine(p["r"], 0.15, max_k=4)
+    finally:
+        ctx.tune = old

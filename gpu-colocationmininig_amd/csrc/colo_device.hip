@@ -421,46 +421,71 @@ struct FilterItem {
 
 constexpr int kFilterRuns = kBlock * 8;
 
-__global__ __launch_bounds__(kBlock) void k_filter(const FilterItem *items, const uint64_t *chunk_prefix,
-                                                   uint64_t n_items, const uint32_t *cands, uint32_t k,
-                                                   const uint32_t *run_cell, const uint32_t *dir,
+// One block = (feature f, up to kBlock candidate members (P, t) with
+// P[t] = f, a tile of kFilterRuns non-empty cells of f).  The block stages
+// each cell's <= 4 quadrant masks and count in LDS once; every thread then
+// tests its candidate's mask against all staged cells (LDS broadcast), so
+// a cell is read from memory once per group of candidates instead of once
+// per candidate.
+struct FilterGroup {
+    uint32_t f, n_memb;
+    uint64_t memb_off;  // first entry in the member list
+    uint64_t run_begin, run_end;
+};
+
+__global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, const uint64_t *chunk_prefix,
+                                                   uint64_t n_groups, const uint32_t *memb, const uint64_t *cmask,
+                                                   uint32_t k, const uint32_t *run_cell, const uint32_t *dir,
                                                    const uint64_t *qmask, DevGrid g, uint32_t *counts) {
-    // find the item of this block
+    __shared__ uint64_t s_q[4][kBlock];
+    __shared__ uint32_t s_n[kBlock];
     uint64_t b = blockIdx.x;
-    uint64_t lo = 0, hi = n_items;
+    uint64_t lo = 0, hi = n_groups;
     while (hi - lo > 1) {
         uint64_t mid = (lo + hi) >> 1;
         if (chunk_prefix[mid] <= b) lo = mid; else hi = mid;
     }
-    const FilterItem it = items[lo];
-    uint64_t r0 = it.run_begin + (b - chunk_prefix[lo]) * kFilterRuns;
-    uint64_t r1 = r0 + kFilterRuns < it.run_end ? r0 + kFilterRuns : it.run_end;
-    const uint32_t *cand = cands + (uint64_t)it.cand * k;
-    uint64_t mask = 0;
-    for (uint32_t t = 0; t < k; t++) mask |= 1ull << cand[t];
-    const uint32_t f = cand[it.t];
-    const uint32_t *dirf = dir + (uint64_t)f * (g.cells + 1);
+    const FilterGroup G = groups[lo];
+    const uint64_t r0 = G.run_begin + (b - chunk_prefix[lo]) * kFilterRuns;
+    const uint64_t r1 = r0 + kFilterRuns < G.run_end ? r0 + kFilterRuns : G.run_end;
+    const uint32_t *dirf = dir + (uint64_t)G.f * (g.cells + 1);
+    const bool mine = threadIdx.x < G.n_memb;
+    const uint32_t me = mine ? memb[G.memb_off + threadIdx.x] : 0;
+    const uint64_t M = mine ? cmask[me / k] : 0;
     uint64_t sum = 0;
-    if (g.gx >= 2 && g.gy >= 2) {
-        for (uint64_t h = r0 + threadIdx.x; h < r1; h += kBlock) {
-            uint32_t c = run_cell[h];
-            uint32_t cy = c / g.gx, cx = c - cy * g.gx;
-            bool ok = false;
+    const bool quads = g.gx >= 2 && g.gy >= 2;
+    for (uint64_t base = r0; base < r1; base += kBlock) {
+        const uint64_t h = base + threadIdx.x;
+        uint64_t q[4] = {0, 0, 0, 0};
+        uint32_t n = 0;
+        if (h < r1 && quads) {
+            const uint32_t c = run_cell[h];
+            const uint32_t cy = c / g.gx, cx = c - cy * g.gx;
 #pragma unroll
             for (int dq = 0; dq < 4; dq++) {
-                uint32_t dqy = dq >> 1, dqx = dq & 1;
+                const uint32_t dqy = dq >> 1, dqx = dq & 1;
                 if (cy < dqy || cx < dqx) continue;
-                uint32_t qy = cy - dqy, qx = cx - dqx;
+                const uint32_t qy = cy - dqy, qx = cx - dqx;
                 if (qy + 1 >= g.gy || qx + 1 >= g.gx) continue;
-                uint64_t qm = qmask[(uint64_t)qy * (g.gx - 1) + qx];
-                ok |= (qm & mask) == mask;
+                q[dq] = qmask[(uint64_t)qy * (g.gx - 1) + qx];
             }
-            if (ok) sum += dirf[c + 1] - dirf[c];
+            n = dirf[c + 1] - dirf[c];
         }
+#pragma unroll
+        for (int dq = 0; dq < 4; dq++) s_q[dq][threadIdx.x] = q[dq];
+        s_n[threadIdx.x] = n;
+        __syncthreads();
+        if (mine) {
+            const uint32_t cnt = (uint32_t)(r1 - base < kBlock ? r1 - base : kBlock);
+            for (uint32_t j = 0; j < cnt; j++) {
+                const bool ok = ((s_q[0][j] & M) == M) | ((s_q[1][j] & M) == M) | ((s_q[2][j] & M) == M) |
+                                ((s_q[3][j] & M) == M);
+                sum += ok ? s_n[j] : 0u;
+            }
+        }
+        __syncthreads();
     }
-    uint64_t ex;
-    uint64_t tot = block_excl_scan(sum, ex);
-    if (threadIdx.x == 0 && tot) atomicAdd(&counts[(uint64_t)it.cand * k + it.t], (uint32_t)tot);
+    if (mine && sum) atomicAdd(&counts[me], (uint32_t)sum);
 }
 
 // threshold_Kernel (K:545-573): ub_t = fl32(double(cnt)/double(fl32(N))),
@@ -486,36 +511,59 @@ int colo::filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_
     std::vector<uint32_t> hc((uint64_t)C * k);
     COLO_HIP(hipMemcpyAsync(hc.data(), d_cands, hc.size() * 4, hipMemcpyDeviceToHost, s));
     COLO_HIP(hipStreamSynchronize(s));
-    std::vector<FilterItem> items;
-    std::vector<uint64_t> prefix;
-    uint64_t chunks = 0;
-    items.reserve((uint64_t)C * k);
+    // members (P, t) grouped by feature; candidate masks
+    std::vector<uint64_t> cmask(C, 0);
+    std::vector<std::vector<uint32_t>> by_f(c->F);
     for (uint32_t i = 0; i < C; i++)
         for (uint32_t t = 0; t < k; t++) {
             uint32_t f = hc[(uint64_t)i * k + t];
             if (f >= c->F) return set_error(c, COLO_EARG, "candidate feature out of range");
-            uint64_t rb = c->run_off[f], re = c->run_off[f + 1];
-            if (re == rb) continue;
-            items.push_back(FilterItem{i, t, rb, re});
+            cmask[i] |= 1ull << f;
+            by_f[f].push_back(i * k + t);
+        }
+    std::vector<uint32_t> memb;
+    std::vector<FilterGroup> groups;
+    std::vector<uint64_t> prefix;
+    uint64_t chunks = 0;
+    for (uint32_t f = 0; f < c->F; f++) {
+        const uint64_t rb = c->run_off[f], re = c->run_off[f + 1];
+        const std::vector<uint32_t> &L = by_f[f];
+        if (re == rb || L.empty()) continue;
+        for (size_t g0 = 0; g0 < L.size(); g0 += kBlock) {
+            const uint32_t n = (uint32_t)std::min<size_t>(kBlock, L.size() - g0);
+            groups.push_back(FilterGroup{f, n, memb.size(), rb, re});
+            memb.insert(memb.end(), L.begin() + g0, L.begin() + g0 + n);
             prefix.push_back(chunks);
             chunks += (re - rb + kFilterRuns - 1) / kFilterRuns;
         }
+    }
+    if ((uint64_t)C * k > 0xffffffffull) return set_error(c, COLO_EUNSUPPORTED, "too many candidates for the filter");
     int rc = fill_u32(d_counts, (uint64_t)C * k, 0, s);
     if (rc) return rc;
-    if (!items.empty()) {
-        size_t bytes = items.size() * sizeof(FilterItem) + prefix.size() * 8;
+    if (!groups.empty()) {
+        size_t bytes = groups.size() * sizeof(FilterGroup) + prefix.size() * 8 + cmask.size() * 8 + memb.size() * 4;
         void *tmp;
         rc = scratch(c, bytes, &tmp);
         if (rc) return rc;
-        FilterItem *d_items = (FilterItem *)tmp;
-        uint64_t *d_prefix = (uint64_t *)((char *)tmp + items.size() * sizeof(FilterItem));
-        COLO_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(FilterItem), hipMemcpyHostToDevice, s));
+        char *q = (char *)tmp;
+        FilterGroup *d_groups = (FilterGroup *)q;
+        q += groups.size() * sizeof(FilterGroup);
+        uint64_t *d_prefix = (uint64_t *)q;
+        q += prefix.size() * 8;
+        uint64_t *d_cmask = (uint64_t *)q;
+        q += cmask.size() * 8;
+        uint32_t *d_memb = (uint32_t *)q;
+        COLO_HIP(hipMemcpyAsync(d_groups, groups.data(), groups.size() * sizeof(FilterGroup), hipMemcpyHostToDevice,
+                                s));
         COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync(d_cmask, cmask.data(), cmask.size() * 8, hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync(d_memb, memb.data(), memb.size() * 4, hipMemcpyHostToDevice, s));
         DevGrid g = c->dev().g;
-        hipLaunchKernelGGL(k_filter, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_items, d_prefix,
-                           (uint64_t)items.size(), d_cands, k, c->d_run_cell, c->d_dir, c->d_qmask, g, d_counts);
+        hipLaunchKernelGGL(k_filter, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_groups, d_prefix,
+                           (uint64_t)groups.size(), d_memb, d_cmask, k, c->d_run_cell, c->d_dir, c->d_qmask, g,
+                           d_counts);
         COLO_HIP(hipGetLastError());
-        // the item arrays live in scratch: finish before the next scratch use
+        // the group arrays live in scratch: finish before the next scratch use
         COLO_HIP(hipStreamSynchronize(s));
     }
     hipLaunchKernelGGL(k_threshold, dim3(grid_for(C)), dim3(kBlock), 0, s, d_counts, d_cands, C, k,

@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
 """Summaries of rocprofv3 output for the miner's kernels.
 
-  kernels  <run_results.db | kernel_stats.csv | kernel_trace.csv>  [--top N]
-      Markdown table of per-kernel calls and durations (microseconds).
+  kernels  <run_results.db | kernel_trace.csv>  [--top N] [--templates]
+      Markdown table of per-kernel calls and durations (microseconds); the
+      instantiations of a kernel template are summed unless --templates.
   traffic  --fetch F.csv --write W.csv --cfetch CF.csv --cwrite CW.csv --kernel NAME [--copy-bytes B]
       HBM bytes per launch of NAME from two `rocprofv3 --pmc FETCH_SIZE` /
       `--pmc WRITE_SIZE` runs (KiB per dispatch), each scaled by what the same
@@ -30,10 +31,13 @@ def _kernel_rows(path):
     return rows
 
 
-def kernels(path, top):
+def kernels(path, top, templates=False):
     agg = collections.defaultdict(list)
     for n, d in _kernel_rows(path):
-        agg[n.split("(")[0].strip() or n].append(d)
+        base = n.split("(")[0].replace("void ", "").strip() or n
+        if not templates:
+            base = base.split("<")[0]
+        agg[base].append(d)
     tot = sum(sum(v) for v in agg.values())
     out = ["| kernel | calls | total us | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
     for n, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
@@ -48,7 +52,7 @@ def _counter_per_dispatch(path, counter, kernel):
     per = collections.defaultdict(float)
     with open(path) as fh:
         for r in csv.DictReader(fh):
-            if r["Counter_Name"] != counter or not r["Kernel_Name"].startswith(kernel):
+            if r["Counter_Name"] != counter or kernel not in r["Kernel_Name"]:
                 continue
             per[r["Dispatch_Id"]] += float(r["Counter_Value"])
     if not per:
@@ -75,13 +79,14 @@ def main():
     k = sub.add_parser("kernels")
     k.add_argument("path")
     k.add_argument("--top", type=int, default=25)
+    k.add_argument("--templates", action="store_true", help="one line per template instantiation")
     t = sub.add_parser("traffic")
     for n in ("fetch", "write", "cfetch", "cwrite", "kernel"):
         t.add_argument(f"--{n}", required=True)
     t.add_argument("--copy-bytes", type=float, default=256 * 1024 * 1024)
     a = ap.parse_args()
     if a.cmd == "kernels":
-        print(kernels(a.path, a.top))
+        print(kernels(a.path, a.top, a.templates))
     else:
         print(json.dumps(traffic(a), indent=1))
 

@@ -720,12 +720,14 @@ int colo::pairwork_batch(colo_ctx *c, const uint32_t *pairs, uint32_t P, uint64_
 }
 
 // Level-2 count for one anchor at sorted position p of feature a
-// (degree2TableSlotCounter_Kernel, K:12-55).
+// (degree2TableSlotCounter_Kernel, K:12-55).  The anchor's slot and bit are
+// indexed by its input rank (the reference's layout) or, with SORTED, by
+// its sorted rank (the batched miner's layout).
+template <bool SORTED = false>
 __device__ inline void join2_count_anchor(const DevData &D, uint32_t a, uint32_t b, uint32_t p, uint32_t *slots,
                                           uint32_t *bits_a, uint32_t *bits_b) {
     const uint32_t sa = D.fstart[a], sb = D.fstart[b];
-    const uint32_t g = D.sidx[p];
-    const uint32_t t = g - sa;
+    const uint32_t t = SORTED ? p - sa : D.sidx[p] - sa;
     const float xa = D.sx[p], ya = D.sy[p];
     const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
     RowSpan rs;
@@ -778,15 +780,19 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
         for (uint32_t m = 0; m < km1; m++) set_bit(bits + seg_off[m], row[m] - D.fstart[feat[m]]);
 }
 
-// Level-2 write for the anchor of input rank t (degree2TableGenerator_Kernel,
-// K:56-100): rows (g, q).
+// Level-2 write for one anchor of feature a (degree2TableGenerator_Kernel,
+// K:56-100): t is its input rank and the rows are (instance id, instance
+// id) -- or, with SORTED, t is its sorted rank and the rows are (sorted
+// position, sorted position).  nbr (optional) receives the second column as
+// sorted positions.
+template <bool SORTED = false>
 __device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t b, uint64_t t, uint32_t nrows,
                                           uint64_t off, uint32_t *table, uint32_t *nbr) {
     const uint32_t g = D.fstart[a] + (uint32_t)t;
-    const float xa = D.x[g], ya = D.y[g];
+    const float xa = SORTED ? D.sx[g] : D.x[g], ya = SORTED ? D.sy[g] : D.y[g];
     const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
     RowSpan rs;
-    int m = row_spans(D.g, dirb, D.cell_of[g], rs);
+    int m = row_spans(D.g, dirb, SORTED ? D.scell[g] : D.cell_of[g], rs);
     uint32_t *out = table + off * 2;
     uint32_t *nb = nbr ? nbr + off : nullptr;
     uint32_t w = 0;
@@ -794,7 +800,7 @@ __device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t
         for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
             if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
                 out[2 * w] = g;
-                out[2 * w + 1] = D.sidx[q];
+                out[2 * w + 1] = SORTED ? q : D.sidx[q];
                 if (nb) nb[w] = q;
                 w++;
             }
@@ -995,7 +1001,9 @@ __device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &i
 }
 
 // Level-k count over the rows of one parent table for ncand candidates that
-// extend it: the row and its members' coordinates are loaded once.
+// extend it: the row and its members' coordinates are loaded once.  Tables
+// of the batched miner hold sorted positions; slots and participation bits
+// are indexed by sorted rank.
 template <int KM1>
 __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
                                              uint32_t *slots, uint32_t *bits) {
@@ -1010,8 +1018,8 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         if (m >= km1) break;
         ids[m] = row[m];
         if (m) {
-            mx[m] = D.x[ids[m]];
-            my[m] = D.y[ids[m]];
+            mx[m] = D.sx[ids[m]];
+            my[m] = D.sy[ids[m]];
         }
     }
     const uint32_t t = ids[0] - D.fstart[c0.f[0]];
@@ -1134,7 +1142,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                     bool ok = valid;
                     for (uint32_t mm = 1; mm < km1 && ok; mm++) {
                         const uint32_t id = rp[mm];
-                        if (sqdist(D.x[id], D.y[id], qx, qy) > r2) ok = false;
+                        if (sqdist(D.sx[id], D.sy[id], qx, qy) > r2) ok = false;
                     }
                     const uint64_t bal = __ballot(ok);
                     if (!WRITE) {
@@ -1151,7 +1159,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                         if (rank < wsl[row]) {
                             uint32_t *out = cd.table + (cd.offsets[row] - cd.off_base + rank) * k;
                             for (uint32_t mm = 0; mm < km1; mm++) out[mm] = rp[mm];
-                            out[km1] = D.sidx[q];
+                            out[km1] = q;
                         }
                     }
                 }
@@ -1197,8 +1205,8 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                 for (uint32_t mm = 1; mm < (uint32_t)kArr; mm++) {
                     if (mm >= km1) break;
                     const uint32_t id = rp[mm];
-                    mx[mm] = D.x[id];
-                    my[mm] = D.y[id];
+                    mx[mm] = D.sx[id];
+                    my[mm] = D.sy[id];
                 }
                 uint32_t cnt = 0;
                 auto step = [&](uint32_t e, uint32_t q, float qx, float qy, bool &h) {
@@ -1216,7 +1224,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                         if (rank < nr) {
                             uint32_t *out = cd.table + (base + rank) * k;
                             for (uint32_t mm = 0; mm < km1; mm++) out[mm] = rp[mm];
-                            out[km1] = D.sidx[q];
+                            out[km1] = q;
                         }
                     }
                     cnt += (uint32_t)__popcll(bal);
@@ -1263,7 +1271,7 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
         uint32_t *bt = bits + cd.bits_off;
         const uint32_t a = cd.f[0], b = cd.f[1];
         for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-            join2_count_anchor(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+            join2_count_anchor<true>(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
     }
 }
 
@@ -1310,8 +1318,8 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
         if (m >= km1) break;
         ids[m] = row[m];
         if (m) {
-            mx[m] = D.x[ids[m]];
-            my[m] = D.y[ids[m]];
+            mx[m] = D.sx[ids[m]];
+            my[m] = D.sy[ids[m]];
         }
     }
     const uint32_t t = ids[0] - D.fstart[c0.f[0]];
@@ -1337,7 +1345,7 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
                     if (m >= km1) break;
                     out[(uint64_t)w * k + m] = ids[m];
                 }
-                out[(uint64_t)w * k + km1] = D.sidx[q];
+                out[(uint64_t)w * k + km1] = q;
                 w++;
             }
         }
@@ -1353,7 +1361,7 @@ __global__ __launch_bounds__(kBlock) void k_write2_batch(DevData D, const CandDe
         const uint32_t *sl = cd.wslots ? cd.wslots : slots + cd.slots_off;
         for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
             uint32_t nr = sl[t];
-            if (nr) join2_write_anchor(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
+            if (nr) join2_write_anchor<true>(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
         }
     }
 }
@@ -1512,17 +1520,18 @@ void colo::seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint6
     if (words_total) *words_total = o - bits_off;
 }
 
-void colo::seg_descs(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, SegDesc *out) {
+void colo::seg_descs(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, SegDesc *out,
+                     bool all_sorted) {
     uint64_t so[kMaxK];
     seg_layout(c, k, cand, bits_off, so, nullptr);
     for (uint32_t j = 0; j < k; j++) {
         uint32_t f = cand[j], N = c->fcount[f];
         out[j].off = so[j];
         out[j].words = (N + 31) / 32;
-        if (j + 1 < k) {  // input-rank indexed
+        if (j + 1 < k && !all_sorted) {  // input-rank indexed
             out[j].first_bit = 0;
             out[j].last_bit = N - 1;
-        } else {          // last feature: sorted-rank indexed
+        } else {          // last feature (every feature with all_sorted): sorted-rank indexed
             out[j].first_bit = c->first_sorted[f];
             out[j].last_bit = c->last_sorted[f];
         }
@@ -2101,7 +2110,9 @@ int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_off
 }
 }  // namespace colo
 
-// Position-sensitive table checksum: sum over i of mix64((i << 32) ^ t[i]).
+// Table checksum independent of row order: the 64-bit sum over rows of a
+// hash of the row's instance ids in column order (positions are mapped to
+// instance ids through map when given).
 __device__ inline uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -2109,10 +2120,17 @@ __device__ inline uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-__global__ __launch_bounds__(kBlock) void k_table_hash(const uint32_t *t, uint64_t n, unsigned long long *out) {
+__global__ __launch_bounds__(kBlock) void k_table_hash(const uint32_t *t, uint64_t rows, uint32_t k,
+                                                       const uint32_t *map, unsigned long long *out) {
     uint64_t s = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-        s += mix64((i << 32) ^ t[i]);
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < rows; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t h = 0x2545F4914F6CDD1Dull ^ k;
+        for (uint32_t m = 0; m < k; m++) {
+            const uint32_t v = map ? map[t[i * k + m]] : t[i * k + m];
+            h = mix64(h ^ (((uint64_t)m << 32) | v));
+        }
+        s += h;
+    }
     uint64_t ex;
     uint64_t tot = block_excl_scan(s, ex);
     if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)tot);
@@ -2124,14 +2142,15 @@ __global__ void k_gather_u64(const uint64_t *a, const uint64_t *idx, uint64_t m,
 }
 
 namespace colo {
-int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t n, uint64_t *h, hipStream_t s) {
+int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, const uint32_t *d_map, uint64_t *h,
+               hipStream_t s) {
     colo_ctx *ctx_for_err = c;
     void *tmp;
     int rc = scratch(c, 16, &tmp);
     if (rc) return rc;
     if ((rc = fill_u32((uint32_t *)tmp, 2, 0, s))) return rc;
-    if (n) hipLaunchKernelGGL(k_table_hash, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, d_t, n,
-                              (unsigned long long *)tmp);
+    if (rows) hipLaunchKernelGGL(k_table_hash, dim3(grid_for(rows, 2048)), dim3(kBlock), 0, s, d_t, rows, k, d_map,
+                                 (unsigned long long *)tmp);
     COLO_HIP(hipGetLastError());
     COLO_HIP(hipMemcpyAsync(c->h_pinned, tmp, 8, hipMemcpyDeviceToHost, s));
     COLO_HIP(hipStreamSynchronize(s));

@@ -231,7 +231,10 @@ int write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem 
                 uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
 int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
                  hipStream_t s);
-int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t n, uint64_t *h, hipStream_t s);
+// Row-order independent checksum of a rows x k table; d_map (optional) maps
+// stored sorted positions to instance ids.
+int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, const uint32_t *d_map, uint64_t *h,
+               hipStream_t s);
 int gather_u64(colo_ctx *c, const uint64_t *d_a, const uint64_t *d_idx, uint64_t m, uint64_t *d_out, hipStream_t s);
 int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s);
 int seg_sums(colo_ctx *c, const uint32_t *d_slots, const uint64_t *seg, uint64_t nseg, uint64_t *d_out,
@@ -239,7 +242,10 @@ int seg_sums(colo_ctx *c, const uint32_t *d_slots, const uint64_t *seg, uint64_t
 // Segment descriptors of one candidate's bitmap (bits_off = base word).
 void seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off,
                 uint64_t *seg_off, uint64_t *words_total);
-void seg_descs(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, SegDesc *out);
+// all_sorted: every segment indexed by sorted rank (batched miner); else
+// segments 0..k-2 by input rank and the last by sorted rank.
+void seg_descs(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, SegDesc *out,
+               bool all_sorted = false);
 void pi_from_segs(const colo_ctx *c, uint32_t k, const uint32_t *cand, const SegOut *so,
                   uint64_t *num_compat, uint64_t *num_exact, float *pi_compat, float *pi_exact);
 int num_cus();

@@ -597,7 +597,7 @@ struct Miner {
         COLO_HIP(hipMemcpyAsync(rows_of.data(), d_rowsel.p, m * 8, hipMemcpyDeviceToHost, s));
         // PI segments of every candidate, one launch
         std::vector<SegDesc> segs(m * k);
-        for (uint64_t u = 0; u < m; u++) seg_descs(c, k, &C[idx[u] * k], desc[u].bits_off, &segs[u * k]);
+        for (uint64_t u = 0; u < m; u++) seg_descs(c, k, &C[idx[u] * k], desc[u].bits_off, &segs[u * k], true);
         if (!d_segs.ensure(segs.size() * sizeof(SegDesc)) || !d_segout.ensure(segs.size() * sizeof(SegOut)))
             return set_error(c, COLO_ENOMEM, "segments");
         COLO_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
@@ -1083,7 +1083,9 @@ struct Miner {
                 rep.printf(" rows %llu", (unsigned long long)p.rows);
                 if (o.hash_tables) {
                     uint64_t h = 0;
-                    if ((rc = table_hash(c, p.tptr, p.rows * k, &h, s))) return rc;
+                    // batched tables hold sorted positions, plain ones instance ids
+                    const uint32_t *map = o.sched == COLO_SCHED_PLAIN ? nullptr : c->d_sidx;
+                    if ((rc = table_hash(c, p.tptr, p.rows, k, map, &h, s))) return rc;
                     rep.printf(" h %016llx", (unsigned long long)h);
                 }
                 rep.printf("\n");

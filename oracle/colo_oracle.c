@@ -757,16 +757,24 @@ uint64_t orc_candgen(const uint32_t *L, uint64_t P, uint32_t km1, uint32_t *out,
 
 /* ---------------------------------------------------------------- hashes */
 
-/* Position-sensitive table checksum: sum over i of mix(i, v_i), 64-bit. */
+/* Table checksum independent of row order: the 64-bit sum over rows of a
+ * hash of the row's instance ids in column order (splitmix64 finaliser as
+ * the mixing step).  A report carries it per instance table, so two miners
+ * that build the same rows in different orders report the same value. */
 static inline uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
-uint64_t orc_table_hash(const uint32_t *t, uint64_t n) {
+uint64_t orc_row_hash(const uint32_t *row, uint32_t k) {
+    uint64_t h = 0x2545F4914F6CDD1Dull ^ k;
+    for (uint32_t m = 0; m < k; m++) h = mix64(h ^ (((uint64_t)m << 32) | row[m]));
+    return h;
+}
+uint64_t orc_table_hash(const uint32_t *t, uint64_t rows, uint32_t k) {
     uint64_t h = 0;
-    for (uint64_t i = 0; i < n; i++) h += mix64((i << 32) ^ t[i]);
+    for (uint64_t i = 0; i < rows; i++) h += orc_row_hash(t + i * k, k);
     return h;
 }
 
@@ -956,7 +964,7 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
             sb_printf(&b, "t");
             for (uint32_t m = 0; m < k; m++) sb_printf(&b, " %u", L[i].pat[m]);
             sb_printf(&b, " rows %llu", (unsigned long long)L[i].rows);
-            if (hash_tables) sb_printf(&b, " h %016llx", (unsigned long long)orc_table_hash(L[i].table, L[i].rows * k));
+            if (hash_tables) sb_printf(&b, " h %016llx", (unsigned long long)orc_table_hash(L[i].table, L[i].rows, k));
             sb_printf(&b, "\n");
         }
         free(need);

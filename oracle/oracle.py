@@ -74,7 +74,9 @@ def lib():
         L.orc_candgen.restype = C.c_uint64
         L.orc_candgen.argtypes = [_u32p, C.c_uint64, C.c_uint32, vp, C.c_uint64]
         L.orc_table_hash.restype = C.c_uint64
-        L.orc_table_hash.argtypes = [_u32p, C.c_uint64]
+        L.orc_table_hash.argtypes = [_u32p, C.c_uint64, C.c_uint32]
+        L.orc_row_hash.restype = C.c_uint64
+        L.orc_row_hash.argtypes = [_u32p, C.c_uint32]
         L.orc_mine.restype = C.c_void_p
         L.orc_mine.argtypes = [vp, C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                C.c_char_p, C.POINTER(C.c_int)]
@@ -244,8 +246,16 @@ def candgen(L, km1):
 
 
 def table_hash(t):
+    """Row-order independent checksum of a rows x k table (orc_table_hash)."""
     t = np.ascontiguousarray(t, np.uint32)
-    return lib().orc_table_hash(t, len(t))
+    if t.ndim != 2:
+        raise ValueError("table_hash takes a rows x k table")
+    return lib().orc_table_hash(t.reshape(-1), t.shape[0], t.shape[1])
+
+
+def row_hash(row):
+    row = np.ascontiguousarray(row, np.uint32)
+    return lib().orc_row_hash(row, len(row))
 
 
 def load_params(path):

@@ -318,3 +318,21 @@ def test_csv_malformed(oracle_mod, tmp_path, body):
     path.write_text(body)
     with pytest.raises(ValueError):
         oracle_mod.Oracle.from_csv(str(path))
+
+
+def test_table_hash_properties():
+    """The report's table checksum: a sum of per-row hashes (the value of an
+    empty table is 0), so it ignores row order, and it changes with any
+    instance id, with column order and with a duplicated row."""
+    import oracle
+    rng = np.random.default_rng(7)
+    t = rng.integers(0, 1 << 31, size=(500, 3)).astype(np.uint32)
+    h = oracle.table_hash(t)
+    assert oracle.table_hash(t[rng.permutation(len(t))]) == h
+    assert oracle.table_hash(t[:0]) == 0
+    assert h == sum(oracle.row_hash(r) for r in t) % (1 << 64)
+    u = t.copy()
+    u[123, 1] ^= 1
+    assert oracle.table_hash(u) != h
+    assert oracle.table_hash(t[:, [1, 0, 2]]) != h
+    assert oracle.table_hash(np.vstack([t, t[:1]])) != h

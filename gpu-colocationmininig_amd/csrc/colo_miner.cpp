@@ -533,11 +533,14 @@ struct Miner {
         if (pred) {
             double total = 0;
             for (auto &r : runs) total += cost[r.first];
-            double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 4));
+            // about 16 items per workgroup at equal predicted cost, each
+            // between 1/4 and 4x the tuned item size
+            double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 16));
+            const double lo = std::max(64.0, c->tune.item_rows / 4.0), hi = c->tune.item_rows * 4.0;
             for (auto &r : runs) {
                 uint64_t n = desc[r.first].n_anchor;
                 double per_row = n ? cost[r.first] / (double)n : 1.0;
-                uint64_t chunk = (uint64_t)std::min(16384.0, std::max(256.0, target / std::max(per_row, 1e-9)));
+                uint64_t chunk = (uint64_t)std::min(hi, std::max(lo, target / std::max(per_row, 1e-9)));
                 size_t before = items.size();
                 make_items(r.first, n, chunk, items, r.second);
                 for (size_t i = before; i < items.size(); i++) iw.push_back(per_row * (items[i].end - items[i].begin));
@@ -1382,8 +1385,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
         std::vector<Axis> axes = {{&TuneParams::count_var, {0, 1}, 0},
                                   {&TuneParams::write_var, {0, 1}, 1},
                                   {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
-        if (o.sched != COLO_SCHED_LEARNED || !o.model)  // the cost model cuts its own items
-            axes.push_back({&TuneParams::item_rows, {256, 1024, 4096}, 2});
+        axes.push_back({&TuneParams::item_rows, {256, 1024, 4096}, 2});
         for (const Axis &ax : axes) {
             const uint32_t start = best.*(ax.field);
             for (uint32_t v : ax.values) {

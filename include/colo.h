@@ -235,8 +235,9 @@ typedef struct colo_mine_stats {
  *                         chunks (a row group's neighbour list spread over
  *                         the lanes)
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
- *   item_rows             rows per work-queue item when no cost model
- *                         partitions the queue (64..65536) */
+ *   item_rows             rows per work-queue item; with the cost model,
+ *                         items are cut at equal predicted cost between
+ *                         item_rows / 4 and 4 x item_rows (64..65536) */
 typedef struct colo_tune {
     uint32_t count_var, write_var, blocks_per_cu, item_rows;
 } colo_tune;

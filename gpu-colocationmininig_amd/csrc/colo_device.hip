@@ -1278,8 +1278,10 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // Level k >= 3 (KM1 = k - 1, 0 for any k): the item's parent rows in chunks
 // of 64, taken by the block's waves in turn.
 // Level k >= 3 (KM1 = k - 1; 0 for any k).  VAR 0: one parent row per
-// thread; VAR 1: wave-cooperative (wave_rows_multi) over 64-row chunks
-// taken by the block's waves in turn.
+// thread, all of the item's candidates per row; VAR 2: one parent row per
+// thread, the item's candidates one after another (one neighbour-list
+// stream at a time); VAR 1: wave-cooperative (wave_rows_multi) over 64-row
+// chunks taken by the block's waves in turn.
 template <int KM1, int VAR>
 __global__ __launch_bounds__(kBlock) void k_countk_batch(DevData D, const CandDesc *desc, const WorkItem *items,
                                                          const uint64_t *seg, uint32_t *slots, uint32_t *bits,
@@ -1290,6 +1292,10 @@ __global__ __launch_bounds__(kBlock) void k_countk_batch(DevData D, const CandDe
         if (VAR == 0) {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                 joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits);
+        } else if (VAR == 2) {
+            for (uint32_t j = 0; j < wi.ncand; j++)
+                for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+                    joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits);
         } else {
             for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
                 const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
@@ -1375,6 +1381,10 @@ __global__ __launch_bounds__(kBlock) void k_writek_batch(DevData D, const CandDe
         if (VAR == 0) {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                 joink_write_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots);
+        } else if (VAR == 2) {
+            for (uint32_t j = 0; j < wi.ncand; j++)
+                for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+                    joink_write_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots);
         } else {
             for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
                 const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
@@ -1406,8 +1416,10 @@ static auto pick_k(uint32_t k) -> decltype(Pick<0>::get()) {
 }
 template <int KM1> struct PickCount0 { static auto get() { return &k_countk_batch<KM1, 0>; } };
 template <int KM1> struct PickCount1 { static auto get() { return &k_countk_batch<KM1, 1>; } };
+template <int KM1> struct PickCount2 { static auto get() { return &k_countk_batch<KM1, 2>; } };
 template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
 template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
+template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
 
 int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                       uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s) {
@@ -1418,7 +1430,8 @@ int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
         hipLaunchKernelGGL(k_count2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
                            d_items, d_seg, d_slots, d_bits, d_queue);
     else
-        hipLaunchKernelGGL(c->tune.count_var ? pick_k<PickCount1>(k) : pick_k<PickCount0>(k),
+        hipLaunchKernelGGL(c->tune.count_var == 2 ? pick_k<PickCount2>(k)
+                           : c->tune.count_var == 1 ? pick_k<PickCount1>(k) : pick_k<PickCount0>(k),
                            dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
                            d_slots, d_bits, d_queue);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "count_batch launch");
@@ -1433,7 +1446,8 @@ int colo::write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
         hipLaunchKernelGGL(k_write2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
                            d_items, d_seg, d_slots, d_queue);
     else
-        hipLaunchKernelGGL(c->tune.write_var ? pick_k<PickWrite1>(k) : pick_k<PickWrite0>(k),
+        hipLaunchKernelGGL(c->tune.write_var == 2 ? pick_k<PickWrite2>(k)
+                           : c->tune.write_var == 1 ? pick_k<PickWrite1>(k) : pick_k<PickWrite0>(k),
                            dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
                            d_slots, d_queue);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "write_batch launch");
@@ -1831,7 +1845,7 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
 
 int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
-    if (t->count_var > 1 || t->write_var > 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+    if (t->count_var > 2 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;

@@ -173,7 +173,8 @@ struct DevPool {
 
 // Launch shapes of the batched passes (chosen by the autotuner).
 struct TuneParams {
-    uint32_t count_var = 0;       // level-k count: 0 one row per thread, 1 wave-cooperative
+    uint32_t count_var = 0;       // level-k count: 0 row per thread, 1 wave-cooperative,
+                                  // 2 row per thread, candidates in turn
     uint32_t write_var = 0;       // level-k write: same choice
     uint32_t blocks_per_cu = 8;   // persistent grid: workgroups per CU
     uint32_t item_rows = 1024;    // rows (anchors) per work item without the cost model

@@ -1382,8 +1382,8 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             std::vector<uint32_t> values;
             int metric;
         };
-        std::vector<Axis> axes = {{&TuneParams::count_var, {0, 1}, 0},
-                                  {&TuneParams::write_var, {0, 1}, 1},
+        std::vector<Axis> axes = {{&TuneParams::count_var, {0, 1, 2}, 0},
+                                  {&TuneParams::write_var, {0, 1, 2}, 1},
                                   {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
         axes.push_back({&TuneParams::item_rows, {256, 1024, 4096}, 2});
         for (const Axis &ax : axes) {

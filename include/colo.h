@@ -231,9 +231,11 @@ typedef struct colo_mine_stats {
 /* Launch shapes of the batched (persistent) passes.  No counterpart in the
  * reference, whose kernels have fixed 1-D launches.
  *   count_var, write_var  level-k count / write pass: 0 = one parent row
- *                         per thread, 1 = wave-cooperative over 64-row
- *                         chunks (a row group's neighbour list spread over
- *                         the lanes)
+ *                         per thread, all candidates of a work item per
+ *                         row; 1 = wave-cooperative over 64-row chunks (a
+ *                         row group's neighbour list spread over the
+ *                         lanes); 2 = one parent row per thread, the work
+ *                         item's candidates one after another
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between

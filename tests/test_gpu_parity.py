@@ -278,7 +278,8 @@ def test_threshold_sweep(c2small, sched):
 
 
 @pytest.mark.parametrize("tune", [dict(count_var=1, write_var=1), dict(blocks_per_cu=4, item_rows=64),
-                                  dict(count_var=1, write_var=0, blocks_per_cu=16, item_rows=4096)])
+                                  dict(count_var=1, write_var=0, blocks_per_cu=16, item_rows=4096),
+                                  dict(count_var=2, write_var=2)])
 def test_launch_shapes_identical(c2small, tune):
     """Every launch shape of the batched passes gives the oracle's report."""
     ctx, orc, p = c2small

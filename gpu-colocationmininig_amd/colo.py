@@ -74,7 +74,7 @@ class MineStats(C.Structure):
 _lib = None
 
 SYMBOLS = [
-    "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_get_tune", "colo_set_tune", "colo_autotune", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
+    "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_get_tune", "colo_set_tune", "colo_autotune", "colo_set_graphs", "colo_graph_info", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_mine_sweep", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
@@ -94,6 +94,8 @@ def lib():
         L.colo_ctx_create.argtypes = [C.POINTER(Dataset), f32, vp, C.POINTER(vp)]
         L.colo_ctx_destroy.argtypes = [vp]
         L.colo_trim.argtypes = [vp]
+        L.colo_set_graphs.argtypes = [vp, C.c_int]
+        L.colo_graph_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.colo_get_tune.argtypes = [vp, C.POINTER(Tune)]
         L.colo_set_tune.argtypes = [vp, C.POINTER(Tune)]
         L.colo_autotune.argtypes = [vp, C.POINTER(MineOpts), C.c_char_p, C.POINTER(Tune)]
@@ -237,6 +239,16 @@ class Context:
         if getattr(self, "h", None):
             lib().colo_ctx_destroy(self.h)
             self.h = None
+
+    def set_graphs(self, on):
+        """Binning replayed from a captured HIP graph (on) or launched eagerly."""
+        self._check(lib().colo_set_graphs(self.h, int(bool(on))))
+
+    def graph_info(self):
+        """(captures, replays) of the binning graph so far."""
+        a, b = C.c_uint32(), C.c_uint32()
+        self._check(lib().colo_graph_info(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     @property
     def tune(self):

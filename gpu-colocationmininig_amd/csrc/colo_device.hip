@@ -381,10 +381,10 @@ __global__ void k_runs(const uint32_t *run_flag, const uint64_t *run_idx, const 
 }
 
 // Presence mask of every cell (bit f set when feature f occurs), from the runs.
-__global__ void k_cellmask(const uint32_t *run_cell, const uint64_t *run_off, uint32_t F, uint64_t H,
+__global__ void k_cellmask(const uint32_t *run_cell, const uint64_t *run_off, uint32_t F, const uint64_t *dH,
                            unsigned long long *cellmask) {
     uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
+    if (h >= *dH) return;
     uint32_t lo = 0, hi = F;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
@@ -403,9 +403,23 @@ __global__ void k_qmask(const uint64_t *cellmask, uint32_t gx, uint32_t gy, uint
     qmask[q] = cellmask[c0] | cellmask[c0 + 1] | cellmask[c0 + gx] | cellmask[c0 + gx + 1];
 }
 
-__global__ void k_u64_at(const uint64_t *a, const uint32_t *idx, uint32_t m, uint64_t *out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) out[i] = a[idx[i]];
+// Binning results for the host, in one block (u64 words): [0] bounds flag,
+// [1] number of runs H, [2 .. 2+F] run offsets per feature (H last),
+// [3+F .. 3+2F) sorted rank of each feature's first input instance,
+// [3+2F .. 3+3F) of its last.
+__global__ void k_bin_tail(const uint32_t *flag, const uint64_t *run_idx, const uint32_t *fstart,
+                           const uint32_t *fcount, uint32_t F, const uint32_t *spos, uint64_t *out) {
+    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t H = out[1];  // written by the run scan
+    if (f == 0) {
+        out[0] = *flag;
+        out[2 + F] = H;
+    }
+    if (f >= F) return;
+    const uint32_t s0 = fstart[f], e = s0 + fcount[f] - 1;
+    out[2 + f] = run_idx[s0];
+    out[3 + F + f] = spos[s0] - s0;
+    out[3 + 2 * F + f] = spos[e] - s0;
 }
 
 // ------------------------------------------------------------ filter
@@ -1606,38 +1620,66 @@ static uint32_t bits_for(uint64_t v) {
     return b;
 }
 
+static void free_bin_bufs(colo_ctx *c) {
+    if (c->bin_exec) (void)hipGraphExecDestroy(c->bin_exec);
+    if (c->bin_graph) (void)hipGraphDestroy(c->bin_graph);
+    c->bin_exec = nullptr;
+    c->bin_graph = nullptr;
+    void *ps[] = {c->b_keys, c->b_keys2, c->b_vals2, c->b_hist, c->b_runidx, c->b_stmp, c->b_small};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    c->b_keys = c->b_keys2 = c->b_vals2 = c->b_hist = c->b_runidx = nullptr;
+    c->b_stmp = c->b_small = nullptr;
+}
+
+static uint64_t bin_tmp_words(const colo_ctx *c) {
+    const uint64_t n = c->n;
+    const uint64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    return std::max<uint64_t>(std::max(scan_tmp_words(ntiles * 256), scan_tmp_words(std::max<uint64_t>(c->dir_len, n))),
+                              (uint64_t)c->F + 3);
+}
+
+// Work buffers of the binning for the current grid.
+static int alloc_bin_bufs(colo_ctx *c, uint32_t keybytes) {
+    int rc;
+    const uint64_t n = c->n;
+    const uint64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    if ((rc = alloc_dev(c, &c->b_keys, keybytes * n))) return rc;
+    if ((rc = alloc_dev(c, &c->b_keys2, keybytes * n))) return rc;
+    if ((rc = alloc_dev(c, &c->b_vals2, 4 * n))) return rc;
+    if ((rc = alloc_dev(c, &c->b_hist, 4 * 256 * (ntiles ? ntiles : 1)))) return rc;
+    if (keybytes < 8 && (rc = alloc_dev(c, &c->b_runidx, 8 * n + 8))) return rc;
+    if ((rc = alloc_dev(c, (void **)&c->b_stmp, 8 * bin_tmp_words(c)))) return rc;
+    if ((rc = alloc_dev(c, (void **)&c->b_small, 8 * (3 + 3ull * c->F)))) return rc;
+    if (!c->h_bin && hipHostMalloc((void **)&c->h_bin, 8 * (3 + 3ull * c->F)) != hipSuccess) {
+        c->h_bin = nullptr;
+        return set_error(c, COLO_ENOMEM, "pinned binning results");
+    }
+    c->b_keybytes = keybytes;
+    return COLO_OK;
+}
+
+// The binning as launches and copies only (no allocation, no host wait),
+// so that it can be captured: cell keys, stable radix sort by (feature,
+// cell), sorted gathers, the dense (feature, cell) directory, the runs of
+// distinct (feature, cell) pairs, cell and quadrant masks, and the results
+// block copied to pinned host memory.
 template <typename K>
-static int bin_impl(colo_ctx *c, hipStream_t s) {
+static int bin_launch(colo_ctx *c, hipStream_t s) {
     colo_ctx *ctx_for_err = c;
     const uint32_t n = c->n;
-    K *keys = nullptr, *keys2 = nullptr;
-    uint32_t *vals2 = nullptr;
+    K *keys = (K *)c->b_keys, *keys2 = (K *)c->b_keys2;
+    uint32_t *vals2 = (uint32_t *)c->b_vals2, *hist = (uint32_t *)c->b_hist;
+    uint64_t *stmp = c->b_stmp, *small = c->b_small;
     int rc;
-    if ((rc = alloc_dev(c, (void **)&keys, sizeof(K) * (uint64_t)n))) return rc;
-    if ((rc = alloc_dev(c, (void **)&keys2, sizeof(K) * (uint64_t)n))) return rc;
-    if ((rc = alloc_dev(c, (void **)&vals2, 4ull * n))) return rc;
-    uint32_t ntiles = (uint32_t)((n + kRadixTile - 1) / kRadixTile);
-    uint32_t *hist = nullptr;
-    if ((rc = alloc_dev(c, (void **)&hist, 4ull * 256 * (ntiles ? ntiles : 1)))) return rc;
-    // scan temporaries; also holds the F+1 per-feature run offsets, plus the
-    // run count in its last word
-    uint64_t tmpw = std::max(scan_tmp_words((uint64_t)ntiles * 256), scan_tmp_words(std::max<uint64_t>(c->dir_len, n)));
-    tmpw = std::max<uint64_t>(tmpw, (uint64_t)c->F + 3);
-    uint64_t *stmp = nullptr;
-    if ((rc = alloc_dev(c, (void **)&stmp, 8 * tmpw))) return rc;
     DevGrid g = c->dev().g;
+    if ((rc = fill_u32(c->d_flag, 4, 0, s))) return rc;
+    if ((rc = fill_u32((uint32_t *)small, 2 * (3 + 3ull * c->F), 0, s))) return rc;
+    K *sk = keys;
     if (n) {
         hipLaunchKernelGGL((k_cell_keys<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->d_x, c->d_y, n,
                            c->d_fstart, c->F, g, c->cellbits, c->d_cell_of, keys, c->d_sidx, c->d_flag);
-        COLO_HIP(hipGetLastError());
-    }
-    uint32_t hflag = 0;
-    COLO_HIP(hipMemcpyAsync(&hflag, c->d_flag, 4, hipMemcpyDeviceToHost, s));
-    COLO_HIP(hipStreamSynchronize(s));
-    if (hflag) return set_error(c, COLO_EBOUNDS, "a point lies outside the grid (cell id >= gx*gy)");
-    K *sk = keys;
-    uint32_t *sv = c->d_sidx;
-    if (n) {
+        uint32_t *sv = c->d_sidx;
         int bits = (int)(c->cellbits + c->fidbits);
         if ((rc = radix_sort<K>(c, keys, c->d_sidx, keys2, vals2, n, bits, hist, stmp, &sk, &sv, s))) return rc;
         if (sv != c->d_sidx) COLO_HIP(hipMemcpyAsync(c->d_sidx, sv, 4ull * n, hipMemcpyDeviceToDevice, s));
@@ -1646,80 +1688,94 @@ static int bin_impl(colo_ctx *c, hipStream_t s) {
     }
     // directory: counts per (f, cell) then exclusive scan in place
     if ((rc = fill_u32(c->d_dir, c->dir_len, 0, s))) return rc;
-    uint32_t *run_flag = vals2;  // reuse
-    if (n) {
+    uint32_t *run_flag = vals2;  // free once the sorted indices are in d_sidx
+    if (n)
         hipLaunchKernelGGL((k_dir_counts<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk, n,
                            c->cellbits, c->cells, c->d_dir, run_flag);
-    }
     scan_rec<uint32_t, uint32_t>(c->d_dir, c->dir_len, c->d_dir, nullptr, stmp, s);
-    COLO_HIP(hipGetLastError());
-    // runs (distinct (feature, cell) pairs) for the filter
-    uint64_t *run_idx = (uint64_t *)keys2;  // reuse: needs 8n bytes
-    uint64_t *run_idx_buf = nullptr;
-    if (sizeof(K) < 8) {
-        if ((rc = alloc_dev(c, (void **)&run_idx_buf, 8ull * n + 8))) return rc;
-        run_idx = run_idx_buf;
-    }
-    uint64_t *d_H = stmp + tmpw - 1;
-    if (n) scan_rec<uint32_t, uint64_t>(run_flag, n, run_idx, d_H, stmp, s);
-    uint64_t H = 0;
+    // runs (distinct (feature, cell) pairs) for the filter; H lands in small[1]
+    uint64_t *run_idx = sizeof(K) < 8 ? (uint64_t *)c->b_runidx : (uint64_t *)keys2;
     if (n) {
-        COLO_HIP(hipMemcpyAsync(&H, d_H, 8, hipMemcpyDeviceToHost, s));
-        COLO_HIP(hipStreamSynchronize(s));
+        scan_rec<uint32_t, uint64_t>(run_flag, n, run_idx, small + 1, stmp, s);
+        hipLaunchKernelGGL(k_runs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, run_flag, run_idx,
+                           c->d_scell, n, c->d_run_cell);
     }
-    c->H = H;
-    if ((rc = alloc_dev(c, (void **)&c->d_run_cell, 4 * (H ? H : 1)))) return rc;
-    if (n) hipLaunchKernelGGL(k_runs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, run_flag, run_idx,
-                              c->d_scell, n, c->d_run_cell);
-    // run offsets per feature: run index at sorted position fstart[f]
-    c->run_off.assign(c->F + 1, H);
-    if (c->F && n) {
-        uint64_t *d_ro = stmp;
-        hipLaunchKernelGGL(k_u64_at, dim3((c->F + kBlock - 1) / kBlock), dim3(kBlock), 0, s, run_idx, c->d_fstart,
-                           c->F, d_ro);
-        COLO_HIP(hipMemcpyAsync(c->run_off.data(), d_ro, 8ull * c->F, hipMemcpyDeviceToHost, s));
-        COLO_HIP(hipStreamSynchronize(s));
-        c->run_off[c->F] = H;
-    }
+    if (c->F && n)
+        hipLaunchKernelGGL(k_bin_tail, dim3((c->F + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->d_flag, run_idx,
+                           c->d_fstart, c->d_fcount, c->F, c->d_spos, small);
     // cell and quadrant masks (F <= 64)
     if (c->F <= 64) {
-        if ((rc = alloc_dev(c, (void **)&c->d_cellmask, 8ull * c->cells))) return rc;
         uint64_t nq = (c->gx >= 2 && c->gy >= 2) ? (uint64_t)(c->gx - 1) * (c->gy - 1) : 0;
-        if ((rc = alloc_dev(c, (void **)&c->d_qmask, 8 * (nq ? nq : 1)))) return rc;
         if ((rc = fill_u32((uint32_t *)c->d_cellmask, 2ull * c->cells, 0, s))) return rc;
-        uint64_t *d_roff = stmp;
-        COLO_HIP(hipMemcpyAsync(d_roff, c->run_off.data(), 8ull * (c->F + 1), hipMemcpyHostToDevice, s));
-        if (H) hipLaunchKernelGGL(k_cellmask, dim3(grid_for(H, 1u << 30)), dim3(kBlock), 0, s, c->d_run_cell,
-                                  d_roff, c->F, H, (unsigned long long *)c->d_cellmask);
+        if (n)
+            hipLaunchKernelGGL(k_cellmask, dim3(grid_for(n, 1u << 30)), dim3(kBlock), 0, s, c->d_run_cell, small + 2,
+                               c->F, small + 1, (unsigned long long *)c->d_cellmask);
         if (nq) hipLaunchKernelGGL(k_qmask, dim3(grid_for(nq, 1u << 30)), dim3(kBlock), 0, s, c->d_cellmask, c->gx,
                                    c->gy, c->d_qmask);
-        COLO_HIP(hipGetLastError());
     }
-    // sorted ranks of each feature's first and last input instance
-    c->first_sorted.assign(c->F, 0);
-    c->last_sorted.assign(c->F, 0);
-    if (c->F && n) {
-        std::vector<uint32_t> spos(n);
-        COLO_HIP(hipMemcpyAsync(spos.data(), c->d_spos, 4ull * n, hipMemcpyDeviceToHost, s));
-        COLO_HIP(hipStreamSynchronize(s));
-        for (uint32_t f = 0; f < c->F; f++) {
-            c->first_sorted[f] = spos[c->fstart[f]] - c->fstart[f];
-            c->last_sorted[f] = spos[c->fstart[f] + c->fcount[f] - 1] - c->fstart[f];
+    COLO_HIP(hipMemcpyAsync(c->h_bin, small, 8 * (3 + 3ull * c->F), hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipGetLastError());
+    return COLO_OK;
+}
+
+// Runs the binning for the current grid -- replaying its captured graph
+// when graphs are on -- and reads the results block.
+template <typename K>
+static int bin_impl(colo_ctx *c, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    int rc;
+    if (!c->b_keys || c->b_keybytes != sizeof(K)) {
+        free_bin_bufs(c);
+        if ((rc = alloc_bin_bufs(c, sizeof(K)))) return rc;
+    }
+    bool done = false;
+    if (c->use_graphs && s) {
+        if (!c->bin_exec) {
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                int lrc = bin_launch<K>(c, s);
+                hipGraph_t graph = nullptr;
+                hipError_t e = hipStreamEndCapture(s, &graph);
+                if (lrc == COLO_OK && e == hipSuccess && graph &&
+                    hipGraphInstantiate(&c->bin_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                    c->bin_graph = graph;
+                    c->graph_captures++;
+                } else {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    c->bin_exec = nullptr;
+                    c->use_graphs = false;  // capture unavailable here: eager from now on
+                }
+            } else {
+                c->use_graphs = false;
+            }
+            (void)hipGetLastError();
+        }
+        if (c->bin_exec) {
+            COLO_HIP(hipGraphLaunch(c->bin_exec, s));
+            c->graph_replays++;
+            done = true;
         }
     }
+    if (!done && (rc = bin_launch<K>(c, s))) return rc;
     COLO_HIP(hipStreamSynchronize(s));
-    (void)hipFree(keys);
-    (void)hipFree(keys2);
-    (void)hipFree(vals2);
-    (void)hipFree(hist);
-    (void)hipFree(stmp);
-    if (run_idx_buf) (void)hipFree(run_idx_buf);
+    const uint64_t *h = c->h_bin;
+    if (h[0]) return set_error(c, COLO_EBOUNDS, "a point lies outside the grid (cell id >= gx*gy)");
+    c->H = c->n ? h[1] : 0;
+    c->run_off.assign(c->F + 1, c->H);
+    c->first_sorted.assign(c->F, 0);
+    c->last_sorted.assign(c->F, 0);
+    if (c->n)
+        for (uint32_t f = 0; f < c->F; f++) {
+            c->run_off[f] = h[2 + f];
+            c->first_sorted[f] = (uint32_t)h[3 + c->F + f];
+            c->last_sorted[f] = (uint32_t)h[3 + 2 * c->F + f];
+        }
     return COLO_OK;
 }
 
 // setGrid (cpp:102-115) for distance r, binary32 throughout; (re)allocates
 // the r-dependent buffers.
 static int set_grid(colo_ctx *c, float r) {
+    free_bin_bufs(c);
     void *old[] = {c->d_dir, c->d_cellmask, c->d_qmask, c->d_run_cell};
     for (void *p : old)
         if (p) (void)hipFree(p);
@@ -1741,7 +1797,14 @@ static int set_grid(colo_ctx *c, float r) {
     c->cellbits = std::max<uint32_t>(1, bits_for(c->cells));
     c->fidbits = bits_for(c->F ? c->F - 1 : 0);
     c->dir_len = (uint64_t)c->F * (c->cells + 1ull);
-    return alloc_dev(c, (void **)&c->d_dir, 4 * c->dir_len);
+    int rc = alloc_dev(c, (void **)&c->d_dir, 4 * c->dir_len);
+    if (!rc) rc = alloc_dev(c, (void **)&c->d_run_cell, 4 * ((uint64_t)c->n + 1));  // runs <= n
+    if (!rc && c->F <= 64) {
+        uint64_t nq = (c->gx >= 2 && c->gy >= 2) ? (uint64_t)(c->gx - 1) * (c->gy - 1) : 0;
+        rc = alloc_dev(c, (void **)&c->d_cellmask, 8ull * c->cells);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_qmask, 8 * (nq ? nq : 1));
+    }
+    return rc;
 }
 
 extern "C" {
@@ -1751,9 +1814,11 @@ const char *colo_last_error(const colo_ctx *ctx) { return ctx ? ctx->err.c_str()
 int colo_bin(colo_ctx *c, float r, void *stream) {
     if (!c) return COLO_EARG;
     hipStream_t s = pick_stream(c, stream);
-    if (r != c->r) c->pairwork.clear();  // same data and r: same binning
-    int rc = set_grid(c, r);
-    if (!rc) rc = fill_u32(c->d_flag, 4, 0, s);
+    int rc = COLO_OK;
+    if (r != c->r || !c->d_dir) {  // new grid: new buffers and a new capture
+        c->pairwork.clear();
+        rc = set_grid(c, r);
+    }
     if (!rc) rc = (c->cellbits + c->fidbits <= 32) ? bin_impl<uint32_t>(c, s) : bin_impl<uint64_t>(c, s);
     return rc;
 }
@@ -1828,9 +1893,24 @@ int colo_ctx_destroy(colo_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    free_bin_bufs(c);
+    if (c->h_bin) (void)hipHostFree(c->h_bin);
     c->pool.release();
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return COLO_OK;
+}
+
+int colo_set_graphs(colo_ctx *c, int on) {
+    if (!c) return COLO_EARG;
+    c->use_graphs = on != 0;
+    return COLO_OK;
+}
+
+int colo_graph_info(const colo_ctx *c, uint32_t *captures, uint32_t *replays) {
+    if (!c) return COLO_EARG;
+    if (captures) *captures = c->graph_captures;
+    if (replays) *replays = c->graph_replays;
     return COLO_OK;
 }
 

@@ -207,6 +207,16 @@ struct colo_ctx {
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
     uint64_t *h_pinned = nullptr;   // small pinned host staging
+    // binning work buffers, kept while the grid is unchanged so that the
+    // binning launches can be captured once per grid and replayed
+    void *b_keys = nullptr, *b_keys2 = nullptr, *b_vals2 = nullptr, *b_hist = nullptr, *b_runidx = nullptr;
+    uint64_t *b_stmp = nullptr, *b_small = nullptr;  // scan temporaries; results block (see bin_launch)
+    uint64_t *h_bin = nullptr;      // pinned copy of the results block
+    uint32_t b_keybytes = 0;
+    hipGraph_t bin_graph = nullptr;
+    hipGraphExec_t bin_exec = nullptr;
+    bool use_graphs = true;
+    uint32_t graph_captures = 0, graph_replays = 0;
     std::vector<uint64_t> pairwork; // F x F 3x3 join work per (first, last), filled on first use
     colo::DevPool pool;             // miner buffers (tables, arenas, slots, offsets)
     std::string err;

@@ -226,6 +226,14 @@ typedef struct colo_mine_stats {
     double level_ms[16];    /* wall time of level k (index k)                 */
 } colo_mine_stats;
 
+/* HIP graphs: the binning (colo_bin, colo_ctx_create) is captured once per
+ * grid (distance r) as a graph and replayed by later colo_bin calls with the
+ * same r; on = 0 runs it eagerly.  Capture falls back to eager launches where
+ * the stream cannot be captured.  colo_graph_info reports the captures and
+ * replays so far.  No counterpart in the reference. */
+int colo_set_graphs(colo_ctx *ctx, int on);
+int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
+
 /* ------------------------------------------------------- launch shapes */
 
 /* Launch shapes of the batched (persistent) passes.  No counterpart in the

@@ -313,3 +313,37 @@ def test_autotune_and_cache(c2small, tmp_path):
         assert rep == orc.mine(p["r"], 0.15, max_k=4)
     finally:
         ctx.tune = old
+
+
+def test_binning_graph_replays(datagen, colo_mod, oracle_mod):
+    """The binning captured as a HIP graph and replayed by colo_bin, with
+    eager mining and synchronisations between the replays: every replay gives
+    the eager binning, and mining after each replay gives the oracle's report;
+    a new r makes a new capture."""
+    _say("binning graph replays")
+    x, y, fc, b, p = _workload(datagen, "c2", N=40000, seed=31)
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    expect = orc.mine(p["r"], 0.2, max_k=3)
+    eager = colo_mod.Context(x, y, fc, b, p["r"])
+    eager.set_graphs(False)
+    eager.rebin(p["r"])
+    ref = eager.binning()
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    caps0, reps0 = ctx.graph_info()
+    for i in range(3):
+        ctx.rebin(p["r"])
+        got = ctx.binning()
+        for a, e in zip(got, ref):
+            np.testing.assert_array_equal(a, e)
+        assert ctx.mine(0.2, max_k=3)[0] == expect, i
+    caps, reps = ctx.graph_info()
+    assert caps0 == 1 and caps == 1 and reps == reps0 + 3
+    ctx.rebin(p["r"] * 1.5)  # a new grid is a new capture
+    assert ctx.graph_info()[0] == 2
+    ctx.rebin(p["r"])
+    assert ctx.graph_info()[0] == 3
+    for a, e in zip(ctx.binning(), ref):
+        np.testing.assert_array_equal(a, e)
+    assert eager.graph_info()[1] == 1  # only the capture at creation was launched
+    assert eager.mine(0.2, max_k=3)[0] == expect

@@ -77,7 +77,7 @@ SYMBOLS = [
     "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_get_tune", "colo_set_tune", "colo_autotune", "colo_set_graphs", "colo_graph_info", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_mine_sweep", "colo_participation", "colo_mine", "colo_free_str",
-    "colo_run_config", "colo_load_csv", "colo_host_dataset", "colo_host_label", "colo_host_free",
+    "colo_run_config", "colo_load_csv", "colo_load_csv_opts", "colo_host_dataset", "colo_host_label", "colo_host_free",
     "colo_candidates", "colo_mlp_create", "colo_mlp_destroy", "colo_mlp_set_params", "colo_mlp_get_params",
     "colo_mlp_set_norm", "colo_mlp_predict", "colo_mlp_grad", "colo_mlp_train", "colo_mlp_save", "colo_mlp_load",
 ]
@@ -121,6 +121,7 @@ def lib():
         L.colo_free_str.argtypes = [vp]
         L.colo_run_config.argtypes = [C.c_char_p, C.POINTER(MineOpts), C.POINTER(vp)]
         L.colo_load_csv.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.colo_load_csv_opts.argtypes = [C.c_char_p, C.c_char_p, u32, C.POINTER(vp)]
         L.colo_host_dataset.argtypes = [vp, C.POINTER(Dataset)]
         L.colo_host_label.restype = C.c_char_p
         L.colo_host_label.argtypes = [vp, u32]
@@ -176,9 +177,16 @@ def _on_stream(fn):
 class HostData:
     """A parsed dataset CSV (colocationFinder::populateData, cpp:139-254)."""
 
-    def __init__(self, path):
+    def __init__(self, path, cache=None, threads=None):
+        """threads: host threads of the parse (None: one per MiB up to the
+        hardware's); cache: binary cache file reused while the CSV is
+        unchanged."""
         h = C.c_void_p()
-        rc = lib().colo_load_csv(path.encode(), C.byref(h))
+        if cache is None and threads is None:
+            rc = lib().colo_load_csv(path.encode(), C.byref(h))
+        else:
+            rc = lib().colo_load_csv_opts(path.encode(), cache.encode() if cache else None, int(threads or 0),
+                                          C.byref(h))
         if rc:
             raise ColoError(rc, f"cannot load {path}")
         self.h = h

@@ -162,6 +162,69 @@ __global__ void k_scan_apply(const Tin *in, uint64_t n, const uint64_t *carry, T
         *total = (carry ? carry[blockIdx.x] : 0) + tot;
 }
 
+// Segmented exclusive scans of many u32 arrays in three launches: tile
+// sums, per-segment carries (one workgroup per segment), tile scans.
+// Segment i writes out_i[0..n_i) and its total at out_i[n_i].
+struct ScanSegDev {
+    const uint32_t *in;
+    uint64_t *out;
+    uint64_t n, tile0, tile1;
+};
+struct ScanTile {
+    uint32_t seg, pad;
+    uint64_t start;
+};
+
+__global__ __launch_bounds__(kBlock) void k_segscan_reduce(const ScanSegDev *segs, const ScanTile *tiles,
+                                                           uint64_t *partial) {
+    const ScanTile t = tiles[blockIdx.x];
+    const ScanSegDev sg = segs[t.seg];
+    const uint64_t base = t.start + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++)
+        if (base + i < sg.n && base + i < t.start + kScanTile) v += sg.in[base + i];
+    uint64_t ex;
+    const uint64_t tot = block_excl_scan(v, ex);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_segscan_carry(const ScanSegDev *segs, uint64_t *partial) {
+    const ScanSegDev sg = segs[blockIdx.x];
+    uint64_t run = 0;
+    for (uint64_t t0 = sg.tile0; t0 < sg.tile1; t0 += kBlock) {
+        const uint64_t t = t0 + threadIdx.x;
+        const uint64_t v = t < sg.tile1 ? partial[t] : 0;
+        uint64_t ex;
+        const uint64_t tot = block_excl_scan(v, ex);
+        if (t < sg.tile1) partial[t] = run + ex;
+        run += tot;
+    }
+    if (threadIdx.x == 0) sg.out[sg.n] = run;
+}
+
+__global__ __launch_bounds__(kBlock) void k_segscan_apply(const ScanSegDev *segs, const ScanTile *tiles,
+                                                          const uint64_t *carry) {
+    const ScanTile t = tiles[blockIdx.x];
+    const ScanSegDev sg = segs[t.seg];
+    const uint64_t base = t.start + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = (base + i < sg.n) ? (uint64_t)sg.in[base + i] : 0;
+        sum += v[i];
+    }
+    uint64_t ex;
+    block_excl_scan(sum, ex);
+    uint64_t run = ex + carry[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < sg.n) sg.out[base + i] = run;
+        run += v[i];
+    }
+}
+
 // Exclusive scan in -> out (may alias), 64-bit carries; *total (device) =
 // sum.  tmp must hold scan_tmp_words(n) uint64.
 static uint64_t scan_tmp_words(uint64_t n) {
@@ -812,7 +875,8 @@ __device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t
     uint32_t w = 0;
     for (int r = 0; r < m && w < nrows; r++) {
         for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
-            if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
+            const float qx = D.sx[q], qy = D.sy[q];
+            if (sqdist(xa, ya, qx, qy) <= D.g.r2) {
                 out[2 * w] = g;
                 out[2 * w + 1] = SORTED ? q : D.sidx[q];
                 if (nb) nb[w] = q;
@@ -1381,7 +1445,8 @@ __global__ __launch_bounds__(kBlock) void k_write2_batch(DevData D, const CandDe
         const uint32_t *sl = cd.wslots ? cd.wslots : slots + cd.slots_off;
         for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
             uint32_t nr = sl[t];
-            if (nr) join2_write_anchor<true>(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
+            if (nr)
+                join2_write_anchor<true>(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
         }
     }
 }
@@ -2192,6 +2257,39 @@ int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uin
     if (!n_segs) return COLO_OK;
     hipLaunchKernelGGL(k_pi_abs, dim3((unsigned)n_segs), dim3(kBlock), 0, s, d_bits, d_segs, d_out);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "pi launch");
+}
+int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (!nseg) return COLO_OK;
+    std::vector<ScanSegDev> sd(nseg);
+    std::vector<ScanTile> tiles;
+    for (uint64_t i = 0; i < nseg; i++) {
+        sd[i] = ScanSegDev{segs[i].in, segs[i].out, segs[i].n, tiles.size(), 0};
+        for (uint64_t b = 0; b < segs[i].n; b += kScanTile) tiles.push_back(ScanTile{(uint32_t)i, 0, b});
+        sd[i].tile1 = tiles.size();
+    }
+    const uint64_t nt = tiles.size();
+    size_t bytes = nseg * sizeof(ScanSegDev) + nt * sizeof(ScanTile) + 8 * nt + 64;
+    void *tmp;
+    int rc = scratch(c, bytes, &tmp);
+    if (rc) return rc;
+    char *q = (char *)tmp;
+    ScanSegDev *d_sd = (ScanSegDev *)q;
+    q += nseg * sizeof(ScanSegDev);
+    ScanTile *d_tiles = (ScanTile *)q;
+    q += nt * sizeof(ScanTile);
+    uint64_t *d_part = (uint64_t *)(((uintptr_t)q + 7) & ~(uintptr_t)7);
+    COLO_HIP(hipMemcpyAsync(d_sd, sd.data(), nseg * sizeof(ScanSegDev), hipMemcpyHostToDevice, s));
+    if (nt) {
+        COLO_HIP(hipMemcpyAsync(d_tiles, tiles.data(), nt * sizeof(ScanTile), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_segscan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, d_part);
+    }
+    hipLaunchKernelGGL(k_segscan_carry, dim3((unsigned)nseg), dim3(kBlock), 0, s, d_sd, d_part);
+    if (nt) hipLaunchKernelGGL(k_segscan_apply, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, d_part);
+    COLO_HIP(hipGetLastError());
+    // the descriptors live in scratch: finish before the next scratch use
+    COLO_HIP(hipStreamSynchronize(s));
+    return COLO_OK;
 }
 int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s) {
     if (!n) return fill_u32((uint32_t *)d_offsets, 2, 0, s);

@@ -53,13 +53,12 @@ struct CandDesc {
     uint32_t *table;               // write pass: output table
     // level k: neighbour lists of the anchors in the last feature (sorted
     // positions, scan order), i.e. the second column of the level-2 instance
-    // table of (f[0], f[k-1]), with its row offsets per anchor input rank
-    // (nb_off[t] - nb_base .. nb_off[t+1] - nb_base); nb_table == nullptr
-    // selects the 3x3 cell scan.
+    // table of (f[0], f[k-1]), with its row offsets per anchor sorted rank
+    // (nb_off[t] - nb_base .. nb_off[t+1] - nb_base)
     const uint32_t *nb_table;
     const uint64_t *nb_off;
     uint64_t nb_base;
-    uint32_t *nbr_out;             // level-2 write pass: neighbour-list output (optional)
+    uint32_t *nbr_out;             // level-2 write pass: neighbour-list output
     const uint32_t *wslots;        // write pass: this table's slot counts (n_anchor)
 };
 
@@ -248,6 +247,14 @@ int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, cons
                hipStream_t s);
 int gather_u64(colo_ctx *c, const uint64_t *d_a, const uint64_t *d_idx, uint64_t m, uint64_t *d_out, hipStream_t s);
 int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s);
+// Exclusive scans of many slot arrays in one pass: out has n + 1 entries,
+// the total last.  Waits for the stream before returning.
+struct ScanSeg {
+    const uint32_t *in;
+    uint64_t *out;
+    uint64_t n;
+};
+int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s);
 int seg_sums(colo_ctx *c, const uint32_t *d_slots, const uint64_t *seg, uint64_t nseg, uint64_t *d_out,
              hipStream_t s);
 // Segment descriptors of one candidate's bitmap (bits_off = base word).

@@ -14,6 +14,7 @@
 // PERSISTENT runs each phase of a level as one launch over a device work
 // queue of (candidate, anchor range) items.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <chrono>
@@ -24,6 +25,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <unordered_set>
 #include <vector>
 
@@ -268,6 +270,10 @@ struct Miner {
                  bytes / 1e9, cur_k_, fr / 1e9, tot / 1e9, c->pool.cached / 1e9);
         return set_error(c, COLO_ENOMEM, b);
     }
+
+    // words per row of a batched level-k table in its level arena: k
+    // positions, plus the neighbour list at level 2
+    static uint64_t table_words(uint32_t k) { return k + (k == 2 ? 1 : 0); }
 
     int sync() {
         COLO_HIP(hipStreamSynchronize(s));
@@ -570,7 +576,7 @@ struct Miner {
             // pre-size the level's table arena from the predicted rows while
             // the count pass runs (the exact sizes are known only after it)
             double words = 0;
-            for (uint64_t u = 0; u < m; u++) words += std::expm1((double)std::max(0.f, (*pred)[u])) * (k + (k == 2 ? 1 : 0));
+            for (uint64_t u = 0; u < m; u++) words += std::expm1((double)std::max(0.f, (*pred)[u])) * table_words(k);
             words *= 1.10;
             if (words < 4e10) {
                 arena_reserved_ = std::make_shared<DevBuf>(&c->pool);
@@ -653,7 +659,7 @@ struct Miner {
         // level 2, the neighbour lists (1 word per row)
         uint64_t words = 0;
         for (size_t i = 0; i < L.size(); i++)
-            if (build[i]) words += L[i].rows * (k + (k == 2 ? 1 : 0));
+            if (build[i]) words += L[i].rows * table_words(k);
         // row offsets of the written tables: one exclusive scan per table
         // into one offsets arena (n_anchor + 1 entries each, the total last)
         uint64_t owords = 0;
@@ -663,22 +669,17 @@ struct Miner {
         if (!arena_offs->ensure(std::max<uint64_t>(owords, 1) * 8)) return oom("offsets arena", std::max<uint64_t>(owords, 1) * 8);
         std::vector<uint64_t> offs_at(L.size(), 0);
         {
-            // size the scan temporaries once for the largest table
-            uint64_t mx = 0;
-            for (size_t i = 0; i < L.size(); i++)
-                if (build[i]) mx = std::max(mx, L[i].nslots);
-            void *tmp;
-            int rcs = scratch(c, 8 * (mx / 1024 + 64) + 4096, &tmp);
-            if (rcs) return rcs;
+            std::vector<ScanSeg> segs;
             uint64_t oat = 0;
             for (size_t i = 0; i < L.size(); i++) {
                 if (!build[i]) continue;
                 offs_at[i] = oat;
-                int rc0 = scan_batch(c, L[i].slots->as<uint32_t>() + L[i].slots_off, L[i].nslots,
-                                     arena_offs->as<uint64_t>() + oat, s);
-                if (rc0) return rc0;
+                segs.push_back(ScanSeg{L[i].slots->as<uint32_t>() + L[i].slots_off, arena_offs->as<uint64_t>() + oat,
+                                       L[i].nslots});
                 oat += L[i].nslots + 1;
             }
+            int rc0 = scan_segments(c, segs.data(), segs.size(), s);
+            if (rc0) return rc0;
         }
         std::shared_ptr<DevBuf> arena;
         if (arena_reserved_ && arena_reserved_->bytes >= words * 4) {
@@ -1193,63 +1194,225 @@ uint64_t colo_candidates(const uint32_t *L, uint64_t P, uint32_t km1, uint32_t *
     return n;
 }
 
-int colo_load_csv(const char *path, colo_host_data **out) {
-    if (!path || !out) return COLO_EARG;
-    *out = nullptr;
-    FILE *fp = fopen(path, "rb");
-    if (!fp) return COLO_EFILE;
-    fseek(fp, 0, SEEK_END);
-    long sz = ftell(fp);
-    fseek(fp, 0, SEEK_SET);
-    std::string buf((size_t)(sz > 0 ? sz : 0), '\0');
-    size_t got = sz > 0 ? fread(&buf[0], 1, (size_t)sz, fp) : 0;
-    fclose(fp);
-    buf.resize(got);
-    buf.push_back('\0');
-    auto h = std::make_unique<colo_host_data>();
-    char *p = &buf[0];
-    char *end = p + got;
-    auto next_line = [&](char *&ln) -> bool {  // getline semantics
-        if (p >= end) { ln = nullptr; return false; }
-        ln = p;
-        char *nl = (char *)memchr(p, '\n', (size_t)(end - p));
-        if (nl) { *nl = 0; p = nl + 1; } else { p = end; }
-        return true;
-    };
-    char *ln;
-    long nl;
-    if (!next_line(ln) || !parse_long(ln, &nl) || nl < 0 || nl > 0xFFFFFFFFL) return COLO_EPARSE;
-    h->n = (uint32_t)nl;
-    h->x.resize(h->n);
-    h->y.resize(h->n);
-    uint64_t row = 0;
-    long last = 0;
-    bool have = false;
-    for (;;) {  // rows until the first empty line (cpp:164)
-        if (!next_line(ln) || ln[0] == 0) break;
-        char *fld[3];
-        long lab;
-        if (split(ln, fld, 3) < 3 || !parse_long(fld[0], &lab)) return COLO_EPARSE;
-        if (row >= h->n) return COLO_EPARSE;
-        if (!have || lab != last) {  // cpp:168-183
-            h->fcount.push_back(0);
-            h->labels.emplace_back(fld[0]);
-            last = lab;
-            have = true;
+// Binary cache of a parsed dataset: magic, source size and mtime, n, F,
+// bounds, per-feature counts and label lengths, label bytes, x, y.
+static const char kCacheMagic[8] = {'C', 'O', 'L', 'O', 'B', 'I', 'N', '1'};
+
+static bool source_stamp(const char *path, uint64_t *size, int64_t *mtime_ns) {
+    struct stat st;
+    if (stat(path, &st) != 0) return false;
+    *size = (uint64_t)st.st_size;
+    *mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000ll + st.st_mtim.tv_nsec;
+    return true;
+}
+
+static bool cache_read(const char *cache, uint64_t size, int64_t mtime, colo_host_data *h) {
+    FILE *fp = fopen(cache, "rb");
+    if (!fp) return false;
+    bool ok = false;
+    do {
+        char mg[8];
+        uint64_t sz;
+        int64_t mt;
+        uint32_t n, F;
+        if (fread(mg, 1, 8, fp) != 8 || memcmp(mg, kCacheMagic, 8)) break;
+        if (fread(&sz, 8, 1, fp) != 1 || fread(&mt, 8, 1, fp) != 1 || sz != size || mt != mtime) break;
+        if (fread(&n, 4, 1, fp) != 1 || fread(&F, 4, 1, fp) != 1 || fread(h->b, 4, 4, fp) != 4) break;
+        h->n = n;
+        h->fcount.resize(F);
+        std::vector<uint32_t> ll(F);
+        if (F && (fread(h->fcount.data(), 4, F, fp) != F || fread(ll.data(), 4, F, fp) != F)) break;
+        h->labels.clear();
+        bool bad = false;
+        for (uint32_t f = 0; f < F && !bad; f++) {
+            std::string l(ll[f], '\0');
+            if (ll[f] && fread(&l[0], 1, ll[f], fp) != ll[f]) bad = true;
+            h->labels.push_back(std::move(l));
         }
-        if (!parse_flt(fld[1], &h->x[row]) || !parse_flt(fld[2], &h->y[row])) return COLO_EPARSE;
-        h->fcount.back()++;
-        row++;
+        if (bad) break;
+        h->x.resize(n);
+        h->y.resize(n);
+        if (n && (fread(h->x.data(), 4, n, fp) != n || fread(h->y.data(), 4, n, fp) != n)) break;
+        ok = true;
+    } while (false);
+    fclose(fp);
+    return ok;
+}
+
+static void cache_write(const char *cache, uint64_t size, int64_t mtime, const colo_host_data *h) {
+    std::string tmp = std::string(cache) + ".tmp";
+    FILE *fp = fopen(tmp.c_str(), "wb");
+    if (!fp) return;
+    uint32_t F = (uint32_t)h->fcount.size();
+    bool ok = fwrite(kCacheMagic, 1, 8, fp) == 8 && fwrite(&size, 8, 1, fp) == 1 && fwrite(&mtime, 8, 1, fp) == 1 &&
+              fwrite(&h->n, 4, 1, fp) == 1 && fwrite(&F, 4, 1, fp) == 1 && fwrite(h->b, 4, 4, fp) == 4;
+    std::vector<uint32_t> ll;
+    for (const std::string &l : h->labels) ll.push_back((uint32_t)l.size());
+    if (ok && F) ok = fwrite(h->fcount.data(), 4, F, fp) == F && fwrite(ll.data(), 4, F, fp) == F;
+    for (const std::string &l : h->labels)
+        if (ok && !l.empty()) ok = fwrite(l.data(), 1, l.size(), fp) == l.size();
+    if (ok && h->n) ok = fwrite(h->x.data(), 4, h->n, fp) == h->n && fwrite(h->y.data(), 4, h->n, fp) == h->n;
+    ok = (fclose(fp) == 0) && ok;
+    if (ok) ok = rename(tmp.c_str(), cache) == 0;
+    if (!ok) remove(tmp.c_str());
+}
+
+// populateData (cpp:139-254) on T threads.  Line 1 is N; the rows run to the
+// first empty line; a feature starts wherever the label differs from the
+// previous row's; the line after the empty one holds the bounds.  Threads
+// take byte ranges starting at line starts: pass 1 counts each range's rows
+// up to its first empty line, pass 2 parses the rows at their global index,
+// and the feature starts are merged in row order.  Any malformed row, a row
+// count other than N or a missing bounds line is COLO_EPARSE, as in a
+// sequential parse.
+static int parse_csv(std::string &buf, uint32_t threads, colo_host_data *h) {
+    char *const base = &buf[0];
+    char *const end = base + buf.size() - 1;  // buf ends with an added '\0'
+    char *p = base;
+    char *nl = (char *)memchr(p, '\n', (size_t)(end - p));
+    if (!nl) return COLO_EPARSE;
+    *nl = 0;
+    long nval;
+    if (!parse_long(p, &nval) || nval < 0 || nval > 0xFFFFFFFFL) return COLO_EPARSE;
+    h->n = (uint32_t)nval;
+    char *body = nl + 1;
+    // default: one range per MiB up to the hardware threads; an explicit
+    // thread count only keeps ranges of at least 16 bytes
+    const uint64_t bytes = (uint64_t)(end - body);
+    uint64_t T = threads ? std::min<uint64_t>(threads, bytes / 16 + 1)
+                         : std::min<uint64_t>(std::max(1u, std::thread::hardware_concurrency()), bytes / (1 << 20) + 1);
+    T = std::max<uint64_t>(1, std::min<uint64_t>(T, 64));
+    std::vector<char *> cs(T + 1);
+    cs[0] = body;
+    cs[T] = end;
+    for (uint64_t t = 1; t < T; t++) {
+        char *q = body + (uint64_t)(end - body) * t / T;
+        if (q < cs[t - 1]) q = cs[t - 1];
+        char *e = q > body ? (char *)memchr(q - 1, '\n', (size_t)(end - (q - 1))) : q - 1;
+        cs[t] = e ? e + 1 : end;  // start of the first line beginning at or after q
+        if (q == body) cs[t] = body;
     }
-    if (row != h->n) return COLO_EPARSE;
+    // pass 1: rows of each range up to its first empty line
+    std::vector<uint64_t> rows(T, 0);
+    std::vector<char *> empty_at(T, nullptr);
+    auto pass1 = [&](uint64_t t) {
+        char *q = cs[t];
+        while (q < cs[t + 1]) {
+            char *e = (char *)memchr(q, '\n', (size_t)(end - q));
+            char *le = e ? e : end;
+            if (le == q) {
+                empty_at[t] = q;
+                return;
+            }
+            rows[t]++;
+            q = e ? e + 1 : end;
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (uint64_t t = 1; t < T; t++) th.emplace_back(pass1, t);
+        pass1(0);
+        for (auto &x : th) x.join();
+    }
+    uint64_t tstop = T, nrows = 0;
+    for (uint64_t t = 0; t < T; t++) {
+        nrows += rows[t];
+        if (empty_at[t]) {
+            tstop = t;
+            break;
+        }
+    }
+    if (tstop == T || nrows != h->n) return COLO_EPARSE;  // no empty line, or not N rows
+    h->x.assign(h->n, 0.f);
+    h->y.assign(h->n, 0.f);
+    std::vector<long> lab(h->n);
+    // pass 2: parse; keep the label text where a range starts or the label changes
+    std::vector<std::vector<std::pair<uint64_t, std::string>>> starts(tstop + 1);
+    std::vector<int> bad(tstop + 1, 0);
+    std::vector<uint64_t> row0(tstop + 1, 0);
+    for (uint64_t t = 1; t <= tstop; t++) row0[t] = row0[t - 1] + rows[t - 1];
+    auto pass2 = [&](uint64_t t) {
+        char *q = cs[t];
+        uint64_t r = row0[t];
+        for (uint64_t i = 0; i < rows[t]; i++) {
+            char *e = (char *)memchr(q, '\n', (size_t)(end - q));
+            if (e) *e = 0;
+            char *fld[3];
+            long lb;
+            if (split(q, fld, 3) < 3 || !parse_long(fld[0], &lb) || !parse_flt(fld[1], &h->x[r]) ||
+                !parse_flt(fld[2], &h->y[r])) {
+                bad[t] = 1;
+                return;
+            }
+            lab[r] = lb;
+            if (i == 0 || lb != lab[r - 1]) starts[t].emplace_back(r, std::string(fld[0]));
+            r++;
+            q = e ? e + 1 : end;
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (uint64_t t = 1; t <= tstop; t++) th.emplace_back(pass2, t);
+        pass2(0);
+        for (auto &x : th) x.join();
+    }
+    for (int b : bad)
+        if (b) return COLO_EPARSE;
+    // features in row order (cpp:168-183)
+    h->fcount.clear();
+    h->labels.clear();
+    uint64_t prev_start = 0;
+    for (uint64_t t = 0; t <= tstop; t++)
+        for (auto &st : starts[t]) {
+            const uint64_t r = st.first;
+            if (r != 0 && lab[r] == lab[r - 1]) continue;  // a range boundary, not a feature start
+            if (!h->fcount.empty()) h->fcount.back() = (uint32_t)(r - prev_start);
+            h->fcount.push_back(0);
+            h->labels.push_back(std::move(st.second));
+            prev_start = r;
+        }
+    if (!h->fcount.empty()) h->fcount.back() = (uint32_t)(h->n - prev_start);
+    // bounds: the line after the first empty one
+    char *q = empty_at[tstop] + 1;
+    if (q > end) return COLO_EPARSE;
+    char *e = (char *)memchr(q, '\n', (size_t)(end - q));
+    if (e) *e = 0;
     char *fld[4];
-    if (!next_line(ln) || split(ln, fld, 4) < 4) return COLO_EPARSE;
+    if (split(q, fld, 4) < 4) return COLO_EPARSE;
     for (int i = 0; i < 4; i++)
         if (!parse_flt(fld[i], &h->b[i])) return COLO_EPARSE;
+    return COLO_OK;
+}
+
+int colo_load_csv_opts(const char *path, const char *cache_path, uint32_t threads, colo_host_data **out) {
+    if (!path || !out) return COLO_EARG;
+    *out = nullptr;
+    auto h = std::make_unique<colo_host_data>();
+    uint64_t size = 0;
+    int64_t mtime = 0;
+    const bool stamped = source_stamp(path, &size, &mtime);
+    if (!(cache_path && stamped && cache_read(cache_path, size, mtime, h.get()))) {
+        FILE *fp = fopen(path, "rb");
+        if (!fp) return COLO_EFILE;
+        fseek(fp, 0, SEEK_END);
+        long sz = ftell(fp);
+        fseek(fp, 0, SEEK_SET);
+        std::string buf((size_t)(sz > 0 ? sz : 0), '\0');
+        size_t got = sz > 0 ? fread(&buf[0], 1, (size_t)sz, fp) : 0;
+        fclose(fp);
+        buf.resize(got);
+        buf.push_back('\0');
+        h = std::make_unique<colo_host_data>();
+        int rc = parse_csv(buf, threads, h.get());
+        if (rc) return rc;
+        if (cache_path && stamped) cache_write(cache_path, size, mtime, h.get());
+    }
     for (auto &l : h->labels) h->label_ptrs.push_back(l.c_str());
     *out = h.release();
     return COLO_OK;
 }
+
+int colo_load_csv(const char *path, colo_host_data **out) { return colo_load_csv_opts(path, nullptr, 0, out); }
 
 int colo_host_dataset(const colo_host_data *h, colo_dataset *ds) {
     if (!h || !ds) return COLO_EARG;

@@ -328,6 +328,13 @@ uint64_t colo_candidates(const uint32_t *L, uint64_t P, uint32_t km1, uint32_t *
  * returned handle. */
 typedef struct colo_host_data colo_host_data;
 int colo_load_csv(const char *path, colo_host_data **out);
+/* The same parse on `threads` host threads (0 = one per MiB of input, up
+ * to the hardware's), with an optional binary cache: when cache_path names
+ * a cache written for this file (same size and modification time) it is
+ * read instead of the CSV; otherwise the CSV is parsed and the cache
+ * written.  Results equal colo_load_csv's.  No counterpart in the
+ * reference, which parses line by line (cpp:139-254). */
+int colo_load_csv_opts(const char *path, const char *cache_path, uint32_t threads, colo_host_data **out);
 int colo_host_dataset(const colo_host_data *h, colo_dataset *ds);
 const char *colo_host_label(const colo_host_data *h, uint32_t f);
 void colo_host_free(colo_host_data *h);

@@ -102,3 +102,68 @@ def test_compat_header_compiles(tmp_path):
     r = subprocess.run([hipcc, "-std=c++17", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"), str(src)],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0, r.stdout
+
+
+def _hd_equal(a, b):
+    assert (a.n, a.F) == (b.n, b.F)
+    assert a.x.view(np.uint32).tolist() == b.x.view(np.uint32).tolist()
+    assert a.y.view(np.uint32).tolist() == b.y.view(np.uint32).tolist()
+    assert a.fcount.tolist() == b.fcount.tolist()
+    assert a.labels == b.labels
+    assert a.bounds.view(np.uint32).tolist() == b.bounds.view(np.uint32).tolist()
+
+
+def test_csv_parallel_equals_sequential(colo_mod, datagen, tmp_path):
+    """The multi-threaded parse equals the one-thread parse on a generated
+    dataset (ranges split mid-feature) and on small files whose thread
+    ranges cut labels, quoted fields and repeated labels at every offset."""
+    w = datagen.write_workload(str(tmp_path / "w"), "c2", N=120_000)
+    one = colo_mod.HostData(w["csv"], threads=1)
+    for t in (2, 7, 16):
+        _hd_equal(colo_mod.HostData(w["csv"], threads=t), one)
+    _hd_equal(colo_mod.HostData(w["csv"]), one)
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        n = int(rng.integers(1, 40))
+        labs = np.sort(rng.integers(0, 6, n))
+        if trial % 3 == 0:  # a label that reappears later becomes a new feature
+            labs[n // 2:] = labs[: n - n // 2]
+        rows = []
+        for l in labs:
+            x, y = (float(v) for v in rng.uniform(0, 100, 2))
+            rows.append(f'"{l}",{x!r},"{y!r}"' if rng.random() < 0.3 else f"{l},{x!r},{y!r}")
+        p = tmp_path / f"s{trial}.csv"
+        p.write_text(f"{n}\n" + "\n".join(rows) + "\n\n0,0,100,100")
+        ref = colo_mod.HostData(str(p), threads=1)
+        assert ref.fcount.sum() == n
+        for t in (2, 3, 5, 8, 13):
+            _hd_equal(colo_mod.HostData(str(p), threads=t), ref)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+@pytest.mark.parametrize("body", ["2\n0,1,2\n0,3,4\n0,0,5,5", "3\n0,1,2\n0,3,4\n\n0,0,5,5",
+                                  "2\n0,1,2\nx,3,4\n\n0,0,5,5", "2\n0,1,2\n0,3,4\n\n0,0,5", "1\n0,1\n\n0,0,5,5"])
+def test_csv_parallel_rejects(colo_mod, tmp_path, threads, body):
+    p = tmp_path / "bad.csv"
+    p.write_text(body)
+    with pytest.raises(colo_mod.ColoError):
+        colo_mod.HostData(str(p), threads=threads)
+
+
+def test_csv_binary_cache(colo_mod, datagen, tmp_path):
+    """The cache is written on the first load, read on the next, and ignored
+    once the CSV changes."""
+    w = datagen.write_workload(str(tmp_path / "w"), "c1", N=5000)
+    cache = str(tmp_path / "data.colobin")
+    a = colo_mod.HostData(w["csv"], cache=cache)
+    assert os.path.getsize(cache) > 0
+    _hd_equal(colo_mod.HostData(w["csv"], cache=cache), a)
+    _hd_equal(colo_mod.HostData(w["csv"]), a)
+    text = open(w["csv"]).read().splitlines()
+    text[1] = text[1].split(",")[0] + ",1.5,2.5"
+    with open(w["csv"], "w") as fh:
+        fh.write("\n".join(text) + "\n")
+    os.utime(w["csv"], ns=(1, 2))
+    b = colo_mod.HostData(w["csv"], cache=cache)
+    assert b.x[0] == np.float32(1.5) and b.y[0] == np.float32(2.5)
+    _hd_equal(colo_mod.HostData(w["csv"], cache=cache), b)

@@ -801,8 +801,8 @@ int colo::pairwork_batch(colo_ctx *c, const uint32_t *pairs, uint32_t P, uint64_
 // indexed by its input rank (the reference's layout) or, with SORTED, by
 // its sorted rank (the batched miner's layout).
 template <bool SORTED = false>
-__device__ inline void join2_count_anchor(const DevData &D, uint32_t a, uint32_t b, uint32_t p, uint32_t *slots,
-                                          uint32_t *bits_a, uint32_t *bits_b) {
+__device__ inline uint32_t join2_count_anchor(const DevData &D, uint32_t a, uint32_t b, uint32_t p, uint32_t *slots,
+                                              uint32_t *bits_a, uint32_t *bits_b) {
     const uint32_t sa = D.fstart[a], sb = D.fstart[b];
     const uint32_t t = SORTED ? p - sa : D.sidx[p] - sa;
     const float xa = D.sx[p], ya = D.sy[p];
@@ -822,6 +822,7 @@ __device__ inline void join2_count_anchor(const DevData &D, uint32_t a, uint32_t
     acc.flush();
     slots[t] = cnt;
     if (cnt) set_bit(bits_a, t);
+    return cnt;
 }
 
 // Level-k count for one parent row (generalInstanceTableSlotCounter_Kernel,
@@ -1084,7 +1085,7 @@ __device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &i
 // are indexed by sorted rank.
 template <int KM1>
 __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
-                                             uint32_t *slots, uint32_t *bits) {
+                                             uint32_t *slots, uint32_t *bits, unsigned long long *s_rows) {
     const CandDesc &c0 = descs[0];
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
@@ -1124,6 +1125,7 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         }
         acc.flush();
         slots[cd.slots_off + i] = cnt;
+        if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
         if (cnt)
 #pragma unroll
             for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
@@ -1169,7 +1171,7 @@ constexpr uint32_t kRegChunks = 4;  // list chunks of 64 kept in registers
 
 template <bool WRITE, int KM1>
 __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t r0, uint32_t n,
-                                uint32_t *slots, uint32_t *bits) {
+                                uint32_t *slots, uint32_t *bits, unsigned long long *s_rows) {
     const uint32_t lane = __lane_id();
     const uint64_t lt = (1ull << lane) - 1;
     const CandDesc &c0 = descs[0];
@@ -1227,6 +1229,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                         const uint32_t cnt = (uint32_t)__popcll(bal & segm);
                         if (valid && e == 0) {
                             slots[cd.slots_off + row] = cnt;
+                            if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
                             if (cnt)
                                 for (uint32_t mm = 0; mm < km1; mm++)
                                     set_bit(bt + cd.seg_off[mm], rp[mm] - D.fstart[cd.f[mm]]);
@@ -1324,7 +1327,10 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                     if (!WRITE) wave_set_bits(bt + cd.seg_off[km1], q - sl, e < m, h);
                 }
                 if (!WRITE) {
-                    if (lane == 0) slots[cd.slots_off + row] = cnt;
+                    if (lane == 0) {
+                        slots[cd.slots_off + row] = cnt;
+                        if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
+                    }
                     if (cnt && lane < km1) set_bit(bt + cd.seg_off[lane], rp[lane] - D.fstart[cd.f[lane]]);
                 }
             }
@@ -1337,10 +1343,11 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
     }
 }
 
-// Level 2: one anchor per thread over the item's anchor range.
+// Level 2: one anchor per thread over the item's anchor range; the item's
+// rows are summed in the workgroup and added to the candidate's total.
 __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDesc *desc, const WorkItem *items,
                                                          const uint64_t *seg, uint32_t *slots, uint32_t *bits,
-                                                         uint32_t *queue) {
+                                                         unsigned long long *rows, uint32_t *queue) {
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
@@ -1348,38 +1355,47 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
         uint32_t *sl = slots + cd.slots_off;
         uint32_t *bt = bits + cd.bits_off;
         const uint32_t a = cd.f[0], b = cd.f[1];
+        uint64_t sum = 0;
         for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-            join2_count_anchor<true>(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+            sum += join2_count_anchor<true>(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0],
+                                            bt + cd.seg_off[1]);
+        uint64_t ex;
+        const uint64_t tot = block_excl_scan(sum, ex);
+        if (threadIdx.x == 0 && tot) atomicAdd(&rows[wi.cand], (unsigned long long)tot);
     }
 }
 
-// Level k >= 3 (KM1 = k - 1, 0 for any k): the item's parent rows in chunks
-// of 64, taken by the block's waves in turn.
 // Level k >= 3 (KM1 = k - 1; 0 for any k).  VAR 0: one parent row per
-// thread, all of the item's candidates per row; VAR 2: one parent row per
+// thread, all of a run's candidates per row; VAR 2: one parent row per
 // thread, the item's candidates one after another (one neighbour-list
 // stream at a time); VAR 1: wave-cooperative (wave_rows_multi) over 64-row
-// chunks taken by the block's waves in turn.
+// chunks taken by the block's waves in turn.  Rows per candidate are summed
+// in LDS over the item and added to the candidates' totals.
 template <int KM1, int VAR>
 __global__ __launch_bounds__(kBlock) void k_countk_batch(DevData D, const CandDesc *desc, const WorkItem *items,
                                                          const uint64_t *seg, uint32_t *slots, uint32_t *bits,
-                                                         uint32_t *queue) {
+                                                         unsigned long long *rows, uint32_t *queue) {
+    __shared__ unsigned long long s_rows[kMaxRun];
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
+        if (threadIdx.x < kMaxRun) s_rows[threadIdx.x] = 0;
+        __syncthreads();
         if (VAR == 0) {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits);
+                joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows);
         } else if (VAR == 2) {
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                    joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits);
+                    joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits, s_rows + j);
         } else {
             for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
                 const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
-                wave_rows_multi<false, KM1>(D, desc + wi.cand, wi.ncand, c0, n, slots, bits);
+                wave_rows_multi<false, KM1>(D, desc + wi.cand, wi.ncand, c0, n, slots, bits, s_rows);
             }
         }
+        __syncthreads();
+        if (threadIdx.x < wi.ncand && s_rows[threadIdx.x]) atomicAdd(&rows[wi.cand + threadIdx.x], s_rows[threadIdx.x]);
     }
 }
 
@@ -1468,7 +1484,7 @@ __global__ __launch_bounds__(kBlock) void k_writek_batch(DevData D, const CandDe
             for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
                 const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
                 wave_rows_multi<true, KM1>(D, desc + wi.cand, wi.ncand, c0, n, const_cast<uint32_t *>(slots),
-                                           nullptr);
+                                           nullptr, nullptr);
             }
         }
     }
@@ -1501,18 +1517,19 @@ template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batc
 template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
 
 int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
-                      uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s) {
+                      uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint64_t *d_rows, uint32_t *d_queue,
+                      hipStream_t s) {
     if (!n_items) return COLO_OK;
     int rc = fill_u32(d_queue, kXcds, 0, s);
     if (rc) return rc;
     if (k == 2)
         hipLaunchKernelGGL(k_count2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
-                           d_items, d_seg, d_slots, d_bits, d_queue);
+                           d_items, d_seg, d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
     else
         hipLaunchKernelGGL(c->tune.count_var == 2 ? pick_k<PickCount2>(k)
                            : c->tune.count_var == 1 ? pick_k<PickCount1>(k) : pick_k<PickCount0>(k),
                            dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
-                           d_slots, d_bits, d_queue);
+                           d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "count_batch launch");
 }
 
@@ -1532,56 +1549,6 @@ int colo::write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "write_batch launch");
 }
 
-// Per-candidate row totals: sums of the slot segments, one chunk of
-// kSumChunk slots per workgroup, one 64-bit atomic per workgroup.
-constexpr uint64_t kSumChunk = 1u << 16;
-
-__global__ __launch_bounds__(kBlock) void k_seg_sum(const uint32_t *slots, const uint64_t *seg_off,
-                                                    const uint64_t *chunk_prefix, uint64_t nseg,
-                                                    unsigned long long *out) {
-    uint64_t b = blockIdx.x;
-    uint64_t lo = 0, hi = nseg;
-    while (hi - lo > 1) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (chunk_prefix[mid] <= b) lo = mid; else hi = mid;
-    }
-    const uint64_t s0 = seg_off[2 * lo], len = seg_off[2 * lo + 1];
-    const uint64_t c0 = (b - chunk_prefix[lo]) * kSumChunk;
-    const uint64_t c1 = c0 + kSumChunk < len ? c0 + kSumChunk : len;
-    uint64_t acc = 0;
-    for (uint64_t i = c0 + threadIdx.x; i < c1; i += kBlock) acc += slots[s0 + i];
-    uint64_t ex;
-    uint64_t tot = block_excl_scan(acc, ex);
-    if (threadIdx.x == 0 && tot) atomicAdd(&out[lo], (unsigned long long)tot);
-}
-
-// Sums segments (offset, length) of the slots arena into out (device, nseg
-// u64).  seg: 2*nseg host values.
-int colo::seg_sums(colo_ctx *c, const uint32_t *d_slots, const uint64_t *seg, uint64_t nseg, uint64_t *d_out,
-                   hipStream_t s) {
-    colo_ctx *ctx_for_err = c;
-    if (!nseg) return COLO_OK;
-    std::vector<uint64_t> prefix(nseg);
-    uint64_t chunks = 0;
-    for (uint64_t i = 0; i < nseg; i++) {
-        prefix[i] = chunks;
-        chunks += (seg[2 * i + 1] + kSumChunk - 1) / kSumChunk;
-    }
-    int rc = fill_u32((uint32_t *)d_out, 2 * nseg, 0, s);
-    if (rc) return rc;
-    if (!chunks) return COLO_OK;
-    void *tmp;
-    if ((rc = scratch(c, nseg * 24 + 64, &tmp))) return rc;
-    uint64_t *d_seg = (uint64_t *)tmp, *d_pre = d_seg + 2 * nseg;
-    COLO_HIP(hipMemcpyAsync(d_seg, seg, nseg * 16, hipMemcpyHostToDevice, s));
-    COLO_HIP(hipMemcpyAsync(d_pre, prefix.data(), nseg * 8, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_seg_sum, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_slots, d_seg, d_pre, nseg,
-                       (unsigned long long *)d_out);
-    COLO_HIP(hipGetLastError());
-    // the segment arrays live in scratch: finish before its next use
-    COLO_HIP(hipStreamSynchronize(s));
-    return COLO_OK;
-}
 
 // -------------------------------------------------------------------- PI
 // One workgroup per bitmap segment: popcount reduction plus the two

@@ -16,6 +16,7 @@ namespace colo {
 
 constexpr int kMaxK = 16;          // largest pattern size the kernels accept
 constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
+constexpr int kMaxRun = 64;        // candidates sharing a parent table in one work item
 
 // Grid geometry as the kernels read it (gridBox, Codes/DataTypes.h:77-83).
 struct DevGrid {
@@ -235,8 +236,11 @@ int filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, f
 // Batched passes over many candidates (one launch each).
 // d_seg: 9 item boundaries, one segment per XCD (see xcd_segments);
 // d_queue: 8 counters.
+// d_rows: one u64 per descriptor, zeroed by the caller; receives the rows
+// (sum of the slots) of every candidate.
 int count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
-                uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint32_t *d_queue, hipStream_t s);
+                uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint64_t *d_rows, uint32_t *d_queue,
+                hipStream_t s);
 int write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                 uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
 int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
@@ -255,8 +259,7 @@ struct ScanSeg {
     uint64_t n;
 };
 int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s);
-int seg_sums(colo_ctx *c, const uint32_t *d_slots, const uint64_t *seg, uint64_t nseg, uint64_t *d_out,
-             hipStream_t s);
+
 // Segment descriptors of one candidate's bitmap (bits_off = base word).
 void seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off,
                 uint64_t *seg_off, uint64_t *words_total);

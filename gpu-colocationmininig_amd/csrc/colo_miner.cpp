@@ -564,12 +564,15 @@ struct Miner {
             int rc0 = upload_items(items, pred ? &iw : nullptr);
             if (rc0) return rc0;
         }
+        if (!d_rowsel.ensure((m + 1) * 8)) return oom("row totals", (m + 1) * 8);
         auto t0 = Clock::now();
         int rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
+        if (!rc) rc = fill_u32(d_rowsel.as<uint32_t>(), 2 * m, 0, s);
         if (rc) return rc;
         size_t e0 = mark();
         rc = count_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
-                         slots_arena->as<uint32_t>(), arena_bits.as<uint32_t>(), d_queue.as<uint32_t>(), s);
+                         slots_arena->as<uint32_t>(), arena_bits.as<uint32_t>(), d_rowsel.as<uint64_t>(),
+                         d_queue.as<uint32_t>(), s);
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Count : kCount);
         if (pred) {
@@ -592,16 +595,8 @@ struct Miner {
             st.count_launches++;
         }
         st.launches += 2;
-        // rows per candidate: segmented sums of the slots arena (offsets are
-        // built later, only for the tables that are written)
-        std::vector<uint64_t> sseg(2 * m);
-        for (uint64_t u = 0; u < m; u++) {
-            sseg[2 * u] = desc[u].slots_off;
-            sseg[2 * u + 1] = desc[u].n_anchor;
-        }
-        if (!d_rowsel.ensure((m + 1) * 8)) return oom("row totals", (m + 1) * 8);
-        rc = seg_sums(c, slots_arena->as<uint32_t>(), sseg.data(), m, d_rowsel.as<uint64_t>(), s);
-        if (rc) return rc;
+        // rows per candidate, summed by the count pass (offsets are built
+        // later, only for the tables that are written)
         std::vector<uint64_t> rows_of(m, 0);
         COLO_HIP(hipMemcpyAsync(rows_of.data(), d_rowsel.p, m * 8, hipMemcpyDeviceToHost, s));
         // PI segments of every candidate, one launch

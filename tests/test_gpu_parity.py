@@ -347,3 +347,20 @@ def test_binning_graph_replays(datagen, colo_mod, oracle_mod):
         np.testing.assert_array_equal(a, e)
     assert eager.graph_info()[1] == 1  # only the capture at creation was launched
     assert eager.mine(0.2, max_k=3)[0] == expect
+
+
+def test_full_c2_all_schedules(datagen, colo_mod, oracle_mod):
+    """The whole c2 workload (500k instances, sizes up to 5) under every
+    schedule, with table checksums, against the oracle's report."""
+    _say("full c2, all schedules")
+    x, y, fc, b, p = _workload(datagen, "c2")
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    expect = orc.mine(p["r"], p["theta"], max_k=p["max_k"])
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    model_path = os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin")
+    model = colo_mod.CostModel.load(model_path)
+    for sched, m in ((colo_mod.COLO_SCHED_PLAIN, None), (colo_mod.COLO_SCHED_PERSISTENT, None),
+                     (colo_mod.COLO_SCHED_LEARNED, model)):
+        rep, st = ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=m)
+        assert rep == expect, sched

@@ -17,11 +17,13 @@ ARCH = os.environ.get("COLO_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off keeps a*a + b*b as two roundings (the reference's
 # binary32 distance); divisions stay IEEE correctly rounded; packed FP32
-# instructions are disabled.
+# instructions are disabled; wave-uniform atomics are combined with DPP
+# reductions rather than a scalar loop over the active lanes.
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP",
     "-Wall", "-Wno-unused-function",
 ]
 

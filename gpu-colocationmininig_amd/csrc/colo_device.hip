@@ -693,10 +693,15 @@ __device__ inline float sqdist(float x1, float y1, float x2, float y2) {
     return sx + sy;
 }
 
+// A pointer read from a descriptor in memory, as a global-memory pointer
+// (global rather than flat loads).
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+__device__ inline const gcu32 *as_global(const uint32_t *p) { return (const gcu32 *)p; }
+
 __device__ inline void set_bit(uint32_t *bits, uint64_t i) {
     uint32_t m = 1u << (i & 31);
     uint32_t *w = bits + (i >> 5);
-    if (!(__atomic_load_n(w, __ATOMIC_RELAXED) & m)) atomicOr(w, m);
+    if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m)) atomicOr(w, m);
 }
 
 // Collects the bits a lane sets in ascending index order (candidates come
@@ -709,7 +714,7 @@ struct BitAcc {
     __device__ inline void flush() {
         if (mask) {
             uint32_t *w = bits + word;
-            if ((__atomic_load_n(w, __ATOMIC_RELAXED) & mask) != mask) atomicOr(w, mask);
+            if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != mask) atomicOr(w, mask);
         }
         mask = 0;
     }
@@ -1067,7 +1072,7 @@ __device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &i
         for (uint32_t d = 0; d < kXcds && got == ~0ull; d++) {
             const uint32_t x = (home + d) % kXcds;
             const uint64_t b = seg[x], e = seg[x + 1];
-            if (b >= e || __atomic_load_n(&ctr[x], __ATOMIC_RELAXED) >= e - b) continue;
+            if (b >= e || __hip_atomic_load(&ctr[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= e - b) continue;
             const uint32_t i = atomicAdd(&ctr[x], 1u);
             if (b + i < e) got = b + i;
         }
@@ -1104,13 +1109,16 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
     const uint32_t t = ids[0] - D.fstart[c0.f[0]];
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
+        // descriptor fields in registers: the loop's bit stores could alias
+        // the descriptor, which would otherwise be re-read every entry
+        const gcu32 *nbt = as_global(cd.nb_table);
         const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
         const uint32_t sl = D.fstart[cd.f[km1]];
         uint32_t *bt = bits + cd.bits_off;
         uint32_t cnt = 0;
         BitAcc acc(bt + cd.seg_off[km1]);
         for (uint64_t e = b0; e < b1; e++) {
-            const uint32_t q = cd.nb_table[e];
+            const uint32_t q = nbt[e];
             const float qx = D.sx[q], qy = D.sy[q];
             bool ok = true;
 #pragma unroll
@@ -1152,7 +1160,7 @@ __device__ inline void wave_set_bits(uint32_t *base, uint32_t rank, bool act, bo
     const uint32_t pw = __shfl_up(w, 1, 64);
     if (act && msk && (lane == 0 || pw != w)) {
         uint32_t *p = base + w;
-        if ((__atomic_load_n(p, __ATOMIC_RELAXED) & msk) != msk) atomicOr(p, msk);
+        if ((__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & msk) != msk) atomicOr(p, msk);
     }
 }
 
@@ -1428,10 +1436,11 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
         const uint32_t nrows = slot_of(cd);
         if (!nrows) continue;
         const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
+        const gcu32 *nbt = as_global(cd.nb_table);
         uint32_t *out = cd.table + (cd.offsets[i] - cd.off_base) * k;
         uint32_t w = 0;
         for (uint64_t e = b0; e < b1 && w < nrows; e++) {
-            const uint32_t q = cd.nb_table[e];
+            const uint32_t q = nbt[e];
             const float qx = D.sx[q], qy = D.sy[q];
             bool ok = true;
 #pragma unroll

@@ -693,6 +693,8 @@ __device__ inline float sqdist(float x1, float y1, float x2, float y2) {
     return sx + sy;
 }
 
+constexpr int kJoinUnroll = 4;  // list entries whose loads are issued together
+
 // A pointer read from a descriptor in memory, as a global-memory pointer
 // (global rather than flat loads).
 typedef __attribute__((address_space(1))) const uint32_t gcu32;
@@ -817,7 +819,22 @@ __device__ inline uint32_t join2_count_anchor(const DevData &D, uint32_t a, uint
     uint32_t cnt = 0;
     BitAcc acc(bits_b);
     for (int r = 0; r < m; r++) {
-        for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
+        uint32_t q = rs.lo[r];
+        for (; q + kJoinUnroll <= rs.hi[r]; q += kJoinUnroll) {
+            float qx[kJoinUnroll], qy[kJoinUnroll];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) {
+                qx[u] = D.sx[q + u];
+                qy[u] = D.sy[q + u];
+            }
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++)
+                if (sqdist(xa, ya, qx[u], qy[u]) <= D.g.r2) {
+                    cnt++;
+                    acc.set(q + u - sb);
+                }
+        }
+        for (; q < rs.hi[r]; q++) {
             if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
                 cnt++;
                 acc.set(q - sb);
@@ -879,16 +896,29 @@ __device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t
     uint32_t *out = table + off * 2;
     uint32_t *nb = nbr ? nbr + off : nullptr;
     uint32_t w = 0;
-    for (int r = 0; r < m && w < nrows; r++) {
-        for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
-            const float qx = D.sx[q], qy = D.sy[q];
-            if (sqdist(xa, ya, qx, qy) <= D.g.r2) {
-                out[2 * w] = g;
-                out[2 * w + 1] = SORTED ? q : D.sidx[q];
-                if (nb) nb[w] = q;
-                w++;
-            }
+    auto emit = [&](uint32_t q) {
+        if (w < nrows) {
+            out[2 * w] = g;
+            out[2 * w + 1] = SORTED ? q : D.sidx[q];
+            if (nb) nb[w] = q;
         }
+        w++;
+    };
+    for (int r = 0; r < m && w < nrows; r++) {
+        uint32_t q = rs.lo[r];
+        for (; q + kJoinUnroll <= rs.hi[r] && w < nrows; q += kJoinUnroll) {
+            float qx[kJoinUnroll], qy[kJoinUnroll];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) {
+                qx[u] = D.sx[q + u];
+                qy[u] = D.sy[q + u];
+            }
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++)
+                if (sqdist(xa, ya, qx[u], qy[u]) <= D.g.r2) emit(q + u);
+        }
+        for (; q < rs.hi[r] && w < nrows; q++)
+            if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) emit(q);
     }
 }
 
@@ -1117,9 +1147,7 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         uint32_t *bt = bits + cd.bits_off;
         uint32_t cnt = 0;
         BitAcc acc(bt + cd.seg_off[km1]);
-        for (uint64_t e = b0; e < b1; e++) {
-            const uint32_t q = nbt[e];
-            const float qx = D.sx[q], qy = D.sy[q];
+        auto test = [&](uint32_t q, float qx, float qy) {
             bool ok = true;
 #pragma unroll
             for (uint32_t m = 1; m < (uint32_t)kArr; m++) {
@@ -1130,6 +1158,25 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
                 cnt++;
                 acc.set(q - sl);
             }
+        };
+        uint64_t e = b0;
+        // four entries per step: their loads are issued together
+        for (; e + kJoinUnroll <= b1; e += kJoinUnroll) {
+            uint32_t q[kJoinUnroll];
+            float qx[kJoinUnroll], qy[kJoinUnroll];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) {
+                qx[u] = D.sx[q[u]];
+                qy[u] = D.sy[q[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
+        }
+        for (; e < b1; e++) {
+            const uint32_t q = nbt[e];
+            test(q, D.sx[q], D.sy[q]);
         }
         acc.flush();
         slots[cd.slots_off + i] = cnt;
@@ -1439,9 +1486,7 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
         const gcu32 *nbt = as_global(cd.nb_table);
         uint32_t *out = cd.table + (cd.offsets[i] - cd.off_base) * k;
         uint32_t w = 0;
-        for (uint64_t e = b0; e < b1 && w < nrows; e++) {
-            const uint32_t q = nbt[e];
-            const float qx = D.sx[q], qy = D.sy[q];
+        auto test = [&](uint32_t q, float qx, float qy) {
             bool ok = true;
 #pragma unroll
             for (uint32_t m = 1; m < (uint32_t)kArr; m++) {
@@ -1449,14 +1494,34 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
                 if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) ok = false;
             }
             if (ok) {
+                if (w < nrows) {
 #pragma unroll
-                for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
-                    if (m >= km1) break;
-                    out[(uint64_t)w * k + m] = ids[m];
+                    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+                        if (m >= km1) break;
+                        out[(uint64_t)w * k + m] = ids[m];
+                    }
+                    out[(uint64_t)w * k + km1] = q;
                 }
-                out[(uint64_t)w * k + km1] = q;
                 w++;
             }
+        };
+        uint64_t e = b0;
+        for (; e + kJoinUnroll <= b1 && w < nrows; e += kJoinUnroll) {
+            uint32_t q[kJoinUnroll];
+            float qx[kJoinUnroll], qy[kJoinUnroll];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) {
+                qx[u] = D.sx[q[u]];
+                qy[u] = D.sy[q[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
+        }
+        for (; e < b1 && w < nrows; e++) {
+            const uint32_t q = nbt[e];
+            test(q, D.sx[q], D.sy[q]);
         }
     }
 }

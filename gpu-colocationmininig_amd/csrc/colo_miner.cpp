@@ -1543,7 +1543,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
         std::vector<Axis> axes = {{&TuneParams::count_var, {0, 1, 2}, 0},
                                   {&TuneParams::write_var, {0, 1, 2}, 1},
                                   {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
-        axes.push_back({&TuneParams::item_rows, {256, 1024, 4096}, 2});
+        axes.push_back({&TuneParams::item_rows, {1024, 4096, 16384}, 2});
         for (const Axis &ax : axes) {
             const uint32_t start = best.*(ax.field);
             for (uint32_t v : ax.values) {

@@ -539,10 +539,10 @@ struct Miner {
         if (pred) {
             double total = 0;
             for (auto &r : runs) total += cost[r.first];
-            // about 16 items per workgroup at equal predicted cost, each
-            // between 1/4 and 4x the tuned item size
-            double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 16));
-            const double lo = std::max(64.0, c->tune.item_rows / 4.0), hi = c->tune.item_rows * 4.0;
+            // about 8 items per workgroup at equal predicted cost, each
+            // between 1/4 and 8x the tuned item size
+            double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 8));
+            const double lo = std::max(64.0, c->tune.item_rows / 4.0), hi = c->tune.item_rows * 8.0;
             for (auto &r : runs) {
                 uint64_t n = desc[r.first].n_anchor;
                 double per_row = n ? cost[r.first] / (double)n : 1.0;

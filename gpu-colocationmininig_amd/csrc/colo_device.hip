@@ -2332,15 +2332,6 @@ int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s
     COLO_HIP(hipStreamSynchronize(s));
     return COLO_OK;
 }
-int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s) {
-    if (!n) return fill_u32((uint32_t *)d_offsets, 2, 0, s);
-    void *tmp;
-    int rc = scratch(c, 8 * scan_tmp_words(n) + 16, &tmp);
-    if (rc) return rc;
-    // d_offsets has n + 1 entries; the sum lands at [n]
-    scan_rec<uint32_t, uint64_t>(d_slots, n, d_offsets, d_offsets + n, (uint64_t *)tmp, s);
-    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "scan launch");
-}
 }  // namespace colo
 
 // Table checksum independent of row order: the 64-bit sum over rows of a
@@ -2369,10 +2360,6 @@ __global__ __launch_bounds__(kBlock) void k_table_hash(const uint32_t *t, uint64
     if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)tot);
 }
 
-__global__ void k_gather_u64(const uint64_t *a, const uint64_t *idx, uint64_t m, uint64_t *out) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) out[i] = a[idx[i]];
-}
 
 namespace colo {
 int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, const uint32_t *d_map, uint64_t *h,
@@ -2389,10 +2376,5 @@ int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, cons
     COLO_HIP(hipStreamSynchronize(s));
     *h = c->h_pinned[0];
     return COLO_OK;
-}
-int gather_u64(colo_ctx *c, const uint64_t *d_a, const uint64_t *d_idx, uint64_t m, uint64_t *d_out, hipStream_t s) {
-    if (!m) return COLO_OK;
-    hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(m, 1u << 30)), dim3(kBlock), 0, s, d_a, d_idx, m, d_out);
-    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "gather launch");
 }
 }  // namespace colo

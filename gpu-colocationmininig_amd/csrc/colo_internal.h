@@ -249,8 +249,6 @@ int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uin
 // stored sorted positions to instance ids.
 int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, const uint32_t *d_map, uint64_t *h,
                hipStream_t s);
-int gather_u64(colo_ctx *c, const uint64_t *d_a, const uint64_t *d_idx, uint64_t m, uint64_t *d_out, hipStream_t s);
-int scan_batch(colo_ctx *c, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets, hipStream_t s);
 // Exclusive scans of many slot arrays in one pass: out has n + 1 entries,
 // the total last.  Waits for the stream before returning.
 struct ScanSeg {

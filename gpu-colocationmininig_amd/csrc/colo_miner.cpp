@@ -635,13 +635,9 @@ struct Miner {
             }
         }
 
-        level_desc_ = std::move(desc);
-        level_idx_ = std::move(idx);
         return COLO_OK;
     }
 
-    std::vector<CandDesc> level_desc_;
-    std::vector<uint64_t> level_idx_;
 
     int build_tables_persistent(uint32_t k, std::vector<Prev> &L, const std::vector<uint8_t> &build,
                                 const std::vector<Prev> &Lprev, const std::vector<uint32_t> &C) {

@@ -1160,19 +1160,28 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
             }
         };
         uint64_t e = b0;
-        // four entries per step: their loads are issued together
-        for (; e + kJoinUnroll <= b1; e += kJoinUnroll) {
-            uint32_t q[kJoinUnroll];
-            float qx[kJoinUnroll], qy[kJoinUnroll];
+        // kJoinUnroll entries per step, software-pipelined: the positions of
+        // the next step are loaded while this step's coordinates arrive
+        if (e + kJoinUnroll <= b1) {
+            uint32_t qn[kJoinUnroll];
 #pragma unroll
-            for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
+            for (int u = 0; u < kJoinUnroll; u++) qn[u] = nbt[e + u];
+            for (; e + kJoinUnroll <= b1; e += kJoinUnroll) {
+                uint32_t q[kJoinUnroll];
+                float qx[kJoinUnroll], qy[kJoinUnroll];
 #pragma unroll
-            for (int u = 0; u < kJoinUnroll; u++) {
-                qx[u] = D.sx[q[u]];
-                qy[u] = D.sy[q[u]];
+                for (int u = 0; u < kJoinUnroll; u++) {
+                    q[u] = qn[u];
+                    qx[u] = D.sx[q[u]];
+                    qy[u] = D.sy[q[u]];
+                }
+                if (e + 2 * kJoinUnroll <= b1) {
+#pragma unroll
+                    for (int u = 0; u < kJoinUnroll; u++) qn[u] = nbt[e + kJoinUnroll + u];
+                }
+#pragma unroll
+                for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
             }
-#pragma unroll
-            for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
         }
         for (; e < b1; e++) {
             const uint32_t q = nbt[e];

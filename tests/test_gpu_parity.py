@@ -263,16 +263,20 @@ def test_seattle_subset(colo_mod, oracle_mod):
     assert rep == orc.mine(10.0, 0.3, max_k=3)
 
 
-@pytest.mark.parametrize("sched", [0, 1])
-def test_threshold_sweep(c2small, sched):
+@pytest.mark.parametrize("sched", [0, 1, 2])
+def test_threshold_sweep(c2small, colo_mod, sched):
     """One sweep over several thresholds equals one mining run per threshold
-    (theta-independent results are reused across the sweep)."""
+    (theta-independent results are reused across the sweep), under every
+    schedule (the learned one with the shipped cost model)."""
     ctx, orc, p = c2small
     _say(f"threshold sweep sched={sched}")
+    model = None
+    if sched == colo_mod.COLO_SCHED_LEARNED:
+        model = colo_mod.CostModel.load(os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin"))
     thetas = [0.25, 0.1, 0.15]
-    out = ctx.mine_sweep(thetas, max_k=4, sched=sched)
+    out = ctx.mine_sweep(thetas, max_k=4, sched=sched, model=model)
     for th, (rep, st) in zip(thetas, out):
-        single, _ = ctx.mine(th, max_k=4, sched=sched)
+        single, _ = ctx.mine(th, max_k=4, sched=sched, model=model)
         assert rep == single, th
     assert out[0][0] == orc.mine(p["r"], 0.25, max_k=4)
 

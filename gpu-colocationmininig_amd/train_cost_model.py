@@ -51,7 +51,11 @@ def main():
     ap.add_argument("--seed", type=int, default=4000)
     ap.add_argument("--eval-n", type=int, default=20_000_000)
     ap.add_argument("--workers", type=int, default=12)
+    ap.add_argument("--eval-only", action="store_true",
+                    help="load --out and redo only the held-out evaluation (updates its .json)")
     a = ap.parse_args()
+    if a.eval_only:
+        return evaluate_only(a)
     rng = np.random.default_rng(a.seed)
     jobs = []
     for i in range(a.datasets):
@@ -96,13 +100,25 @@ def main():
     x, y, fc, S, p = ev.get()
     pool.close()
     pool.join()
+    summary = {"records": int(len(X)), "datasets": a.datasets, "skipped": skipped, "final_loss": loss,
+               "train_mean_abs_log_err": train_err, "model": os.path.basename(a.out)}
+    summary.update(held_out(a, m, x, y, fc, S, p, t0))
+    print(json.dumps(summary), flush=True)
+    with open(os.path.splitext(a.out)[0] + ".json", "w") as fh:
+        json.dump(summary, fh, indent=1)
+
+
+def held_out(a, m, x, y, fc, S, p, t0):
+    """Held-out evaluation: model error on the persistent run's log, and the
+    learned / persistent / plain mining times (identical reports)."""
+    import colo
+    import torch
+    tmp = tempfile.mkdtemp(prefix="colo_eval_")
     ctx = colo.Context(x, y, fc, [0, 0, S, S], p["r"])
     elog = os.path.join(tmp, "eval.bin")
     rep_p, _ = ctx.mine(p["theta"], max_k=4, sched=colo.COLO_SCHED_PERSISTENT, hash_tables=False, log_path=elog)
     Xe, Te = colo.read_log(elog)
-    eval_err = float(np.mean(np.abs(m.predict(Xe) - Te))) if len(Xe) else None
-    times = {}
-    reps = {}
+    times, reps, lst = {}, {}, None
     for name, sched, model in (("learned", colo.COLO_SCHED_LEARNED, m), ("persistent", colo.COLO_SCHED_PERSISTENT, None),
                                ("plain", colo.COLO_SCHED_PLAIN, None)):
         ctx.mine(p["theta"], max_k=4, sched=sched, model=model, hash_tables=False)  # warm
@@ -113,18 +129,33 @@ def main():
         times[name] = time.perf_counter() - ts
         if name == "learned":
             lst = st
-    summary = {"records": int(len(X)), "datasets": a.datasets, "skipped": skipped, "final_loss": loss,
-               "train_mean_abs_log_err": train_err, "eval_records": int(len(Xe)),
-               "eval_mean_abs_log_err": eval_err, "eval_instances": int(len(x)), "eval_features": int(p["F"]),
-               "eval_theta": p["theta"], "eval_r": p["r"],
-               "eval_learned_mean_abs_log_err": lst["model_abs_err"] / max(lst["model_n"], 1),
-               "eval_overflow_fallbacks": lst["overflows"], "eval_arenas": lst["arenas"],
-               "eval_mining_s": times, "eval_reports_identical": reps["learned"] == reps["persistent"] == reps["plain"]
-               == rep_p, "seconds": time.time() - t0, "model": os.path.basename(a.out)}
+    return {"eval_records": int(len(Xe)),
+            "eval_mean_abs_log_err": float(np.mean(np.abs(m.predict(Xe) - Te))) if len(Xe) else None,
+            "eval_instances": int(len(x)), "eval_features": int(p["F"]), "eval_theta": p["theta"], "eval_r": p["r"],
+            "eval_learned_mean_abs_log_err": lst["model_abs_err"] / max(lst["model_n"], 1),
+            "eval_overflow_fallbacks": lst["overflows"], "eval_arenas": lst["arenas"], "eval_mining_s": times,
+            "eval_reports_identical": reps["learned"] == reps["persistent"] == reps["plain"] == rep_p,
+            "seconds": time.time() - t0}
+
+
+def evaluate_only(a):
+    import colo
+    t0 = time.time()
+    ecfg = datagen.training_config(np.random.default_rng(a.seed + 999), n_range=(a.eval_n, a.eval_n))
+    ecfg["F"] = 40
+    x, y, fc, S, p = make_dataset((a.seed + 999, ecfg))
+    m = colo.CostModel.load(a.out)
+    js = os.path.splitext(a.out)[0] + ".json"
+    summary = {}
+    if os.path.exists(js):
+        with open(js) as fh:
+            summary = json.load(fh)
+    summary.update(held_out(a, m, x, y, fc, S, p, t0))
     print(json.dumps(summary), flush=True)
-    with open(os.path.splitext(a.out)[0] + ".json", "w") as fh:
+    with open(js, "w") as fh:
         json.dump(summary, fh, indent=1)
+    return 0
 
 
 if __name__ == "__main__":
-    main()
+    sys.exit(main())

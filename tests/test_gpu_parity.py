@@ -368,3 +368,27 @@ def test_full_c2_all_schedules(datagen, colo_mod, oracle_mod):
                      (colo_mod.COLO_SCHED_LEARNED, model)):
         rep, st = ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=m)
         assert rep == expect, sched
+
+
+@pytest.mark.parametrize("sched", ["plain", "persistent", "learned"])
+def test_cli_drop_in(datagen, colo_mod, oracle_mod, tmp_path, sched):
+    """The drop-in executable reads the reference's config.txt, parameter.txt
+    and CSV and writes ColocationDegree_<k>.txt files equal to the oracle's,
+    then its timing line (cpp:1144)."""
+    import subprocess
+    _say(f"cli drop-in {sched}")
+    w = datagen.write_workload(str(tmp_path / "case"), "c1", N=6000)
+    exe = os.path.join(os.path.dirname(colo_mod.__file__), "colo_miner")
+    r = subprocess.run([exe, w["config"], "--sched", sched, "--max-k", "4"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip(), "no timing line"
+    orc = oracle_mod.Oracle.from_csv(w["csv"])
+    o = str(tmp_path / "orc_")
+    orc.mine(w["params"]["r"], w["params"]["theta"], max_k=4, out_prefix=o)
+    loc = str(tmp_path / "case") + "/"
+    for k in (2, 3, 4):
+        fg, fo = loc + f"ColocationDegree_{k}.txt", o + f"ColocationDegree_{k}.txt"
+        assert os.path.exists(fg) == os.path.exists(fo), k
+        if os.path.exists(fo):
+            assert open(fg).read() == open(fo).read(), k

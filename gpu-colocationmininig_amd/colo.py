@@ -46,11 +46,17 @@ class MineOpts(C.Structure):
                 ("model", C.c_void_p), ("log_path", C.c_char_p)]
 
 
+TUNE_UNSET = 0xFFFFFFFF
+
+
 class Tune(C.Structure):
-    _fields_ = [(n, C.c_uint32) for n in ("count_var", "write_var", "blocks_per_cu", "item_rows")]
+    _fields_ = [(n, C.c_uint32) for n in ("count_var", "write_var", "blocks_per_cu", "item_rows")] + \
+               [("count_var_k", C.c_uint32 * 16)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["count_var_k"] = list(self.count_var_k)
+        return d
 
 
 class MineStats(C.Structure):
@@ -63,11 +69,13 @@ class MineStats(C.Structure):
                [(n, C.c_double) for n in ("l2_count_kernel_ms", "l2_count_bytes", "l2_write_kernel_ms",
                                           "l2_write_bytes", "model_abs_err", "model_n", "arena_bytes")] + \
                [(n, C.c_uint64) for n in ("overflows", "arenas")] + \
-               [(n, C.c_double) for n in ("model_ms", "alloc_ms", "alloc_bytes")] + [("level_ms", C.c_double * 16)]
+               [(n, C.c_double) for n in ("model_ms", "alloc_ms", "alloc_bytes")] + [("level_ms", C.c_double * 16)] + \
+               [("level_count_kernel_ms", C.c_double * 16)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
         d["level_ms"] = list(self.level_ms)
+        d["level_count_kernel_ms"] = list(self.level_count_kernel_ms)
         return d
 
 
@@ -267,9 +275,16 @@ class Context:
 
     @tune.setter
     def tune(self, d):
+        """Sets launch shapes; a count_var without count_var_k applies to
+        every level."""
         cur = self.tune
         cur.update(d)
-        self._check(lib().colo_set_tune(self.h, C.byref(Tune(**cur))))
+        if "count_var" in d and "count_var_k" not in d:
+            cur["count_var_k"] = [TUNE_UNSET] * 16
+        t = Tune(**{k: v for k, v in cur.items() if k != "count_var_k"})
+        for i, v in enumerate(cur["count_var_k"]):
+            t.count_var_k[i] = v
+        self._check(lib().colo_set_tune(self.h, C.byref(t)))
 
     @_on_stream
     def autotune(self, theta, max_k=0, sched=COLO_SCHED_PERSISTENT, model=None, cache=None, pi_mode=0,

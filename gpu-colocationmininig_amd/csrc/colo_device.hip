@@ -1199,6 +1199,82 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
     }
 }
 
+// Level-k count of two adjacent parent rows i, i + 1 with the same anchor:
+// every candidate's list is read once and each entry tested against both
+// rows (results equal two calls of joink_count_row_multi).
+template <int KM1>
+__device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
+                                         uint32_t *slots, uint32_t *bits, unsigned long long *s_rows) {
+    const CandDesc &c0 = descs[0];
+    const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
+    constexpr int kArr = KM1 ? KM1 : kMaxK;
+    const uint32_t *row0 = c0.parent + i * km1, *row1 = row0 + km1;
+    uint32_t ids0[kArr], ids1[kArr];
+    float mx0[kArr], my0[kArr], mx1[kArr], my1[kArr];
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        ids0[m] = row0[m];
+        ids1[m] = row1[m];
+        if (m) {
+            mx0[m] = D.sx[ids0[m]];
+            my0[m] = D.sy[ids0[m]];
+            mx1[m] = D.sx[ids1[m]];
+            my1[m] = D.sy[ids1[m]];
+        }
+    }
+    const uint32_t t = ids0[0] - D.fstart[c0.f[0]];
+    for (uint32_t j = 0; j < ncand; j++) {
+        const CandDesc &cd = descs[j];
+        const gcu32 *nbt = as_global(cd.nb_table);
+        const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
+        const uint32_t sl = D.fstart[cd.f[km1]];
+        uint32_t *bt = bits + cd.bits_off;
+        uint32_t cnt0 = 0, cnt1 = 0;
+        BitAcc acc(bt + cd.seg_off[km1]);
+        auto test = [&](uint32_t q, float qx, float qy) {
+            bool ok0 = true, ok1 = true;
+#pragma unroll
+            for (uint32_t m = 1; m < (uint32_t)kArr; m++) {
+                if (m >= km1) break;
+                if (sqdist(mx0[m], my0[m], qx, qy) > D.g.r2) ok0 = false;
+                if (sqdist(mx1[m], my1[m], qx, qy) > D.g.r2) ok1 = false;
+            }
+            cnt0 += ok0;
+            cnt1 += ok1;
+            if (ok0 || ok1) acc.set(q - sl);
+        };
+        uint64_t e = b0;
+        for (; e + kJoinUnroll <= b1; e += kJoinUnroll) {
+            uint32_t q[kJoinUnroll];
+            float qx[kJoinUnroll], qy[kJoinUnroll];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) {
+                qx[u] = D.sx[q[u]];
+                qy[u] = D.sy[q[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
+        }
+        for (; e < b1; e++) {
+            const uint32_t q = nbt[e];
+            test(q, D.sx[q], D.sy[q]);
+        }
+        acc.flush();
+        slots[cd.slots_off + i] = cnt0;
+        slots[cd.slots_off + i + 1] = cnt1;
+        if (cnt0 + cnt1) atomicAdd(&s_rows[j], (unsigned long long)(cnt0 + cnt1));
+#pragma unroll
+        for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+            if (m >= km1) break;
+            if (cnt0) set_bit(bt + cd.seg_off[m], ids0[m] - D.fstart[cd.f[m]]);
+            if (cnt1) set_bit(bt + cd.seg_off[m], ids1[m] - D.fstart[cd.f[m]]);
+        }
+    }
+}
+
 // Sets bit `rank` of `base` for every lane with `hit`, one atomic per
 // distinct 32-bit word.  Ranks must be ascending over the participating
 // lanes (list entries are sorted positions), so lanes of one word are
@@ -1432,9 +1508,11 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // Level k >= 3 (KM1 = k - 1; 0 for any k).  VAR 0: one parent row per
 // thread, all of a run's candidates per row; VAR 2: one parent row per
 // thread, the item's candidates one after another (one neighbour-list
-// stream at a time); VAR 1: wave-cooperative (wave_rows_multi) over 64-row
-// chunks taken by the block's waves in turn.  Rows per candidate are summed
-// in LDS over the item and added to the candidates' totals.
+// stream at a time); VAR 3: two adjacent rows per thread, their lists read
+// once when they share the anchor; VAR 1: wave-cooperative
+// (wave_rows_multi) over 64-row chunks taken by the block's waves in turn.
+// Rows per candidate are summed in LDS over the item and added to the
+// candidates' totals.
 template <int KM1, int VAR>
 __global__ __launch_bounds__(kBlock) void k_countk_batch(DevData D, const CandDesc *desc, const WorkItem *items,
                                                          const uint64_t *seg, uint32_t *slots, uint32_t *bits,
@@ -1452,6 +1530,19 @@ __global__ __launch_bounds__(kBlock) void k_countk_batch(DevData D, const CandDe
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                     joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits, s_rows + j);
+        } else if (VAR == 3) {
+            // two adjacent rows per thread, one list pass when they share the anchor
+            const CandDesc &c0 = desc[wi.cand];
+            const uint32_t km1 = c0.k - 1;
+            for (uint64_t i = wi.begin + 2 * (uint64_t)threadIdx.x; i < wi.end; i += 2 * kBlock) {
+                if (i + 1 < wi.end && c0.parent[i * km1] == c0.parent[(i + 1) * km1]) {
+                    joink_count_rows2<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows);
+                } else {
+                    joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows);
+                    if (i + 1 < wi.end)
+                        joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i + 1, slots, bits, s_rows);
+                }
+            }
         } else {
             for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
                 const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
@@ -1595,6 +1686,7 @@ static auto pick_k(uint32_t k) -> decltype(Pick<0>::get()) {
 template <int KM1> struct PickCount0 { static auto get() { return &k_countk_batch<KM1, 0>; } };
 template <int KM1> struct PickCount1 { static auto get() { return &k_countk_batch<KM1, 1>; } };
 template <int KM1> struct PickCount2 { static auto get() { return &k_countk_batch<KM1, 2>; } };
+template <int KM1> struct PickCount3 { static auto get() { return &k_countk_batch<KM1, 3>; } };
 template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
 template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
 template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
@@ -1608,11 +1700,15 @@ int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
     if (k == 2)
         hipLaunchKernelGGL(k_count2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
                            d_items, d_seg, d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
-    else
-        hipLaunchKernelGGL(c->tune.count_var == 2 ? pick_k<PickCount2>(k)
-                           : c->tune.count_var == 1 ? pick_k<PickCount1>(k) : pick_k<PickCount0>(k),
+    else {
+        const uint32_t var = c->tune.count_var_of(k);
+        hipLaunchKernelGGL(var == 3   ? pick_k<PickCount3>(k)
+                           : var == 2 ? pick_k<PickCount2>(k)
+                           : var == 1 ? pick_k<PickCount1>(k)
+                                      : pick_k<PickCount0>(k),
                            dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
                            d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
+    }
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "count_batch launch");
 }
 
@@ -2032,6 +2128,7 @@ int colo_graph_info(const colo_ctx *c, uint32_t *captures, uint32_t *replays) {
 int colo_get_tune(const colo_ctx *c, colo_tune *o) {
     if (!c || !o) return COLO_EARG;
     o->count_var = c->tune.count_var;
+    for (int k = 0; k < 16; k++) o->count_var_k[k] = c->tune.count_var_k[k];
     o->write_var = c->tune.write_var;
     o->blocks_per_cu = c->tune.blocks_per_cu;
     o->item_rows = c->tune.item_rows;
@@ -2040,10 +2137,14 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
 
 int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
-    if (t->count_var > 2 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+    for (int k = 0; k < 16; k++)
+        if (t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 3)
+            return set_error(c, COLO_EARG, "launch shape out of range");
+    if (t->count_var > 3 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;
+    for (int k = 0; k < 16; k++) c->tune.count_var_k[k] = t->count_var_k[k];
     c->tune.write_var = t->write_var;
     c->tune.blocks_per_cu = t->blocks_per_cu;
     c->tune.item_rows = t->item_rows;

@@ -174,7 +174,14 @@ struct DevPool {
 // Launch shapes of the batched passes (chosen by the autotuner).
 struct TuneParams {
     uint32_t count_var = 0;       // level-k count: 0 row per thread, 1 wave-cooperative,
-                                  // 2 row per thread, candidates in turn
+                                  // 2 row per thread, candidates in turn, 3 two rows per thread
+    uint32_t count_var_k[16] = {COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET,
+                                COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET,
+                                COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET,
+                                COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET};
+    uint32_t count_var_of(uint32_t k) const {
+        return (k < 16 && count_var_k[k] != COLO_TUNE_UNSET) ? count_var_k[k] : count_var;
+    }
     uint32_t write_var = 0;       // level-k write: same choice
     uint32_t blocks_per_cu = 8;   // persistent grid: workgroups per CU
     uint32_t item_rows = 1024;    // rows (anchors) per work item without the cost model

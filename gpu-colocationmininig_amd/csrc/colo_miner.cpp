@@ -211,7 +211,7 @@ struct Miner {
     enum Cat { kCount, kWrite, kFilter, kPi, kL2Count, kL2Write };
     std::vector<hipEvent_t> pool_;
     size_t used_ = 0;
-    struct Span { size_t a, b; int cat; };
+    struct Span { size_t a, b; int cat; uint32_t k; };
     std::vector<Span> spans_;
     size_t mark() {
         if (used_ == pool_.size()) {
@@ -222,13 +222,14 @@ struct Miner {
         (void)hipEventRecord(pool_[used_], s);
         return used_++;
     }
-    void span(size_t a, size_t b, int cat) {
-        if (a != SIZE_MAX && b != SIZE_MAX) spans_.push_back(Span{a, b, cat});
+    void span(size_t a, size_t b, int cat, uint32_t k = 0) {
+        if (a != SIZE_MAX && b != SIZE_MAX) spans_.push_back(Span{a, b, cat, k});
     }
     void harvest() {
         for (const Span &sp : spans_) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, pool_[sp.a], pool_[sp.b]) != hipSuccess) continue;
+            if ((sp.cat == kCount || sp.cat == kL2Count) && sp.k < 16) st.level_count_kernel_ms[sp.k] += ms;
             switch (sp.cat) {
                 case kCount: st.count_kernel_ms += ms; break;
                 case kWrite: st.write_kernel_ms += ms; break;
@@ -574,7 +575,7 @@ struct Miner {
                          slots_arena->as<uint32_t>(), arena_bits.as<uint32_t>(), d_rowsel.as<uint64_t>(),
                          d_queue.as<uint32_t>(), s);
         if (rc) return rc;
-        span(e0, mark(), k == 2 ? kL2Count : kCount);
+        span(e0, mark(), k == 2 ? kL2Count : kCount, k);
         if (pred) {
             // pre-size the level's table arena from the predicted rows while
             // the count pass runs (the exact sizes are known only after it)
@@ -1479,25 +1480,39 @@ static std::string tune_key(const colo_ctx *c, const colo_mine_opts &o) {
     for (char &ch : arch)
         if (ch == ' ') ch = '_';
     char b[256];
-    snprintf(b, sizeof b, "%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
+    snprintf(b, sizeof b, "v2/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
              fbits(o.theta), o.max_k, o.sched);
     return b;
 }
 
+// Cache lines: key, count_var, the 16 per-level count variants, write_var,
+// blocks_per_cu, item_rows.
 static bool tune_load(const char *path, const std::string &key, TuneParams *t) {
     FILE *fp = fopen(path, "rb");
     if (!fp) return false;
-    char k[256];
-    unsigned a, b, cc, d;
+    char line[1024];
     bool found = false;
-    while (fscanf(fp, "%255s %u %u %u %u", k, &a, &b, &cc, &d) == 5)
-        if (key == k) {  // the last entry of a key wins
-            t->count_var = a;
-            t->write_var = b;
-            t->blocks_per_cu = cc;
-            t->item_rows = d;
-            found = true;
+    while (fgets(line, sizeof line, fp)) {
+        char k[256];
+        int used = 0;
+        if (sscanf(line, "%255s%n", k, &used) != 1 || key != k) continue;
+        unsigned long v[20];
+        const char *p = line + used;
+        int n = 0;
+        for (; n < 20; n++) {
+            char *e;
+            v[n] = strtoul(p, &e, 10);
+            if (e == p) break;
+            p = e;
         }
+        if (n != 20) continue;
+        t->count_var = (uint32_t)v[0];  // the last entry of a key wins
+        for (int i = 0; i < 16; i++) t->count_var_k[i] = (uint32_t)v[1 + i];
+        t->write_var = (uint32_t)v[17];
+        t->blocks_per_cu = (uint32_t)v[18];
+        t->item_rows = (uint32_t)v[19];
+        found = true;
+    }
     fclose(fp);
     return found;
 }
@@ -1512,8 +1527,9 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
     const std::string key = tune_key(c, o);
     TuneParams best = c->tune;
     if (!(cache_path && tune_load(cache_path, key, &best))) {
-        // kernel time of one run: {level-k count, level-k write, all batched passes}
-        auto measure = [&](const TuneParams &t, double *m3) -> int {
+        // kernel time of one run: {level-k count, level-k write, all batched
+        // passes}; lvl: count-pass kernel time per level
+        auto measure = [&](const TuneParams &t, double *m3, double *lvl) -> int {
             c->tune = t;
             Miner M(c, o);
             int rc = M.run(nullptr);
@@ -1522,12 +1538,45 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             m3[0] = M.st.count_kernel_ms - M.st.l2_count_kernel_ms;
             m3[1] = M.st.write_kernel_ms - M.st.l2_write_kernel_ms;
             m3[2] = M.st.count_kernel_ms + M.st.write_kernel_ms;
+            if (lvl) memcpy(lvl, M.st.level_count_kernel_ms, sizeof M.st.level_count_kernel_ms);
             return COLO_OK;
         };
         double cur[3], m[3];
-        int rc = measure(best, cur);  // warm-up (pool, cached features)
-        if (!rc) rc = measure(best, cur);
+        int rc = measure(best, cur, nullptr);  // warm-up (pool, cached features)
         if (rc) {
+            c->tune = best;
+            return rc;
+        }
+        // count variant per level: every variant on every level, the fastest kept
+        {
+            const uint32_t vars[] = {0, 1, 2, 3};
+            double lv[4][16];
+            for (int vi = 0; vi < 4; vi++) {
+                TuneParams t = best;
+                t.count_var = vars[vi];
+                for (int k = 0; k < 16; k++) t.count_var_k[k] = COLO_TUNE_UNSET;
+                if ((rc = measure(t, m, lv[vi]))) {
+                    c->tune = best;
+                    return rc;
+                }
+            }
+            double tot[4] = {0, 0, 0, 0};
+            for (int vi = 0; vi < 4; vi++)
+                for (int k = 3; k < 16; k++) tot[vi] += lv[vi][k];
+            int bv = 0;
+            for (int vi = 1; vi < 4; vi++)
+                if (tot[vi] < tot[bv]) bv = vi;
+            best.count_var = vars[bv];
+            for (int k = 0; k < 16; k++) {
+                best.count_var_k[k] = COLO_TUNE_UNSET;
+                if (k < 3 || lv[0][k] <= 0) continue;
+                int kv = 0;
+                for (int vi = 1; vi < 4; vi++)
+                    if (lv[vi][k] < lv[kv][k]) kv = vi;
+                best.count_var_k[k] = vars[kv];
+            }
+        }
+        if ((rc = measure(best, cur, nullptr))) {
             c->tune = best;
             return rc;
         }
@@ -1536,8 +1585,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             std::vector<uint32_t> values;
             int metric;
         };
-        std::vector<Axis> axes = {{&TuneParams::count_var, {0, 1, 2}, 0},
-                                  {&TuneParams::write_var, {0, 1, 2}, 1},
+        std::vector<Axis> axes = {{&TuneParams::write_var, {0, 1, 2}, 1},
                                   {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
         axes.push_back({&TuneParams::item_rows, {1024, 4096, 16384}, 2});
         for (const Axis &ax : axes) {
@@ -1546,7 +1594,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
                 if (v == start) continue;
                 TuneParams t = best;
                 t.*(ax.field) = v;
-                if ((rc = measure(t, m))) {
+                if ((rc = measure(t, m, nullptr))) {
                     c->tune = best;
                     return rc;
                 }
@@ -1559,8 +1607,9 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
         if (cache_path) {
             FILE *fp = fopen(cache_path, "ab");
             if (fp) {
-                fprintf(fp, "%s %u %u %u %u\n", key.c_str(), best.count_var, best.write_var, best.blocks_per_cu,
-                        best.item_rows);
+                fprintf(fp, "%s %u", key.c_str(), best.count_var);
+                for (int k = 0; k < 16; k++) fprintf(fp, " %u", best.count_var_k[k]);
+                fprintf(fp, " %u %u %u\n", best.write_var, best.blocks_per_cu, best.item_rows);
                 fclose(fp);
             }
         }

@@ -224,6 +224,7 @@ typedef struct colo_mine_stats {
     double alloc_ms;        /* host time in hipMalloc / hipFree (pool misses) */
     double alloc_bytes;     /* bytes obtained from hipMalloc during the run   */
     double level_ms[16];    /* wall time of level k (index k)                 */
+    double level_count_kernel_ms[16]; /* count-pass kernel time of level k    */
 } colo_mine_stats;
 
 /* HIP graphs: the binning (colo_bin, colo_ctx_create) is captured once per
@@ -243,14 +244,19 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         row; 1 = wave-cooperative over 64-row chunks (a
  *                         row group's neighbour list spread over the
  *                         lanes); 2 = one parent row per thread, the work
- *                         item's candidates one after another
+ *                         item's candidates one after another; 3 (count
+ *                         only) = two adjacent rows per thread, their list
+ *                         read once when they share the anchor
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between
  *                         item_rows / 4 and 4 x item_rows (64..65536) */
 typedef struct colo_tune {
     uint32_t count_var, write_var, blocks_per_cu, item_rows;
+    /* count variant of level k (index k, 3..15); COLO_TUNE_UNSET = count_var */
+    uint32_t count_var_k[16];
 } colo_tune;
+#define COLO_TUNE_UNSET 0xffffffffu
 
 int colo_get_tune(const colo_ctx *ctx, colo_tune *out);
 int colo_set_tune(colo_ctx *ctx, const colo_tune *t);

@@ -51,7 +51,7 @@ colo::DevData colo_ctx::dev() const {
     DevData d;
     d.g = DevGrid{minX, minY, r, r2, gx, gy, cells, F};
     d.x = d_x; d.y = d_y; d.cell_of = d_cell_of; d.sidx = d_sidx; d.spos = d_spos;
-    d.sx = d_sx; d.sy = d_sy; d.scell = d_scell; d.dir = d_dir;
+    d.sxy = d_sxy; d.scell = d_scell; d.dir = d_dir;
     d.fstart = d_fstart; d.fcount = d_fcount;
     return d;
 }
@@ -400,13 +400,12 @@ static int radix_sort(colo_ctx *c, K *keys, uint32_t *vals, K *keys2, uint32_t *
 // Sorted-order gathers and the inverse permutation.
 template <typename K>
 __global__ void k_gather_sorted(const K *skeys, const uint32_t *sidx, uint32_t n, uint32_t cellbits,
-                                const float *x, const float *y, float *sx, float *sy, uint32_t *spos,
+                                const float *x, const float *y, float2 *sxy, uint32_t *spos,
                                 uint32_t *scell) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint32_t i = sidx[p];
-    sx[p] = x[i];
-    sy[p] = y[i];
+    sxy[p] = make_float2(x[i], y[i]);
     spos[i] = p;
     scell[p] = (uint32_t)(skeys[p] & (((K)1 << cellbits) - 1));
 }
@@ -812,7 +811,7 @@ __device__ inline uint32_t join2_count_anchor(const DevData &D, uint32_t a, uint
                                               uint32_t *bits_a, uint32_t *bits_b) {
     const uint32_t sa = D.fstart[a], sb = D.fstart[b];
     const uint32_t t = SORTED ? p - sa : D.sidx[p] - sa;
-    const float xa = D.sx[p], ya = D.sy[p];
+    const float xa = D.sxy[p].x, ya = D.sxy[p].y;
     const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
     RowSpan rs;
     int m = row_spans(D.g, dirb, D.scell[p], rs);
@@ -824,8 +823,8 @@ __device__ inline uint32_t join2_count_anchor(const DevData &D, uint32_t a, uint
             float qx[kJoinUnroll], qy[kJoinUnroll];
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++) {
-                qx[u] = D.sx[q + u];
-                qy[u] = D.sy[q + u];
+                qx[u] = D.sxy[q + u].x;
+                qy[u] = D.sxy[q + u].y;
             }
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++)
@@ -835,7 +834,7 @@ __device__ inline uint32_t join2_count_anchor(const DevData &D, uint32_t a, uint
                 }
         }
         for (; q < rs.hi[r]; q++) {
-            if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) {
+            if (sqdist(xa, ya, D.sxy[q].x, D.sxy[q].y) <= D.g.r2) {
                 cnt++;
                 acc.set(q - sb);
             }
@@ -864,7 +863,7 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
     BitAcc acc(bits + seg_off[km1]);
     for (int r = 0; r < nr; r++) {
         for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
-            const float qx = D.sx[q], qy = D.sy[q];
+            const float qx = D.sxy[q].x, qy = D.sxy[q].y;
             bool ok = true;
             for (uint32_t m = 0; m < km1; m++)
                 if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
@@ -889,7 +888,7 @@ template <bool SORTED = false>
 __device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t b, uint64_t t, uint32_t nrows,
                                           uint64_t off, uint32_t *table, uint32_t *nbr) {
     const uint32_t g = D.fstart[a] + (uint32_t)t;
-    const float xa = SORTED ? D.sx[g] : D.x[g], ya = SORTED ? D.sy[g] : D.y[g];
+    const float xa = SORTED ? D.sxy[g].x : D.x[g], ya = SORTED ? D.sxy[g].y : D.y[g];
     const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
     RowSpan rs;
     int m = row_spans(D.g, dirb, SORTED ? D.scell[g] : D.cell_of[g], rs);
@@ -910,15 +909,15 @@ __device__ inline void join2_write_anchor(const DevData &D, uint32_t a, uint32_t
             float qx[kJoinUnroll], qy[kJoinUnroll];
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++) {
-                qx[u] = D.sx[q + u];
-                qy[u] = D.sy[q + u];
+                qx[u] = D.sxy[q + u].x;
+                qy[u] = D.sxy[q + u].y;
             }
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++)
                 if (sqdist(xa, ya, qx[u], qy[u]) <= D.g.r2) emit(q + u);
         }
         for (; q < rs.hi[r] && w < nrows; q++)
-            if (sqdist(xa, ya, D.sx[q], D.sy[q]) <= D.g.r2) emit(q);
+            if (sqdist(xa, ya, D.sxy[q].x, D.sxy[q].y) <= D.g.r2) emit(q);
     }
 }
 
@@ -939,7 +938,7 @@ __device__ inline void joink_write_row(const DevData &D, const uint32_t *row, ui
     uint32_t w = 0;
     for (int r = 0; r < nr && w < nrows; r++) {
         for (uint32_t q = rs.lo[r]; q < rs.hi[r]; q++) {
-            const float qx = D.sx[q], qy = D.sy[q];
+            const float qx = D.sxy[q].x, qy = D.sxy[q].y;
             bool ok = true;
             for (uint32_t m = 0; m < km1; m++)
                 if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
@@ -975,7 +974,7 @@ __device__ inline void joink_count_row_nbr(const DevData &D, const uint32_t *row
     BitAcc acc(bits + seg_off[km1]);
     for (uint64_t i = b0; i < b1; i++) {
         const uint32_t q = nbr[i];  // sorted position
-        const float qx = D.sx[q], qy = D.sy[q];
+        const float qx = D.sxy[q].x, qy = D.sxy[q].y;
         bool ok = true;
         for (uint32_t m = 1; m < km1; m++)
             if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
@@ -1005,7 +1004,7 @@ __device__ inline void joink_write_row_nbr(const DevData &D, const uint32_t *row
     uint32_t w = 0;
     for (uint64_t i = b0; i < b1 && w < nrows; i++) {
         const uint32_t q = nbr[i];
-        const float qx = D.sx[q], qy = D.sy[q];
+        const float qx = D.sxy[q].x, qy = D.sxy[q].y;
         bool ok = true;
         for (uint32_t m = 1; m < km1; m++)
             if (sqdist(mx[m], my[m], qx, qy) > D.g.r2) { ok = false; break; }
@@ -1114,10 +1113,32 @@ __device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &i
     return item != ~0ull;
 }
 
+// Sets the participation bits of a row's members: every check load is
+// issued before the first atomic, so the members cost one round trip.
+template <int kArr>
+__device__ inline void set_member_bits(uint32_t *bt, const uint64_t *seg_off, const uint32_t *rk, uint32_t km1) {
+    uint32_t *w[kArr];
+    uint32_t cur[kArr];
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        w[m] = bt + seg_off[m] + (rk[m] >> 5);
+        cur[m] = __hip_atomic_load(w[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        const uint32_t msk = 1u << (rk[m] & 31);
+        if (!(cur[m] & msk)) atomicOr(w[m], msk);
+    }
+}
+
 // Level-k count over the rows of one parent table for ncand candidates that
-// extend it: the row and its members' coordinates are loaded once.  Tables
-// of the batched miner hold sorted positions; slots and participation bits
-// are indexed by sorted rank.
+// extend it: the row and its members' coordinates are loaded once, and the
+// members' ranks too (every candidate extends the same parent features).
+// Tables of the batched miner hold sorted positions; slots and participation
+// bits are indexed by sorted rank.  The next candidate's list bounds are
+// loaded before this candidate's stores, one step ahead.
 template <int KM1>
 __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
                                              uint32_t *slots, uint32_t *bits, unsigned long long *s_rows) {
@@ -1125,24 +1146,31 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
     const uint32_t *row = c0.parent + i * km1;
-    uint32_t ids[kArr];
+    uint32_t rk[kArr];
     float mx[kArr], my[kArr];
 #pragma unroll
     for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
         if (m >= km1) break;
-        ids[m] = row[m];
+        const uint32_t id = row[m];
+        rk[m] = id - D.fstart[c0.f[m]];
         if (m) {
-            mx[m] = D.sx[ids[m]];
-            my[m] = D.sy[ids[m]];
+            mx[m] = D.sxy[id].x;
+            my[m] = D.sxy[id].y;
         }
     }
-    const uint32_t t = ids[0] - D.fstart[c0.f[0]];
+    const uint32_t t = rk[0];
+    uint64_t nb0 = c0.nb_off[t] - c0.nb_base, nb1 = c0.nb_off[t + 1] - c0.nb_base;
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
+        const uint64_t b0 = nb0, b1 = nb1;
+        if (j + 1 < ncand) {
+            const CandDesc &cn = descs[j + 1];
+            nb0 = cn.nb_off[t] - cn.nb_base;
+            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+        }
         // descriptor fields in registers: the loop's bit stores could alias
         // the descriptor, which would otherwise be re-read every entry
         const gcu32 *nbt = as_global(cd.nb_table);
-        const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
         const uint32_t sl = D.fstart[cd.f[km1]];
         uint32_t *bt = bits + cd.bits_off;
         uint32_t cnt = 0;
@@ -1172,8 +1200,8 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
 #pragma unroll
                 for (int u = 0; u < kJoinUnroll; u++) {
                     q[u] = qn[u];
-                    qx[u] = D.sx[q[u]];
-                    qy[u] = D.sy[q[u]];
+                    qx[u] = D.sxy[q[u]].x;
+                    qy[u] = D.sxy[q[u]].y;
                 }
                 if (e + 2 * kJoinUnroll <= b1) {
 #pragma unroll
@@ -1185,17 +1213,14 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         }
         for (; e < b1; e++) {
             const uint32_t q = nbt[e];
-            test(q, D.sx[q], D.sy[q]);
+            test(q, D.sxy[q].x, D.sxy[q].y);
         }
         acc.flush();
         slots[cd.slots_off + i] = cnt;
-        if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
-        if (cnt)
-#pragma unroll
-            for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
-                if (m >= km1) break;
-                set_bit(bt + cd.seg_off[m], ids[m] - D.fstart[cd.f[m]]);
-            }
+        if (cnt) {
+            atomicAdd(&s_rows[j], (unsigned long long)cnt);
+            set_member_bits<kArr>(bt, cd.seg_off, rk, km1);
+        }
     }
 }
 
@@ -1209,25 +1234,32 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
     const uint32_t *row0 = c0.parent + i * km1, *row1 = row0 + km1;
-    uint32_t ids0[kArr], ids1[kArr];
+    uint32_t rk0[kArr], rk1[kArr];
     float mx0[kArr], my0[kArr], mx1[kArr], my1[kArr];
 #pragma unroll
     for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
         if (m >= km1) break;
-        ids0[m] = row0[m];
-        ids1[m] = row1[m];
+        const uint32_t id0 = row0[m], id1 = row1[m], fs = D.fstart[c0.f[m]];
+        rk0[m] = id0 - fs;
+        rk1[m] = id1 - fs;
         if (m) {
-            mx0[m] = D.sx[ids0[m]];
-            my0[m] = D.sy[ids0[m]];
-            mx1[m] = D.sx[ids1[m]];
-            my1[m] = D.sy[ids1[m]];
+            mx0[m] = D.sxy[id0].x;
+            my0[m] = D.sxy[id0].y;
+            mx1[m] = D.sxy[id1].x;
+            my1[m] = D.sxy[id1].y;
         }
     }
-    const uint32_t t = ids0[0] - D.fstart[c0.f[0]];
+    const uint32_t t = rk0[0];
+    uint64_t nb0 = c0.nb_off[t] - c0.nb_base, nb1 = c0.nb_off[t + 1] - c0.nb_base;
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
+        const uint64_t b0 = nb0, b1 = nb1;
+        if (j + 1 < ncand) {
+            const CandDesc &cn = descs[j + 1];
+            nb0 = cn.nb_off[t] - cn.nb_base;
+            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+        }
         const gcu32 *nbt = as_global(cd.nb_table);
-        const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
         const uint32_t sl = D.fstart[cd.f[km1]];
         uint32_t *bt = bits + cd.bits_off;
         uint32_t cnt0 = 0, cnt1 = 0;
@@ -1252,26 +1284,22 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
             for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++) {
-                qx[u] = D.sx[q[u]];
-                qy[u] = D.sy[q[u]];
+                qx[u] = D.sxy[q[u]].x;
+                qy[u] = D.sxy[q[u]].y;
             }
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
         }
         for (; e < b1; e++) {
             const uint32_t q = nbt[e];
-            test(q, D.sx[q], D.sy[q]);
+            test(q, D.sxy[q].x, D.sxy[q].y);
         }
         acc.flush();
         slots[cd.slots_off + i] = cnt0;
         slots[cd.slots_off + i + 1] = cnt1;
         if (cnt0 + cnt1) atomicAdd(&s_rows[j], (unsigned long long)(cnt0 + cnt1));
-#pragma unroll
-        for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
-            if (m >= km1) break;
-            if (cnt0) set_bit(bt + cd.seg_off[m], ids0[m] - D.fstart[cd.f[m]]);
-            if (cnt1) set_bit(bt + cd.seg_off[m], ids1[m] - D.fstart[cd.f[m]]);
-        }
+        if (cnt0) set_member_bits<kArr>(bt, cd.seg_off, rk0, km1);
+        if (cnt1) set_member_bits<kArr>(bt, cd.seg_off, rk1, km1);
     }
 }
 
@@ -1350,8 +1378,8 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                 float qx = 0.f, qy = 0.f;
                 if (act) {
                     q = cd.nb_table[b0 + e];
-                    qx = D.sx[q];
-                    qy = D.sy[q];
+                    qx = D.sxy[q].x;
+                    qy = D.sxy[q].y;
                 }
                 bool hit = false;
                 for (uint32_t s0 = 0; s0 < nrows; s0 += R) {
@@ -1362,7 +1390,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                     bool ok = valid;
                     for (uint32_t mm = 1; mm < km1 && ok; mm++) {
                         const uint32_t id = rp[mm];
-                        if (sqdist(D.sx[id], D.sy[id], qx, qy) > r2) ok = false;
+                        if (sqdist(D.sxy[id].x, D.sxy[id].y, qx, qy) > r2) ok = false;
                     }
                     const uint64_t bal = __ballot(ok);
                     if (!WRITE) {
@@ -1407,8 +1435,8 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                 hr[c] = false;
                 if (e < m) {
                     qr[c] = cd.nb_table[b0 + e];
-                    xr[c] = D.sx[qr[c]];
-                    yr[c] = D.sy[qr[c]];
+                    xr[c] = D.sxy[qr[c]].x;
+                    yr[c] = D.sxy[qr[c]].y;
                 }
             }
             for (uint32_t rl = 0; rl < nrows; rl++) {
@@ -1426,8 +1454,8 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                 for (uint32_t mm = 1; mm < (uint32_t)kArr; mm++) {
                     if (mm >= km1) break;
                     const uint32_t id = rp[mm];
-                    mx[mm] = D.sx[id];
-                    my[mm] = D.sy[id];
+                    mx[mm] = D.sxy[id].x;
+                    my[mm] = D.sxy[id].y;
                 }
                 uint32_t cnt = 0;
                 auto step = [&](uint32_t e, uint32_t q, float qx, float qy, bool &h) {
@@ -1459,8 +1487,8 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                     float qx = 0.f, qy = 0.f;
                     if (e < m) {
                         q = cd.nb_table[b0 + e];
-                        qx = D.sx[q];
-                        qy = D.sy[q];
+                        qx = D.sxy[q].x;
+                        qy = D.sxy[q].y;
                     }
                     bool h = false;
                     step(e, q, qx, qy, h);
@@ -1513,24 +1541,27 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // (wave_rows_multi) over 64-row chunks taken by the block's waves in turn.
 // Rows per candidate are summed in LDS over the item and added to the
 // candidates' totals.
+// VAR 4 and 5 run the code of VAR 0 and 3 with the registers capped at 64
+// per lane, for 8 waves per SIMD (the compiler spills what does not fit).
 template <int KM1, int VAR>
-__global__ __launch_bounds__(kBlock) void k_countk_batch(DevData D, const CandDesc *desc, const WorkItem *items,
-                                                         const uint64_t *seg, uint32_t *slots, uint32_t *bits,
-                                                         unsigned long long *rows, uint32_t *queue) {
+__global__ __launch_bounds__(kBlock, VAR >= 4 ? 8 : 1) void k_countk_batch(DevData D, const CandDesc *desc,
+                                                                           const WorkItem *items, const uint64_t *seg,
+                                                                           uint32_t *slots, uint32_t *bits,
+                                                                           unsigned long long *rows, uint32_t *queue) {
     __shared__ unsigned long long s_rows[kMaxRun];
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         if (threadIdx.x < kMaxRun) s_rows[threadIdx.x] = 0;
         __syncthreads();
-        if (VAR == 0) {
+        if (VAR == 0 || VAR == 4) {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                 joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows);
         } else if (VAR == 2) {
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                     joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits, s_rows + j);
-        } else if (VAR == 3) {
+        } else if (VAR == 3 || VAR == 5) {
             // two adjacent rows per thread, one list pass when they share the anchor
             const CandDesc &c0 = desc[wi.cand];
             const uint32_t km1 = c0.k - 1;
@@ -1573,8 +1604,8 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
         if (m >= km1) break;
         ids[m] = row[m];
         if (m) {
-            mx[m] = D.sx[ids[m]];
-            my[m] = D.sy[ids[m]];
+            mx[m] = D.sxy[ids[m]].x;
+            my[m] = D.sxy[ids[m]].y;
         }
     }
     const uint32_t t = ids[0] - D.fstart[c0.f[0]];
@@ -1613,15 +1644,15 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
             for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++) {
-                qx[u] = D.sx[q[u]];
-                qy[u] = D.sy[q[u]];
+                qx[u] = D.sxy[q[u]].x;
+                qy[u] = D.sxy[q[u]].y;
             }
 #pragma unroll
             for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
         }
         for (; e < b1 && w < nrows; e++) {
             const uint32_t q = nbt[e];
-            test(q, D.sx[q], D.sy[q]);
+            test(q, D.sxy[q].x, D.sxy[q].y);
         }
     }
 }
@@ -1687,6 +1718,8 @@ template <int KM1> struct PickCount0 { static auto get() { return &k_countk_batc
 template <int KM1> struct PickCount1 { static auto get() { return &k_countk_batch<KM1, 1>; } };
 template <int KM1> struct PickCount2 { static auto get() { return &k_countk_batch<KM1, 2>; } };
 template <int KM1> struct PickCount3 { static auto get() { return &k_countk_batch<KM1, 3>; } };
+template <int KM1> struct PickCount4 { static auto get() { return &k_countk_batch<KM1, 4>; } };
+template <int KM1> struct PickCount5 { static auto get() { return &k_countk_batch<KM1, 5>; } };
 template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
 template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
 template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
@@ -1702,7 +1735,9 @@ int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
                            d_items, d_seg, d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
     else {
         const uint32_t var = c->tune.count_var_of(k);
-        hipLaunchKernelGGL(var == 3   ? pick_k<PickCount3>(k)
+        hipLaunchKernelGGL(var == 5   ? pick_k<PickCount5>(k)
+                           : var == 4 ? pick_k<PickCount4>(k)
+                           : var == 3 ? pick_k<PickCount3>(k)
                            : var == 2 ? pick_k<PickCount2>(k)
                            : var == 1 ? pick_k<PickCount1>(k)
                                       : pick_k<PickCount0>(k),
@@ -1895,7 +1930,7 @@ static int bin_launch(colo_ctx *c, hipStream_t s) {
         if ((rc = radix_sort<K>(c, keys, c->d_sidx, keys2, vals2, n, bits, hist, stmp, &sk, &sv, s))) return rc;
         if (sv != c->d_sidx) COLO_HIP(hipMemcpyAsync(c->d_sidx, sv, 4ull * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL((k_gather_sorted<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk,
-                           c->d_sidx, n, c->cellbits, c->d_x, c->d_y, c->d_sx, c->d_sy, c->d_spos, c->d_scell);
+                           c->d_sidx, n, c->cellbits, c->d_x, c->d_y, c->d_sxy, c->d_spos, c->d_scell);
     }
     // directory: counts per (f, cell) then exclusive scan in place
     if ((rc = fill_u32(c->d_dir, c->dir_len, 0, s))) return rc;
@@ -2068,8 +2103,7 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
         const uint64_t n = c->n;
         if (!rc) rc = alloc_dev(c, (void **)&c->d_x, 4 * n);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_y, 4 * n);
-        if (!rc) rc = alloc_dev(c, (void **)&c->d_sx, 4 * n);
-        if (!rc) rc = alloc_dev(c, (void **)&c->d_sy, 4 * n);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_sxy, 8 * n);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_cell_of, 4 * n);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_sidx, 4 * n);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_spos, 4 * n);
@@ -2099,7 +2133,7 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
 int colo_ctx_destroy(colo_ctx *c) {
     if (!c) return COLO_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *ptrs[] = {c->d_x, c->d_y, c->d_sx, c->d_sy, c->d_cell_of, c->d_sidx, c->d_spos, c->d_scell, c->d_dir,
+    void *ptrs[] = {c->d_x, c->d_y, c->d_sxy, c->d_cell_of, c->d_sidx, c->d_spos, c->d_scell, c->d_dir,
                     c->d_fstart, c->d_fcount, c->d_cellmask, c->d_qmask, c->d_run_cell, c->d_flag, c->d_scratch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -2138,9 +2172,9 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
 int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
     for (int k = 0; k < 16; k++)
-        if (t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 3)
+        if (t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 5)
             return set_error(c, COLO_EARG, "launch shape out of range");
-    if (t->count_var > 3 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+    if (t->count_var > 5 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;

@@ -33,7 +33,7 @@ struct DevData {
     const uint32_t *cell_of;       // input order -> cell
     const uint32_t *sidx;          // sorted position -> input index
     const uint32_t *spos;          // input index -> sorted position
-    const float *sx, *sy;          // coordinates in sorted order
+    const float2 *sxy;             // coordinates in sorted order, interleaved
     const uint32_t *scell;         // cell in sorted order
     const uint32_t *dir;           // F x (cells+1): first sorted position of (f, c)
     const uint32_t *fstart;        // F
@@ -199,7 +199,8 @@ struct colo_ctx {
     uint32_t gx = 0, gy = 0, cells = 0;
     uint32_t cellbits = 0, fidbits = 0;
     // device buffers
-    float *d_x = nullptr, *d_y = nullptr, *d_sx = nullptr, *d_sy = nullptr;
+    float *d_x = nullptr, *d_y = nullptr;
+    float2 *d_sxy = nullptr;
     uint32_t *d_cell_of = nullptr, *d_sidx = nullptr, *d_spos = nullptr, *d_scell = nullptr;
     uint32_t *d_dir = nullptr;
     uint64_t dir_len = 0;

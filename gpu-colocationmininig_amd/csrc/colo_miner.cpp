@@ -1547,36 +1547,37 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             c->tune = best;
             return rc;
         }
-        // count variant per level: every variant on every level, the fastest kept
-        {
-            const uint32_t vars[] = {0, 1, 2, 3};
-            double lv[4][16];
-            for (int vi = 0; vi < 4; vi++) {
+        // count variant per level: every variant on every level, the fastest
+        // kept; chosen before and again after the other parameters, whose
+        // best values can move the per-level optimum
+        auto pick_levels = [&]() -> int {
+            const uint32_t vars[] = {0, 1, 2, 3, 4, 5};
+            constexpr int nv = 6;
+            double lv[nv][16];
+            for (int vi = 0; vi < nv; vi++) {
                 TuneParams t = best;
                 t.count_var = vars[vi];
                 for (int k = 0; k < 16; k++) t.count_var_k[k] = COLO_TUNE_UNSET;
-                if ((rc = measure(t, m, lv[vi]))) {
-                    c->tune = best;
-                    return rc;
-                }
+                if (int r = measure(t, m, lv[vi])) return r;
             }
-            double tot[4] = {0, 0, 0, 0};
-            for (int vi = 0; vi < 4; vi++)
+            double tot[nv] = {};
+            for (int vi = 0; vi < nv; vi++)
                 for (int k = 3; k < 16; k++) tot[vi] += lv[vi][k];
             int bv = 0;
-            for (int vi = 1; vi < 4; vi++)
+            for (int vi = 1; vi < nv; vi++)
                 if (tot[vi] < tot[bv]) bv = vi;
             best.count_var = vars[bv];
             for (int k = 0; k < 16; k++) {
                 best.count_var_k[k] = COLO_TUNE_UNSET;
                 if (k < 3 || lv[0][k] <= 0) continue;
                 int kv = 0;
-                for (int vi = 1; vi < 4; vi++)
+                for (int vi = 1; vi < nv; vi++)
                     if (lv[vi][k] < lv[kv][k]) kv = vi;
                 best.count_var_k[k] = vars[kv];
             }
-        }
-        if ((rc = measure(best, cur, nullptr))) {
+            return measure(best, cur, nullptr);
+        };
+        if ((rc = pick_levels())) {
             c->tune = best;
             return rc;
         }
@@ -1603,6 +1604,10 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
                     memcpy(cur, m, sizeof cur);
                 }
             }
+        }
+        if ((rc = pick_levels())) {
+            c->tune = best;
+            return rc;
         }
         if (cache_path) {
             FILE *fp = fopen(cache_path, "ab");

@@ -246,11 +246,13 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         lanes); 2 = one parent row per thread, the work
  *                         item's candidates one after another; 3 (count
  *                         only) = two adjacent rows per thread, their list
- *                         read once when they share the anchor
+ *                         read once when they share the anchor; 4 and 5
+ *                         (count only) = 0 and 3 with registers capped for
+ *                         8 waves per SIMD
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between
- *                         item_rows / 4 and 4 x item_rows (64..65536) */
+ *                         item_rows / 4 and 8 x item_rows (64..65536) */
 typedef struct colo_tune {
     uint32_t count_var, write_var, blocks_per_cu, item_rows;
     /* count variant of level k (index k, 3..15); COLO_TUNE_UNSET = count_var */

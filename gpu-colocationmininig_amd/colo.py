@@ -51,11 +51,12 @@ TUNE_UNSET = 0xFFFFFFFF
 
 class Tune(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("count_var", "write_var", "blocks_per_cu", "item_rows")] + \
-               [("count_var_k", C.c_uint32 * 16)]
+               [("count_var_k", C.c_uint32 * 16), ("item_rows_k", C.c_uint32 * 16)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
         d["count_var_k"] = list(self.count_var_k)
+        d["item_rows_k"] = list(self.item_rows_k)
         return d
 
 
@@ -275,15 +276,17 @@ class Context:
 
     @tune.setter
     def tune(self, d):
-        """Sets launch shapes; a count_var without count_var_k applies to
-        every level."""
+        """Sets launch shapes; a count_var without count_var_k (an
+        item_rows without item_rows_k) applies to every level."""
         cur = self.tune
         cur.update(d)
-        if "count_var" in d and "count_var_k" not in d:
-            cur["count_var_k"] = [TUNE_UNSET] * 16
-        t = Tune(**{k: v for k, v in cur.items() if k != "count_var_k"})
-        for i, v in enumerate(cur["count_var_k"]):
-            t.count_var_k[i] = v
+        for one, per in (("count_var", "count_var_k"), ("item_rows", "item_rows_k")):
+            if one in d and per not in d:
+                cur[per] = [TUNE_UNSET] * 16
+        t = Tune(**{k: v for k, v in cur.items() if not k.endswith("_k")})
+        for i in range(16):
+            t.count_var_k[i] = cur["count_var_k"][i]
+            t.item_rows_k[i] = cur["item_rows_k"][i]
         self._check(lib().colo_set_tune(self.h, C.byref(t)))
 
     @_on_stream

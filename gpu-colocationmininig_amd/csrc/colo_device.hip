@@ -2162,7 +2162,10 @@ int colo_graph_info(const colo_ctx *c, uint32_t *captures, uint32_t *replays) {
 int colo_get_tune(const colo_ctx *c, colo_tune *o) {
     if (!c || !o) return COLO_EARG;
     o->count_var = c->tune.count_var;
-    for (int k = 0; k < 16; k++) o->count_var_k[k] = c->tune.count_var_k[k];
+    for (int k = 0; k < 16; k++) {
+        o->count_var_k[k] = c->tune.count_var_k[k];
+        o->item_rows_k[k] = c->tune.item_rows_k[k];
+    }
     o->write_var = c->tune.write_var;
     o->blocks_per_cu = c->tune.blocks_per_cu;
     o->item_rows = c->tune.item_rows;
@@ -2172,13 +2175,17 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
 int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
     for (int k = 0; k < 16; k++)
-        if (t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 5)
+        if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 5) ||
+            (t->item_rows_k[k] != COLO_TUNE_UNSET && (t->item_rows_k[k] < 64 || t->item_rows_k[k] > 65536)))
             return set_error(c, COLO_EARG, "launch shape out of range");
     if (t->count_var > 5 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;
-    for (int k = 0; k < 16; k++) c->tune.count_var_k[k] = t->count_var_k[k];
+    for (int k = 0; k < 16; k++) {
+        c->tune.count_var_k[k] = t->count_var_k[k];
+        c->tune.item_rows_k[k] = t->item_rows_k[k];
+    }
     c->tune.write_var = t->write_var;
     c->tune.blocks_per_cu = t->blocks_per_cu;
     c->tune.item_rows = t->item_rows;

@@ -543,7 +543,7 @@ struct Miner {
             // about 8 items per workgroup at equal predicted cost, each
             // between 1/4 and 8x the tuned item size
             double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 8));
-            const double lo = std::max(64.0, c->tune.item_rows / 4.0), hi = c->tune.item_rows * 8.0;
+            const double lo = std::max(64.0, c->tune.item_rows_of(k) / 4.0), hi = c->tune.item_rows_of(k) * 8.0;
             for (auto &r : runs) {
                 uint64_t n = desc[r.first].n_anchor;
                 double per_row = n ? cost[r.first] / (double)n : 1.0;
@@ -553,7 +553,7 @@ struct Miner {
                 for (size_t i = before; i < items.size(); i++) iw.push_back(per_row * (items[i].end - items[i].begin));
             }
         } else {
-            for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows, items, r.second);
+            for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows_of(k), items, r.second);
         }
         slots_arena = std::make_shared<DevBuf>(&c->pool);
         if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return oom("slots arena", std::max<uint64_t>(slots_tot, 1) * 4);
@@ -720,7 +720,8 @@ struct Miner {
             if (k == 2) st.l2_rows += p.rows;
         }
         if (desc.empty()) return COLO_OK;
-        for (auto &r : parent_runs(desc, k)) make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows, items, r.second);
+        for (auto &r : parent_runs(desc, k))
+            make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows_of(k), items, r.second);
         if (!d_desc.ensure(desc.size() * sizeof(CandDesc)) || !d_queue.ensure(64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
@@ -1480,37 +1481,41 @@ static std::string tune_key(const colo_ctx *c, const colo_mine_opts &o) {
     for (char &ch : arch)
         if (ch == ' ') ch = '_';
     char b[256];
-    snprintf(b, sizeof b, "v2/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
+    snprintf(b, sizeof b, "v3/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
              fbits(o.theta), o.max_k, o.sched);
     return b;
 }
 
-// Cache lines: key, count_var, the 16 per-level count variants, write_var,
-// blocks_per_cu, item_rows.
+// Cache lines: key, count_var, write_var, blocks_per_cu, item_rows, the 16
+// per-level count variants, the 16 per-level item sizes.
 static bool tune_load(const char *path, const std::string &key, TuneParams *t) {
     FILE *fp = fopen(path, "rb");
     if (!fp) return false;
-    char line[1024];
+    char line[2048];
     bool found = false;
     while (fgets(line, sizeof line, fp)) {
         char k[256];
         int used = 0;
         if (sscanf(line, "%255s%n", k, &used) != 1 || key != k) continue;
-        unsigned long v[20];
+        constexpr int nf = 4 + 16 + 16;
+        unsigned long v[nf];
         const char *p = line + used;
         int n = 0;
-        for (; n < 20; n++) {
+        for (; n < nf; n++) {
             char *e;
             v[n] = strtoul(p, &e, 10);
             if (e == p) break;
             p = e;
         }
-        if (n != 20) continue;
+        if (n != nf) continue;
         t->count_var = (uint32_t)v[0];  // the last entry of a key wins
-        for (int i = 0; i < 16; i++) t->count_var_k[i] = (uint32_t)v[1 + i];
-        t->write_var = (uint32_t)v[17];
-        t->blocks_per_cu = (uint32_t)v[18];
-        t->item_rows = (uint32_t)v[19];
+        t->write_var = (uint32_t)v[1];
+        t->blocks_per_cu = (uint32_t)v[2];
+        t->item_rows = (uint32_t)v[3];
+        for (int i = 0; i < 16; i++) {
+            t->count_var_k[i] = (uint32_t)v[4 + i];
+            t->item_rows_k[i] = (uint32_t)v[20 + i];
+        }
         found = true;
     }
     fclose(fp);
@@ -1528,7 +1533,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
     TuneParams best = c->tune;
     if (!(cache_path && tune_load(cache_path, key, &best))) {
         // kernel time of one run: {level-k count, level-k write, all batched
-        // passes}; lvl: count-pass kernel time per level
+        // passes}; lvl: wall time per level
         auto measure = [&](const TuneParams &t, double *m3, double *lvl) -> int {
             c->tune = t;
             Miner M(c, o);
@@ -1538,83 +1543,75 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             m3[0] = M.st.count_kernel_ms - M.st.l2_count_kernel_ms;
             m3[1] = M.st.write_kernel_ms - M.st.l2_write_kernel_ms;
             m3[2] = M.st.count_kernel_ms + M.st.write_kernel_ms;
-            if (lvl) memcpy(lvl, M.st.level_count_kernel_ms, sizeof M.st.level_count_kernel_ms);
+            if (lvl) memcpy(lvl, M.st.level_ms, sizeof M.st.level_ms);
             return COLO_OK;
         };
+        auto fail = [&](int rc) {
+            c->tune = best;
+            return rc;
+        };
+        for (int k = 0; k < 16; k++) best.count_var_k[k] = best.item_rows_k[k] = COLO_TUNE_UNSET;
         double cur[3], m[3];
         int rc = measure(best, cur, nullptr);  // warm-up (pool, cached features)
-        if (rc) {
-            c->tune = best;
-            return rc;
-        }
-        // count variant per level: every variant on every level, the fastest
-        // kept; chosen before and again after the other parameters, whose
-        // best values can move the per-level optimum
-        auto pick_levels = [&]() -> int {
-            const uint32_t vars[] = {0, 1, 2, 3, 4, 5};
-            constexpr int nv = 6;
-            double lv[nv][16];
-            for (int vi = 0; vi < nv; vi++) {
-                TuneParams t = best;
-                t.count_var = vars[vi];
-                for (int k = 0; k < 16; k++) t.count_var_k[k] = COLO_TUNE_UNSET;
-                if (int r = measure(t, m, lv[vi])) return r;
-            }
-            double tot[nv] = {};
-            for (int vi = 0; vi < nv; vi++)
-                for (int k = 3; k < 16; k++) tot[vi] += lv[vi][k];
-            int bv = 0;
-            for (int vi = 1; vi < nv; vi++)
-                if (tot[vi] < tot[bv]) bv = vi;
-            best.count_var = vars[bv];
-            for (int k = 0; k < 16; k++) {
-                best.count_var_k[k] = COLO_TUNE_UNSET;
-                if (k < 3 || lv[0][k] <= 0) continue;
-                int kv = 0;
-                for (int vi = 1; vi < nv; vi++)
-                    if (lv[vi][k] < lv[kv][k]) kv = vi;
-                best.count_var_k[k] = vars[kv];
-            }
-            return measure(best, cur, nullptr);
-        };
-        if ((rc = pick_levels())) {
-            c->tune = best;
-            return rc;
-        }
+        if (!rc) rc = measure(best, cur, nullptr);
+        if (rc) return fail(rc);
+        // whole-run parameters, one at a time
         struct Axis {
             uint32_t TuneParams::*field;
             std::vector<uint32_t> values;
             int metric;
         };
-        std::vector<Axis> axes = {{&TuneParams::write_var, {0, 1, 2}, 1},
-                                  {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
-        axes.push_back({&TuneParams::item_rows, {1024, 4096, 16384}, 2});
+        const std::vector<Axis> axes = {{&TuneParams::write_var, {0, 1, 2}, 1},
+                                        {&TuneParams::blocks_per_cu, {4, 8, 16}, 2},
+                                        {&TuneParams::item_rows, {1024, 4096, 16384}, 2}};
         for (const Axis &ax : axes) {
             const uint32_t start = best.*(ax.field);
             for (uint32_t v : ax.values) {
                 if (v == start) continue;
                 TuneParams t = best;
                 t.*(ax.field) = v;
-                if ((rc = measure(t, m, nullptr))) {
-                    c->tune = best;
-                    return rc;
-                }
+                if ((rc = measure(t, m, nullptr))) return fail(rc);
                 if (m[ax.metric] < cur[ax.metric]) {
                     best = t;
                     memcpy(cur, m, sizeof cur);
                 }
             }
         }
-        if ((rc = pick_levels())) {
-            c->tune = best;
-            return rc;
+        // per level: every (count variant, item size) pair, judged by the
+        // level's wall time (count, scan and write of that level)
+        const uint32_t vars[] = {0, 1, 2, 3, 4, 5}, sizes[] = {1024, 4096, 16384};
+        constexpr int nv = 6, ns = 3;
+        double lv[nv][ns][16], tot[nv] = {};
+        for (int si = 0; si < ns; si++)
+            for (int vi = 0; vi < nv; vi++) {
+                TuneParams t = best;
+                t.count_var = vars[vi];
+                t.item_rows = sizes[si];
+                if ((rc = measure(t, m, lv[vi][si]))) return fail(rc);
+                if (sizes[si] == best.item_rows)
+                    for (int k = 3; k < 16; k++) tot[vi] += lv[vi][si][k];
+            }
+        int bv = 0;
+        for (int vi = 1; vi < nv; vi++)
+            if (tot[vi] < tot[bv]) bv = vi;
+        best.count_var = vars[bv];  // levels not seen here
+        for (int k = 2; k < 16; k++) {
+            if (lv[0][0][k] <= 0) continue;
+            int kv = 0, ks = 0;
+            for (int si = 0; si < ns; si++)
+                for (int vi = 0; vi < nv; vi++)
+                    if (lv[vi][si][k] < lv[kv][ks][k]) kv = vi, ks = si;
+            if (k >= 3) best.count_var_k[k] = vars[kv];
+            best.item_rows_k[k] = sizes[ks];
         }
         if (cache_path) {
             FILE *fp = fopen(cache_path, "ab");
             if (fp) {
-                fprintf(fp, "%s %u", key.c_str(), best.count_var);
+                fprintf(fp, "%s %u %u %u %u", key.c_str(), best.count_var, best.write_var, best.blocks_per_cu,
+                        best.item_rows);
                 for (int k = 0; k < 16; k++) fprintf(fp, " %u", best.count_var_k[k]);
-                fprintf(fp, " %u %u %u\n", best.write_var, best.blocks_per_cu, best.item_rows);
+                for (int k = 0; k < 16; k++) fprintf(fp, " %u", best.item_rows_k[k]);
+                fprintf(fp, "\n");
                 fclose(fp);
             }
         }

@@ -257,6 +257,8 @@ typedef struct colo_tune {
     uint32_t count_var, write_var, blocks_per_cu, item_rows;
     /* count variant of level k (index k, 3..15); COLO_TUNE_UNSET = count_var */
     uint32_t count_var_k[16];
+    /* item rows of level k (index k, 2..15); COLO_TUNE_UNSET = item_rows */
+    uint32_t item_rows_k[16];
 } colo_tune;
 #define COLO_TUNE_UNSET 0xffffffffu
 
@@ -265,7 +267,8 @@ int colo_set_tune(colo_ctx *ctx, const colo_tune *t);
 /* Autotuner: mines the context's data with opts under candidate launch
  * shapes, one parameter at a time (coordinate descent), and keeps the one
  * with the least HIP-event kernel time in the passes the parameter
- * affects; the context then uses the result.  With cache_path, results are
+ * affects; the count variant and the item size are chosen per level, by
+ * each level's time.  The context then uses the result.  With cache_path, results are
  * stored under a key of device and workload (instances, features, r, theta,
  * max_k, schedule) and a cached entry is taken without measuring.  Every
  * shape gives the same results; only time differs. */

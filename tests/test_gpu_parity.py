@@ -285,7 +285,8 @@ def test_threshold_sweep(c2small, colo_mod, sched):
                                   dict(count_var=1, write_var=0, blocks_per_cu=16, item_rows=4096),
                                   dict(count_var=2, write_var=2), dict(count_var=3, write_var=1),
                                   dict(count_var=0, count_var_k=[0xFFFFFFFF] * 3 + [3, 1, 2] + [0xFFFFFFFF] * 10),
-                                  dict(count_var=4, count_var_k=[0xFFFFFFFF] * 3 + [5, 4, 5] + [0xFFFFFFFF] * 10)])
+                                  dict(count_var=4, count_var_k=[0xFFFFFFFF] * 3 + [5, 4, 5] + [0xFFFFFFFF] * 10),
+                                  dict(item_rows=1024, item_rows_k=[0xFFFFFFFF] * 2 + [64, 256, 16384] + [0xFFFFFFFF] * 11)])
 def test_launch_shapes_identical(c2small, tune):
     """Every launch shape of the batched passes gives the oracle's report."""
     ctx, orc, p = c2small
@@ -311,6 +312,7 @@ def test_autotune_and_cache(c2small, tmp_path):
         t1 = ctx.autotune(0.15, max_k=4, cache=cache)
         assert os.path.getsize(cache) > 0
         assert t1["count_var_k"][3] <= 5 and all(v <= 5 or v == 0xFFFFFFFF for v in t1["count_var_k"]), t1
+        assert t1["item_rows_k"][2] in (1024, 4096, 16384), t1
         ctx.tune = old
         t2 = ctx.autotune(0.15, max_k=4, cache=cache)
         assert t1 == t2 and ctx.tune == t1

@@ -1113,6 +1113,50 @@ __device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &i
     return item != ~0ull;
 }
 
+// Segmented OR over the lanes of a wave holding the same 32-bit word w:
+// the first lane of each run of equal words gets the OR of the run's masks,
+// every other lane 0.  Lanes of one word in separate runs each lead their
+// own run (still correct, one atomic more).  All 64 lanes must take part;
+// lanes without a bit pass msk 0.
+__device__ inline uint32_t wave_word_or(uint32_t w, uint32_t msk) {
+    const uint32_t lane = __lane_id();
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t ow = __shfl_down(w, off, 64);
+        const uint32_t om = __shfl_down(msk, off, 64);
+        if (lane + off < 64 && ow == w) msk |= om;
+    }
+    const uint32_t pw = __shfl_up(w, 1, 64);
+    return (lane == 0 || pw != w) ? msk : 0u;
+}
+
+// Participation bits of the members of the wave's rows for one candidate,
+// combined over the wave: adjacent rows share their anchor and mostly their
+// members' 32-bit words, so one check and one atomic per distinct word
+// replace one per row.  All 64 lanes must take part (hit false for lanes
+// without a row or without an extension).
+template <int kArr>
+__device__ inline void wave_member_bits(uint32_t *bt, const uint64_t *seg_off, const uint32_t *rk, uint32_t km1,
+                                        bool hit) {
+    uint32_t *w[kArr];
+    uint32_t lm[kArr], cur[kArr];
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        lm[m] = wave_word_or(hit ? rk[m] >> 5 : 0xffffffffu, hit ? 1u << (rk[m] & 31) : 0u);
+        w[m] = bt + seg_off[m] + (rk[m] >> 5);
+    }
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        cur[m] = lm[m] ? __hip_atomic_load(w[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0u;
+    }
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        if ((cur[m] & lm[m]) != lm[m]) atomicOr(w[m], lm[m]);
+    }
+}
+
 // Sets the participation bits of a row's members: every check load is
 // issued before the first atomic, so the members cost one round trip.
 template <int kArr>
@@ -1138,10 +1182,13 @@ __device__ inline void set_member_bits(uint32_t *bt, const uint64_t *seg_off, co
 // members' ranks too (every candidate extends the same parent features).
 // Tables of the batched miner hold sorted positions; slots and participation
 // bits are indexed by sorted rank.  The next candidate's list bounds are
-// loaded before this candidate's stores, one step ahead.
-template <int KM1>
+// loaded before this candidate's stores, one step ahead.  WAVE: all 64
+// lanes of the wave call together (act false for lanes without a row) and
+// the members' bits are combined over the wave.
+template <int KM1, bool WAVE = false>
 __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
-                                             uint32_t *slots, uint32_t *bits, unsigned long long *s_rows) {
+                                             uint32_t *slots, uint32_t *bits, unsigned long long *s_rows,
+                                             bool act = true) {
     const CandDesc &c0 = descs[0];
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
@@ -1151,19 +1198,27 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
 #pragma unroll
     for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
         if (m >= km1) break;
-        const uint32_t id = row[m];
-        rk[m] = id - D.fstart[c0.f[m]];
-        if (m) {
-            mx[m] = D.sxy[id].x;
-            my[m] = D.sxy[id].y;
+        rk[m] = 0;
+        mx[m] = my[m] = 0.f;
+        if (act) {
+            const uint32_t id = row[m];
+            rk[m] = id - D.fstart[c0.f[m]];
+            if (m) {
+                mx[m] = D.sxy[id].x;
+                my[m] = D.sxy[id].y;
+            }
         }
     }
     const uint32_t t = rk[0];
-    uint64_t nb0 = c0.nb_off[t] - c0.nb_base, nb1 = c0.nb_off[t + 1] - c0.nb_base;
+    uint64_t nb0 = 0, nb1 = 0;
+    if (act) {
+        nb0 = c0.nb_off[t] - c0.nb_base;
+        nb1 = c0.nb_off[t + 1] - c0.nb_base;
+    }
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
         const uint64_t b0 = nb0, b1 = nb1;
-        if (j + 1 < ncand) {
+        if (act && j + 1 < ncand) {
             const CandDesc &cn = descs[j + 1];
             nb0 = cn.nb_off[t] - cn.nb_base;
             nb1 = cn.nb_off[t + 1] - cn.nb_base;
@@ -1216,11 +1271,12 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
             test(q, D.sxy[q].x, D.sxy[q].y);
         }
         acc.flush();
-        slots[cd.slots_off + i] = cnt;
-        if (cnt) {
-            atomicAdd(&s_rows[j], (unsigned long long)cnt);
+        if (act) slots[cd.slots_off + i] = cnt;
+        if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
+        if (WAVE)
+            wave_member_bits<kArr>(bt, cd.seg_off, rk, km1, cnt != 0);
+        else if (cnt)
             set_member_bits<kArr>(bt, cd.seg_off, rk, km1);
-        }
     }
 }
 
@@ -1555,8 +1611,12 @@ __global__ __launch_bounds__(kBlock, VAR >= 4 ? 8 : 1) void k_countk_batch(DevDa
         if (threadIdx.x < kMaxRun) s_rows[threadIdx.x] = 0;
         __syncthreads();
         if (VAR == 0 || VAR == 4) {
-            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows);
+            // waves stay converged over the rows so the member bits can be
+            // combined over the wave
+            for (uint64_t i0 = wi.begin + (threadIdx.x & ~63u); i0 < wi.end; i0 += kBlock) {
+                const uint64_t i = i0 + __lane_id();
+                joink_count_row_multi<KM1, true>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows, i < wi.end);
+            }
         } else if (VAR == 2) {
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)

@@ -248,7 +248,7 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         only) = two adjacent rows per thread, their list
  *                         read once when they share the anchor; 4 and 5
  *                         (count only) = 0 and 3 with registers capped for
- *                         8 waves per SIMD
+ *                         8 waves per SIMD; the autotuner tries 0 and 3-5
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between

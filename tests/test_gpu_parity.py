@@ -286,7 +286,8 @@ def test_threshold_sweep(c2small, colo_mod, sched):
                                   dict(count_var=2, write_var=2), dict(count_var=3, write_var=1),
                                   dict(count_var=0, count_var_k=[0xFFFFFFFF] * 3 + [3, 1, 2] + [0xFFFFFFFF] * 10),
                                   dict(count_var=4, count_var_k=[0xFFFFFFFF] * 3 + [5, 4, 5] + [0xFFFFFFFF] * 10),
-                                  dict(item_rows=1024, item_rows_k=[0xFFFFFFFF] * 2 + [64, 256, 16384] + [0xFFFFFFFF] * 11)])
+                                  dict(item_rows=1024, item_rows_k=[0xFFFFFFFF] * 2 + [64, 256, 16384] + [0xFFFFFFFF] * 11),
+                                  dict(count_var=4, item_rows=256), dict(count_var=0, item_rows=64)])
 def test_launch_shapes_identical(c2small, tune):
     """Every launch shape of the batched passes gives the oracle's report."""
     ctx, orc, p = c2small

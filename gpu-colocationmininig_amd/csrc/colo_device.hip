@@ -1113,27 +1113,35 @@ __device__ inline bool next_item(uint32_t *ctr, const uint64_t *seg, uint64_t &i
     return item != ~0ull;
 }
 
-// Segmented OR over the lanes of a wave holding the same 32-bit word w:
-// the first lane of each run of equal words gets the OR of the run's masks,
-// every other lane 0.  Lanes of one word in separate runs each lead their
-// own run (still correct, one atomic more).  All 64 lanes must take part;
-// lanes without a bit pass msk 0.
+// Segmented OR over the lanes of a wave holding the same 32-bit word w, in
+// runs of at most CAP lanes (CAP a power of two, log2(CAP) shuffle steps):
+// the first lane of each run of equal words, and every CAP-th lane inside
+// a run, gets the OR of the masks of the next CAP lanes of its word; every
+// other lane 0.  Every lane's mask reaches a leader: leaders are at most CAP
+// lanes apart inside a run.  Lanes of one word in separate runs each lead
+// their own run (still correct, one atomic more).  All 64 lanes must take
+// part; lanes without a bit pass msk 0.
+template <uint32_t CAP>
 __device__ inline uint32_t wave_word_or(uint32_t w, uint32_t msk) {
+    static_assert(CAP >= 2 && CAP <= 64 && (CAP & (CAP - 1)) == 0, "CAP: power of two, 2..64");
     const uint32_t lane = __lane_id();
-    for (uint32_t off = 1; off < 64; off <<= 1) {
+#pragma unroll
+    for (uint32_t off = 1; off < CAP; off <<= 1) {
         const uint32_t ow = __shfl_down(w, off, 64);
         const uint32_t om = __shfl_down(msk, off, 64);
         if (lane + off < 64 && ow == w) msk |= om;
     }
     const uint32_t pw = __shfl_up(w, 1, 64);
-    return (lane == 0 || pw != w) ? msk : 0u;
+    return ((lane & (CAP - 1)) == 0 || pw != w) ? msk : 0u;
 }
 
 // Participation bits of the members of the wave's rows for one candidate,
 // combined over the wave: adjacent rows share their anchor and mostly their
 // members' 32-bit words, so one check and one atomic per distinct word
-// replace one per row.  All 64 lanes must take part (hit false for lanes
-// without a row or without an extension).
+// (per run of up to 8 lanes for the anchor, 2 for the other members; more
+// shuffle steps cost more than the atomics they save) replace one per row.
+// All 64 lanes must take part (hit false for lanes without a row or
+// without an extension).
 template <int kArr>
 __device__ inline void wave_member_bits(uint32_t *bt, const uint64_t *seg_off, const uint32_t *rk, uint32_t km1,
                                         bool hit) {
@@ -1142,7 +1150,8 @@ __device__ inline void wave_member_bits(uint32_t *bt, const uint64_t *seg_off, c
 #pragma unroll
     for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
         if (m >= km1) break;
-        lm[m] = wave_word_or(hit ? rk[m] >> 5 : 0xffffffffu, hit ? 1u << (rk[m] & 31) : 0u);
+        const uint32_t wd = hit ? rk[m] >> 5 : 0xffffffffu, b = hit ? 1u << (rk[m] & 31) : 0u;
+        lm[m] = m == 0 ? wave_word_or<8>(wd, b) : wave_word_or<2>(wd, b);
         w[m] = bt + seg_off[m] + (rk[m] >> 5);
     }
 #pragma unroll
@@ -1273,8 +1282,9 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         acc.flush();
         if (act) slots[cd.slots_off + i] = cnt;
         if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
-        if (WAVE)
-            wave_member_bits<kArr>(bt, cd.seg_off, rk, km1, cnt != 0);
+        if (WAVE) {
+            if (__ballot(cnt != 0)) wave_member_bits<kArr>(bt, cd.seg_off, rk, km1, cnt != 0);
+        }
         else if (cnt)
             set_member_bits<kArr>(bt, cd.seg_off, rk, km1);
     }

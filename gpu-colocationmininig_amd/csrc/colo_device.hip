@@ -1369,6 +1369,97 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
     }
 }
 
+// Level-k count of R adjacent parent rows i .. i + R - 1 with the same
+// anchor: every candidate's list is read once and each entry tested against
+// all R rows (results equal R calls of joink_count_row_multi).  Parent rows
+// of deep levels come in long runs per anchor, so one list read serves
+// several rows.
+template <int KM1, int R>
+__device__ inline void joink_count_rowsR(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
+                                         uint32_t *slots, uint32_t *bits, unsigned long long *s_rows) {
+    const CandDesc &c0 = descs[0];
+    const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
+    constexpr int kArr = KM1 ? KM1 : kMaxK;
+    uint32_t rk[R][kArr];
+    float mx[R][kArr], my[R][kArr];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t *row = c0.parent + (i + r) * km1;
+#pragma unroll
+        for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+            if (m >= km1) break;
+            const uint32_t id = row[m];
+            rk[r][m] = id - D.fstart[c0.f[m]];
+            if (m) {
+                mx[r][m] = D.sxy[id].x;
+                my[r][m] = D.sxy[id].y;
+            }
+        }
+    }
+    const uint32_t t = rk[0][0];
+    uint64_t nb0 = c0.nb_off[t] - c0.nb_base, nb1 = c0.nb_off[t + 1] - c0.nb_base;
+    for (uint32_t j = 0; j < ncand; j++) {
+        const CandDesc &cd = descs[j];
+        const uint64_t b0 = nb0, b1 = nb1;
+        if (j + 1 < ncand) {
+            const CandDesc &cn = descs[j + 1];
+            nb0 = cn.nb_off[t] - cn.nb_base;
+            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+        }
+        const gcu32 *nbt = as_global(cd.nb_table);
+        const uint32_t sl = D.fstart[cd.f[km1]];
+        uint32_t *bt = bits + cd.bits_off;
+        uint32_t cnt[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) cnt[r] = 0;
+        BitAcc acc(bt + cd.seg_off[km1]);
+        auto test = [&](uint32_t q, float qx, float qy) {
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                bool ok = true;
+#pragma unroll
+                for (uint32_t m = 1; m < (uint32_t)kArr; m++) {
+                    if (m >= km1) break;
+                    if (sqdist(mx[r][m], my[r][m], qx, qy) > D.g.r2) ok = false;
+                }
+                cnt[r] += ok;
+                any |= ok;
+            }
+            if (any) acc.set(q - sl);
+        };
+        uint64_t e = b0;
+        for (; e + kJoinUnroll <= b1; e += kJoinUnroll) {
+            uint32_t q[kJoinUnroll];
+            float qx[kJoinUnroll], qy[kJoinUnroll];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) {
+                qx[u] = D.sxy[q[u]].x;
+                qy[u] = D.sxy[q[u]].y;
+            }
+#pragma unroll
+            for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
+        }
+        for (; e < b1; e++) {
+            const uint32_t q = nbt[e];
+            test(q, D.sxy[q].x, D.sxy[q].y);
+        }
+        acc.flush();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            slots[cd.slots_off + i + r] = cnt[r];
+            tot += cnt[r];
+        }
+        if (tot) atomicAdd(&s_rows[j], (unsigned long long)tot);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (cnt[r]) set_member_bits<kArr>(bt, cd.seg_off, rk[r], km1);
+    }
+}
+
 // Sets bit `rank` of `base` for every lane with `hit`, one atomic per
 // distinct 32-bit word.  Ranks must be ascending over the participating
 // lanes (list entries are sorted positions), so lanes of one word are
@@ -1607,10 +1698,11 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // (wave_rows_multi) over 64-row chunks taken by the block's waves in turn.
 // Rows per candidate are summed in LDS over the item and added to the
 // candidates' totals.
-// VAR 4 and 5 run the code of VAR 0 and 3 with the registers capped at 64
-// per lane, for 8 waves per SIMD (the compiler spills what does not fit).
+// VAR 4, 5 and 7 run the code of VAR 0, 3 and 6 with the registers capped
+// at 64 per lane, for 8 waves per SIMD (the compiler spills what does not
+// fit).
 template <int KM1, int VAR>
-__global__ __launch_bounds__(kBlock, VAR >= 4 ? 8 : 1) void k_countk_batch(DevData D, const CandDesc *desc,
+__global__ __launch_bounds__(kBlock, (VAR == 4 || VAR == 5 || VAR == 7) ? 8 : 1) void k_countk_batch(DevData D, const CandDesc *desc,
                                                                            const WorkItem *items, const uint64_t *seg,
                                                                            uint32_t *slots, uint32_t *bits,
                                                                            unsigned long long *rows, uint32_t *queue) {
@@ -1631,6 +1723,26 @@ __global__ __launch_bounds__(kBlock, VAR >= 4 ? 8 : 1) void k_countk_batch(DevDa
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                     joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits, s_rows + j);
+        } else if (VAR == 6 || VAR == 7) {
+            // four adjacent rows per thread, one list pass when they share the
+            // anchor, else pairs as in VAR 3
+            const CandDesc &c0 = desc[wi.cand];
+            const uint32_t km1 = c0.k - 1;
+            for (uint64_t i = wi.begin + 4 * (uint64_t)threadIdx.x; i < wi.end; i += 4 * kBlock) {
+                if (i + 3 < wi.end && c0.parent[i * km1] == c0.parent[(i + 3) * km1]) {
+                    joink_count_rowsR<KM1, 4>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows);
+                } else {
+                    for (uint64_t h = i; h < i + 4 && h < wi.end; h += 2) {
+                        if (h + 1 < wi.end && h + 1 < i + 4 && c0.parent[h * km1] == c0.parent[(h + 1) * km1]) {
+                            joink_count_rows2<KM1>(D, desc + wi.cand, wi.ncand, h, slots, bits, s_rows);
+                        } else {
+                            joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, h, slots, bits, s_rows);
+                            if (h + 1 < wi.end && h + 1 < i + 4)
+                                joink_count_row_multi<KM1>(D, desc + wi.cand, wi.ncand, h + 1, slots, bits, s_rows);
+                        }
+                    }
+                }
+            }
         } else if (VAR == 3 || VAR == 5) {
             // two adjacent rows per thread, one list pass when they share the anchor
             const CandDesc &c0 = desc[wi.cand];
@@ -1790,6 +1902,8 @@ template <int KM1> struct PickCount2 { static auto get() { return &k_countk_batc
 template <int KM1> struct PickCount3 { static auto get() { return &k_countk_batch<KM1, 3>; } };
 template <int KM1> struct PickCount4 { static auto get() { return &k_countk_batch<KM1, 4>; } };
 template <int KM1> struct PickCount5 { static auto get() { return &k_countk_batch<KM1, 5>; } };
+template <int KM1> struct PickCount6 { static auto get() { return &k_countk_batch<KM1, 6>; } };
+template <int KM1> struct PickCount7 { static auto get() { return &k_countk_batch<KM1, 7>; } };
 template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
 template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
 template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
@@ -1805,7 +1919,9 @@ int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
                            d_items, d_seg, d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
     else {
         const uint32_t var = c->tune.count_var_of(k);
-        hipLaunchKernelGGL(var == 5   ? pick_k<PickCount5>(k)
+        hipLaunchKernelGGL(var == 7   ? pick_k<PickCount7>(k)
+                           : var == 6 ? pick_k<PickCount6>(k)
+                           : var == 5 ? pick_k<PickCount5>(k)
                            : var == 4 ? pick_k<PickCount4>(k)
                            : var == 3 ? pick_k<PickCount3>(k)
                            : var == 2 ? pick_k<PickCount2>(k)
@@ -2245,10 +2361,10 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
 int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
     for (int k = 0; k < 16; k++)
-        if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 5) ||
+        if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 7) ||
             (t->item_rows_k[k] != COLO_TUNE_UNSET && (t->item_rows_k[k] < 64 || t->item_rows_k[k] > 65536)))
             return set_error(c, COLO_EARG, "launch shape out of range");
-    if (t->count_var > 5 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+    if (t->count_var > 7 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;

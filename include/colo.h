@@ -246,9 +246,12 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         lanes); 2 = one parent row per thread, the work
  *                         item's candidates one after another; 3 (count
  *                         only) = two adjacent rows per thread, their list
- *                         read once when they share the anchor; 4 and 5
- *                         (count only) = 0 and 3 with registers capped for
- *                         8 waves per SIMD; the autotuner tries 0 and 3-5
+ *                         read once when they share the anchor; 6 (count
+ *                         only) = four adjacent rows per thread, one list
+ *                         pass when all four share the anchor; 4, 5 and 7
+ *                         (count only) = 0, 3 and 6 with registers capped
+ *                         for 8 waves per SIMD; the autotuner tries 0 and
+ *                         3-7
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between

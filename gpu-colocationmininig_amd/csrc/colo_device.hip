@@ -1778,9 +1778,11 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
     uint32_t ids[kArr];
     float mx[kArr], my[kArr];
     auto slot_of = [&](const CandDesc &cd) { return cd.wslots ? cd.wslots[i] : slots[cd.slots_off + i]; };
-    bool any = false;
-    for (uint32_t j = 0; j < ncand; j++) any |= slot_of(descs[j]) != 0;
-    if (!any) return;
+    // candidates with rows from this parent row (a run holds at most 64)
+    uint64_t todo = 0;
+    for (uint32_t j = 0; j < ncand; j++)
+        if (slot_of(descs[j])) todo |= 1ull << j;
+    if (!todo) return;
 #pragma unroll
     for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
         if (m >= km1) break;
@@ -1791,11 +1793,23 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
         }
     }
     const uint32_t t = ids[0] - D.fstart[c0.f[0]];
+    // list bounds loaded one candidate ahead (the candidate index stays
+    // wave-uniform, so descriptor fields remain scalar loads)
+    uint64_t nb0 = 0, nb1 = 0;
+    if (todo & 1) {
+        nb0 = c0.nb_off[t] - c0.nb_base;
+        nb1 = c0.nb_off[t + 1] - c0.nb_base;
+    }
     for (uint32_t j = 0; j < ncand; j++) {
+        const uint64_t b0 = nb0, b1 = nb1;
+        if (j + 1 < ncand && ((todo >> (j + 1)) & 1)) {
+            const CandDesc &cn = descs[j + 1];
+            nb0 = cn.nb_off[t] - cn.nb_base;
+            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+        }
+        if (!((todo >> j) & 1)) continue;
         const CandDesc &cd = descs[j];
         const uint32_t nrows = slot_of(cd);
-        if (!nrows) continue;
-        const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
         const gcu32 *nbt = as_global(cd.nb_table);
         uint32_t *out = cd.table + (cd.offsets[i] - cd.off_base) * k;
         uint32_t w = 0;

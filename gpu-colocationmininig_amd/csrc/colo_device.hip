@@ -879,6 +879,83 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
         for (uint32_t m = 0; m < km1; m++) set_bit(bits + seg_off[m], row[m] - D.fstart[feat[m]]);
 }
 
+// Batched level-2 join of one anchor (sorted position p of feature a): its
+// (up to three) row ranges of feature b are walked as one stream, range 0,
+// then 1, then 2 -- the scan order -- kFlat entries per step, so a short
+// range does not cost a dependent round trip of its own.  f(q, hit) is
+// called for every entry in stream order while keep() holds.
+constexpr int kFlat = 8;
+template <typename F, typename K>
+__device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F &&f, K &&keep) {
+    const float2 pa = D.sxy[p];
+    const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
+    RowSpan rs;
+    const int m = row_spans(D.g, dirb, D.scell[p], rs);
+    const uint32_t lo0 = m > 0 ? rs.lo[0] : 0, lo1 = m > 1 ? rs.lo[1] : 0, lo2 = m > 2 ? rs.lo[2] : 0;
+    const uint32_t p1 = m > 0 ? rs.hi[0] - lo0 : 0;
+    const uint32_t p2 = p1 + (m > 1 ? rs.hi[1] - lo1 : 0);
+    const uint32_t tot = p2 + (m > 2 ? rs.hi[2] - lo2 : 0);
+    for (uint32_t s0 = 0; s0 < tot && keep(); s0 += kFlat) {
+        uint32_t q[kFlat];
+        float2 c[kFlat];
+#pragma unroll
+        for (int u = 0; u < kFlat; u++) {
+            const uint32_t e = s0 + u;
+            q[u] = e < p1 ? lo0 + e : e < p2 ? lo1 + (e - p1) : lo2 + (e - p2);
+            if (e < tot) c[u] = D.sxy[q[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kFlat; u++)
+            if (s0 + u < tot) f(q[u], sqdist(pa.x, pa.y, c[u].x, c[u].y) <= D.g.r2);
+    }
+}
+
+// Batched level-2 count of anchor rank t of feature a (join2_count_anchor
+// with SORTED, entries walked by join2_stream).
+__device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32_t b, uint32_t t, uint32_t *slots,
+                                            uint32_t *bits_a, uint32_t *bits_b) {
+    const uint32_t sb = D.fstart[b];
+    uint32_t cnt = 0;
+    BitAcc acc(bits_b);
+    join2_stream(
+        D, b, D.fstart[a] + t,
+        [&](uint32_t q, bool hit) {
+            if (hit) {
+                cnt++;
+                acc.set(q - sb);
+            }
+        },
+        [] { return true; });
+    acc.flush();
+    slots[t] = cnt;
+    if (cnt) set_bit(bits_a, t);
+    return cnt;
+}
+
+// Batched level-2 write of anchor rank t of feature a (join2_write_anchor
+// with SORTED): its nrows rows (position, position) at row offset off, the
+// second column also into nbr.
+__device__ inline void join2_write_flat(const DevData &D, uint32_t a, uint32_t b, uint32_t t, uint32_t nrows,
+                                        uint64_t off, uint32_t *table, uint32_t *nbr) {
+    const uint32_t g = D.fstart[a] + t;
+    uint32_t *out = table + off * 2;
+    uint32_t *nb = nbr ? nbr + off : nullptr;
+    uint32_t w = 0;
+    join2_stream(
+        D, b, g,
+        [&](uint32_t q, bool hit) {
+            if (hit) {
+                if (w < nrows) {
+                    out[2 * w] = g;
+                    out[2 * w + 1] = q;
+                    if (nb) nb[w] = q;
+                }
+                w++;
+            }
+        },
+        [&] { return w < nrows; });
+}
+
 // Level-2 write for one anchor of feature a (degree2TableGenerator_Kernel,
 // K:56-100): t is its input rank and the rows are (instance id, instance
 // id) -- or, with SORTED, t is its sorted rank and the rows are (sorted
@@ -1682,8 +1759,7 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
         const uint32_t a = cd.f[0], b = cd.f[1];
         uint64_t sum = 0;
         for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-            sum += join2_count_anchor<true>(D, a, b, D.fstart[a] + (uint32_t)i, sl, bt + cd.seg_off[0],
-                                            bt + cd.seg_off[1]);
+            sum += join2_count_flat(D, a, b, (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
         uint64_t ex;
         const uint64_t tot = block_excl_scan(sum, ex);
         if (threadIdx.x == 0 && tot) atomicAdd(&rows[wi.cand], (unsigned long long)tot);
@@ -1863,7 +1939,7 @@ __global__ __launch_bounds__(kBlock) void k_write2_batch(DevData D, const CandDe
         for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
             uint32_t nr = sl[t];
             if (nr)
-                join2_write_anchor<true>(D, cd.f[0], cd.f[1], t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
+                join2_write_flat(D, cd.f[0], cd.f[1], (uint32_t)t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
         }
     }
 }

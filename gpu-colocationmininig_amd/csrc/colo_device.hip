@@ -642,6 +642,13 @@ int colo::filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_
         // the group arrays live in scratch: finish before the next scratch use
         COLO_HIP(hipStreamSynchronize(s));
     }
+    return threshold_batch(c, d_cands, C, k, theta, d_counts, d_pass, s);
+}
+
+int colo::threshold_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
+                          const uint32_t *d_counts, uint8_t *d_pass, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (C == 0) return COLO_OK;
     hipLaunchKernelGGL(k_threshold, dim3(grid_for(C)), dim3(kBlock), 0, s, d_counts, d_cands, C, k,
                        c->d_fcount, theta, d_pass);
     COLO_HIP(hipGetLastError());

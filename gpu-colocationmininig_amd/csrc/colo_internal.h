@@ -248,6 +248,10 @@ int scan_u32_u64(colo_ctx *c, const uint32_t *in, uint64_t n, uint64_t *out, uin
 int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 int filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
                  uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
+// The filter's pass flags (threshold_Kernel, K:545-573) from counts already
+// known (a threshold sweep keeps the theta-independent counts).
+int threshold_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
+                    const uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
 // Batched passes over many candidates (one launch each).
 // d_seg: 9 item boundaries, one segment per XCD (see xcd_segments);
 // d_queue: 8 counters.

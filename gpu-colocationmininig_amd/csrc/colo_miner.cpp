@@ -184,6 +184,7 @@ struct SweepMemo {
     };
     std::map<std::vector<uint32_t>, Res> res;
     std::map<std::pair<uint32_t, uint32_t>, Prev> l2;
+    std::map<std::vector<uint32_t>, std::vector<uint32_t>> fcnt;  // filter counts (theta-independent)
     uint64_t hits = 0, misses = 0;
 };
 
@@ -877,15 +878,35 @@ struct Miner {
                 if (!d_cands.ensure(C.size() * 4) || !d_counts.ensure(C.size() * 4) || !d_pass.ensure(nC))
                     return set_error(c, COLO_ENOMEM, "filter buffers");
                 COLO_HIP(hipMemcpyAsync(d_cands.p, C.data(), C.size() * 4, hipMemcpyHostToDevice, s));
+                // in a sweep, counts known from an earlier threshold need
+                // only the threshold test
+                bool cached = memo != nullptr;
+                if (memo)
+                    for (uint64_t i = 0; i < nC && cached; i++) {
+                        auto it = memo->fcnt.find(std::vector<uint32_t>(&C[i * k], &C[i * k] + k));
+                        if (it == memo->fcnt.end()) cached = false;
+                        else memcpy(&fcnt[i * k], it->second.data(), k * 4);
+                    }
                 size_t e0 = mark();
-                int rc = filter_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
+                int rc;
+                if (cached) {
+                    COLO_HIP(hipMemcpyAsync(d_counts.p, fcnt.data(), fcnt.size() * 4, hipMemcpyHostToDevice, s));
+                    rc = threshold_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
+                                         d_pass.as<uint8_t>(), s);
+                } else {
+                    rc = filter_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
                                       d_pass.as<uint8_t>(), s);
+                }
                 if (rc) return rc;
                 span(e0, mark(), kFilter);
-                COLO_HIP(hipMemcpyAsync(fcnt.data(), d_counts.p, fcnt.size() * 4, hipMemcpyDeviceToHost, s));
+                if (!cached) COLO_HIP(hipMemcpyAsync(fcnt.data(), d_counts.p, fcnt.size() * 4, hipMemcpyDeviceToHost, s));
                 COLO_HIP(hipMemcpyAsync(pass.data(), d_pass.p, nC, hipMemcpyDeviceToHost, s));
                 if ((rc = sync())) return rc;
-                st.launches += 2;
+                st.launches += cached ? 1 : 2;
+                if (memo && !cached)
+                    for (uint64_t i = 0; i < nC; i++)
+                        memo->fcnt[std::vector<uint32_t>(&C[i * k], &C[i * k] + k)] =
+                            std::vector<uint32_t>(&fcnt[i * k], &fcnt[i * k] + k);
             }
             st.filter_ms += ms_since(tf);
             uint64_t npass = 0;

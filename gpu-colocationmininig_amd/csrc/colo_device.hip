@@ -1250,17 +1250,19 @@ __device__ inline void wave_member_bits(uint32_t *bt, const uint64_t *seg_off, c
     }
 }
 
-// Sets the participation bits of a row's members: every check load is
-// issued before the first atomic, so the members cost one round trip.
+// Sets the participation bits of a row's members m0 .. km1-1: every check
+// load is issued before the first atomic, so the members cost one round
+// trip.
 template <int kArr>
-__device__ inline void set_member_bits(uint32_t *bt, const uint64_t *seg_off, const uint32_t *rk, uint32_t km1) {
+__device__ inline void set_member_bits(uint32_t *bt, const uint64_t *seg_off, const uint32_t *rk, uint32_t km1,
+                                       uint32_t m0 = 0) {
     uint32_t *w[kArr];
     uint32_t cur[kArr];
 #pragma unroll
     for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
         if (m >= km1) break;
         w[m] = bt + seg_off[m] + (rk[m] >> 5);
-        cur[m] = __hip_atomic_load(w[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        cur[m] = m >= m0 ? __hip_atomic_load(w[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0u;
     }
 #pragma unroll
     for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
@@ -1448,8 +1450,9 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
         slots[cd.slots_off + i] = cnt0;
         slots[cd.slots_off + i + 1] = cnt1;
         if (cnt0 + cnt1) atomicAdd(&s_rows[j], (unsigned long long)(cnt0 + cnt1));
+        // the two rows share the anchor: its bit is set once
         if (cnt0) set_member_bits<kArr>(bt, cd.seg_off, rk0, km1);
-        if (cnt1) set_member_bits<kArr>(bt, cd.seg_off, rk1, km1);
+        if (cnt1) set_member_bits<kArr>(bt, cd.seg_off, rk1, km1, cnt0 ? 1u : 0u);
     }
 }
 
@@ -1538,9 +1541,15 @@ __device__ inline void joink_count_rowsR(const DevData &D, const CandDesc *descs
             tot += cnt[r];
         }
         if (tot) atomicAdd(&s_rows[j], (unsigned long long)tot);
+        // the R rows share the anchor: its bit is set with the first row
+        // that has an extension, the other members per row
+        bool first = true;
 #pragma unroll
         for (int r = 0; r < R; r++)
-            if (cnt[r]) set_member_bits<kArr>(bt, cd.seg_off, rk[r], km1);
+            if (cnt[r]) {
+                set_member_bits<kArr>(bt, cd.seg_off, rk[r], km1, first ? 0u : 1u);
+                first = false;
+            }
     }
 }
 

@@ -1456,6 +1456,123 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
     }
 }
 
+// Level-k count of parent rows i and i + 1 (those below end) with the wave
+// converged: all 64 lanes call together, each with its own pair.  When the
+// two rows share the anchor one pass over the anchor's list tests both;
+// otherwise each row's list is walked in turn.  The members' bits are
+// combined over the wave as in joink_count_row_multi.  Results equal two
+// calls of joink_count_row_multi.
+template <int KM1>
+__device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
+                                          uint64_t end, uint32_t *slots, uint32_t *bits,
+                                          unsigned long long *s_rows) {
+    const CandDesc &c0 = descs[0];
+    const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
+    constexpr int kArr = KM1 ? KM1 : kMaxK;
+    const bool act0 = i < end, act1 = i + 1 < end;
+    uint32_t rk0[kArr], rk1[kArr];
+    float mx0[kArr], my0[kArr], mx1[kArr], my1[kArr];
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        rk0[m] = rk1[m] = 0;
+        mx0[m] = my0[m] = mx1[m] = my1[m] = 0.f;
+        const uint32_t fs = D.fstart[c0.f[m]];
+        if (act0) {
+            const uint32_t id = c0.parent[i * km1 + m];
+            rk0[m] = id - fs;
+            if (m) {
+                mx0[m] = D.sxy[id].x;
+                my0[m] = D.sxy[id].y;
+            }
+        }
+        if (act1) {
+            const uint32_t id = c0.parent[(i + 1) * km1 + m];
+            rk1[m] = id - fs;
+            if (m) {
+                mx1[m] = D.sxy[id].x;
+                my1[m] = D.sxy[id].y;
+            }
+        }
+    }
+    const bool shared = act1 && rk0[0] == rk1[0];
+    const bool own1 = act1 && !shared;  // row i + 1 walks its own list
+    const uint32_t t0 = rk0[0], t1 = rk1[0];
+    uint64_t na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
+    if (act0) {
+        na0 = c0.nb_off[t0] - c0.nb_base;
+        na1 = c0.nb_off[t0 + 1] - c0.nb_base;
+    }
+    if (own1) {
+        nb0 = c0.nb_off[t1] - c0.nb_base;
+        nb1 = c0.nb_off[t1 + 1] - c0.nb_base;
+    }
+    for (uint32_t j = 0; j < ncand; j++) {
+        const CandDesc &cd = descs[j];
+        const uint64_t a0 = na0, a1 = na1, e0 = nb0, e1 = nb1;
+        if (j + 1 < ncand) {
+            const CandDesc &cn = descs[j + 1];
+            if (act0) {
+                na0 = cn.nb_off[t0] - cn.nb_base;
+                na1 = cn.nb_off[t0 + 1] - cn.nb_base;
+            }
+            if (own1) {
+                nb0 = cn.nb_off[t1] - cn.nb_base;
+                nb1 = cn.nb_off[t1 + 1] - cn.nb_base;
+            }
+        }
+        const gcu32 *nbt = as_global(cd.nb_table);
+        const uint32_t sl = D.fstart[cd.f[km1]];
+        uint32_t *bt = bits + cd.bits_off;
+        uint32_t cnt0 = 0, cnt1 = 0;
+        BitAcc acc(bt + cd.seg_off[km1]);
+        // pass 1: the anchor list of row i, tested against row i and, when
+        // shared, row i + 1; pass 2: row i + 1's own list
+        auto walk = [&](uint64_t b0, uint64_t b1, bool use0, bool use1) {
+            auto test = [&](uint32_t q, float qx, float qy) {
+                bool ok0 = use0, ok1 = use1;
+#pragma unroll
+                for (uint32_t m = 1; m < (uint32_t)kArr; m++) {
+                    if (m >= km1) break;
+                    if (sqdist(mx0[m], my0[m], qx, qy) > D.g.r2) ok0 = false;
+                    if (sqdist(mx1[m], my1[m], qx, qy) > D.g.r2) ok1 = false;
+                }
+                cnt0 += ok0;
+                cnt1 += ok1;
+                if (ok0 || ok1) acc.set(q - sl);
+            };
+            uint64_t e = b0;
+            for (; e + kJoinUnroll <= b1; e += kJoinUnroll) {
+                uint32_t q[kJoinUnroll];
+                float qx[kJoinUnroll], qy[kJoinUnroll];
+#pragma unroll
+                for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
+#pragma unroll
+                for (int u = 0; u < kJoinUnroll; u++) {
+                    qx[u] = D.sxy[q[u]].x;
+                    qy[u] = D.sxy[q[u]].y;
+                }
+#pragma unroll
+                for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
+            }
+            for (; e < b1; e++) {
+                const uint32_t q = nbt[e];
+                test(q, D.sxy[q].x, D.sxy[q].y);
+            }
+        };
+        walk(a0, a1, act0, shared);
+        walk(e0, e1, false, own1);
+        acc.flush();
+        if (act0) slots[cd.slots_off + i] = cnt0;
+        if (act1) slots[cd.slots_off + i + 1] = cnt1;
+        if (cnt0 + cnt1) atomicAdd(&s_rows[j], (unsigned long long)(cnt0 + cnt1));
+        if (__ballot(cnt0 + cnt1 != 0)) {
+            wave_member_bits<kArr>(bt, cd.seg_off, rk0, km1, cnt0 != 0);
+            wave_member_bits<kArr>(bt, cd.seg_off, rk1, km1, cnt1 != 0);
+        }
+    }
+}
+
 // Level-k count of R adjacent parent rows i .. i + R - 1 with the same
 // anchor: every candidate's list is read once and each entry tested against
 // all R rows (results equal R calls of joink_count_row_multi).  Parent rows
@@ -1790,11 +1907,11 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // (wave_rows_multi) over 64-row chunks taken by the block's waves in turn.
 // Rows per candidate are summed in LDS over the item and added to the
 // candidates' totals.
-// VAR 4, 5 and 7 run the code of VAR 0, 3 and 6 with the registers capped
-// at 64 per lane, for 8 waves per SIMD (the compiler spills what does not
-// fit).
+// VAR 4, 5, 7 and 9 run the code of VAR 0, 3, 6 and 8 with the registers
+// capped at 64 per lane, for 8 waves per SIMD (the compiler spills what
+// does not fit).
 template <int KM1, int VAR>
-__global__ __launch_bounds__(kBlock, (VAR == 4 || VAR == 5 || VAR == 7) ? 8 : 1) void k_countk_batch(DevData D, const CandDesc *desc,
+__global__ __launch_bounds__(kBlock, (VAR == 4 || VAR == 5 || VAR == 7 || VAR == 9) ? 8 : 1) void k_countk_batch(DevData D, const CandDesc *desc,
                                                                            const WorkItem *items, const uint64_t *seg,
                                                                            uint32_t *slots, uint32_t *bits,
                                                                            unsigned long long *rows, uint32_t *queue) {
@@ -1815,6 +1932,11 @@ __global__ __launch_bounds__(kBlock, (VAR == 4 || VAR == 5 || VAR == 7) ? 8 : 1)
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                     joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits, s_rows + j);
+        } else if (VAR == 8 || VAR == 9) {
+            // two adjacent rows per lane, waves converged
+            for (uint64_t i0 = wi.begin + 2 * (uint64_t)(threadIdx.x & ~63u); i0 < wi.end; i0 += 2 * kBlock)
+                joink_count_rows2w<KM1>(D, desc + wi.cand, wi.ncand, i0 + 2 * __lane_id(), wi.end, slots, bits,
+                                        s_rows);
         } else if (VAR == 6 || VAR == 7) {
             // four adjacent rows per thread, one list pass when they share the
             // anchor, else pairs as in VAR 3
@@ -2010,6 +2132,8 @@ template <int KM1> struct PickCount4 { static auto get() { return &k_countk_batc
 template <int KM1> struct PickCount5 { static auto get() { return &k_countk_batch<KM1, 5>; } };
 template <int KM1> struct PickCount6 { static auto get() { return &k_countk_batch<KM1, 6>; } };
 template <int KM1> struct PickCount7 { static auto get() { return &k_countk_batch<KM1, 7>; } };
+template <int KM1> struct PickCount8 { static auto get() { return &k_countk_batch<KM1, 8>; } };
+template <int KM1> struct PickCount9 { static auto get() { return &k_countk_batch<KM1, 9>; } };
 template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
 template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
 template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
@@ -2025,7 +2149,9 @@ int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
                            d_items, d_seg, d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
     else {
         const uint32_t var = c->tune.count_var_of(k);
-        hipLaunchKernelGGL(var == 7   ? pick_k<PickCount7>(k)
+        hipLaunchKernelGGL(var == 9   ? pick_k<PickCount9>(k)
+                           : var == 8 ? pick_k<PickCount8>(k)
+                           : var == 7 ? pick_k<PickCount7>(k)
                            : var == 6 ? pick_k<PickCount6>(k)
                            : var == 5 ? pick_k<PickCount5>(k)
                            : var == 4 ? pick_k<PickCount4>(k)
@@ -2467,10 +2593,10 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
 int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
     for (int k = 0; k < 16; k++)
-        if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 7) ||
+        if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 9) ||
             (t->item_rows_k[k] != COLO_TUNE_UNSET && (t->item_rows_k[k] < 64 || t->item_rows_k[k] > 65536)))
             return set_error(c, COLO_EARG, "launch shape out of range");
-    if (t->count_var > 7 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+    if (t->count_var > 9 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;

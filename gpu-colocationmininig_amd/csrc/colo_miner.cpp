@@ -1602,8 +1602,8 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
         // level's wall time (count, scan and write of that level); the
         // wave-cooperative and candidate-major counts (1, 2) are left out,
         // slower on every workload measured
-        const uint32_t vars[] = {0, 3, 4, 5, 6, 7}, sizes[] = {1024, 4096, 16384};
-        constexpr int nv = 6, ns = 3;
+        const uint32_t vars[] = {0, 3, 4, 5, 6, 7, 8, 9}, sizes[] = {1024, 4096, 16384};
+        constexpr int nv = 8, ns = 3;
         double lv[nv][ns][16], tot[nv] = {};
         for (int si = 0; si < ns; si++)
             for (int vi = 0; vi < nv; vi++) {

@@ -288,7 +288,8 @@ def test_threshold_sweep(c2small, colo_mod, sched):
                                   dict(count_var=4, count_var_k=[0xFFFFFFFF] * 3 + [5, 4, 5] + [0xFFFFFFFF] * 10),
                                   dict(item_rows=1024, item_rows_k=[0xFFFFFFFF] * 2 + [64, 256, 16384] + [0xFFFFFFFF] * 11),
                                   dict(count_var=4, item_rows=256), dict(count_var=0, item_rows=64),
-                                  dict(count_var=6), dict(count_var=7, item_rows=256)])
+                                  dict(count_var=6), dict(count_var=7, item_rows=256),
+                                  dict(count_var=8), dict(count_var=9, item_rows=64)])
 def test_launch_shapes_identical(c2small, tune):
     """Every launch shape of the batched passes gives the oracle's report."""
     ctx, orc, p = c2small
@@ -313,7 +314,7 @@ def test_autotune_and_cache(c2small, tmp_path):
     try:
         t1 = ctx.autotune(0.15, max_k=4, cache=cache)
         assert os.path.getsize(cache) > 0
-        assert t1["count_var_k"][3] <= 7 and all(v <= 7 or v == 0xFFFFFFFF for v in t1["count_var_k"]), t1
+        assert t1["count_var_k"][3] <= 9 and all(v <= 9 or v == 0xFFFFFFFF for v in t1["count_var_k"]), t1
         assert t1["item_rows_k"][2] in (1024, 4096, 16384), t1
         ctx.tune = old
         t2 = ctx.autotune(0.15, max_k=4, cache=cache)

@@ -1167,8 +1167,6 @@ __global__ void k_joink_write_nbr(DevData D, const uint32_t *parent, uint64_t R,
 // ---- batched (persistent) kernels: a fixed grid drains a work queue of
 // (candidate, anchor range) items in queue order.
 
-constexpr int kItemAnchors = kBlock * 4;
-
 // Work queue with one segment per XCD.  Workgroups are dispatched to the 8
 // XCDs round-robin by workgroup id, so workgroup b drains segment b % 8
 // first -- consecutive items (adjacent anchors of one candidate, which read
@@ -1910,11 +1908,12 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // VAR 4, 5, 7 and 9 run the code of VAR 0, 3, 6 and 8 with the registers
 // capped at 64 per lane, for 8 waves per SIMD (the compiler spills what
 // does not fit).
+constexpr bool capped_var(int v) { return v == 4 || v == 5 || v == 7 || v == 9; }
+
 template <int KM1, int VAR>
-__global__ __launch_bounds__(kBlock, (VAR == 4 || VAR == 5 || VAR == 7 || VAR == 9) ? 8 : 1) void k_countk_batch(DevData D, const CandDesc *desc,
-                                                                           const WorkItem *items, const uint64_t *seg,
-                                                                           uint32_t *slots, uint32_t *bits,
-                                                                           unsigned long long *rows, uint32_t *queue) {
+__global__ __launch_bounds__(kBlock, capped_var(VAR) ? 8 : 1) void k_countk_batch(
+    DevData D, const CandDesc *desc, const WorkItem *items, const uint64_t *seg, uint32_t *slots, uint32_t *bits,
+    unsigned long long *rows, uint32_t *queue) {
     __shared__ unsigned long long s_rows[kMaxRun];
     uint64_t it;
     while (next_item(queue, seg, it)) {
@@ -2491,7 +2490,6 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
     if (!ds || !out) return COLO_EARG;
     *out = nullptr;
     colo_ctx *c = new colo_ctx();
-    colo_ctx *ctx_for_err = c;
     int rc = COLO_OK;
     c->n = ds->n;
     c->F = ds->F;

@@ -1905,13 +1905,14 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // (wave_rows_multi) over 64-row chunks taken by the block's waves in turn.
 // Rows per candidate are summed in LDS over the item and added to the
 // candidates' totals.
-// VAR 4, 5, 7 and 9 run the code of VAR 0, 3, 6 and 8 with the registers
-// capped at 64 per lane, for 8 waves per SIMD (the compiler spills what
-// does not fit).
-constexpr bool capped_var(int v) { return v == 4 || v == 5 || v == 7 || v == 9; }
+// VAR 4 and 5 run the code of VAR 0 and 3 with the registers capped at 64
+// per lane, for 8 waves per SIMD; VAR 7 and 9 the code of VAR 6 and 8 at
+// 102 per lane, for 5 waves per SIMD (the compiler spills what does not
+// fit).
+constexpr int var_waves(int v) { return v == 4 || v == 5 ? 8 : v == 7 || v == 9 ? 5 : 1; }
 
 template <int KM1, int VAR>
-__global__ __launch_bounds__(kBlock, capped_var(VAR) ? 8 : 1) void k_countk_batch(
+__global__ __launch_bounds__(kBlock, var_waves(VAR)) void k_countk_batch(
     DevData D, const CandDesc *desc, const WorkItem *items, const uint64_t *seg, uint32_t *slots, uint32_t *bits,
     unsigned long long *rows, uint32_t *queue) {
     __shared__ unsigned long long s_rows[kMaxRun];

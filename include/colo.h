@@ -251,10 +251,10 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         pass when all four share the anchor; 8 (count
  *                         only) = two adjacent rows per lane with the wave
  *                         converged, one list pass when they share the
- *                         anchor, member bits combined over the wave; 4,
- *                         5, 7 and 9 (count only) = 0, 3, 6 and 8 with
- *                         registers capped for 8 waves per SIMD; the
- *                         autotuner tries 0 and 3-9
+ *                         anchor, member bits combined over the wave; 4
+ *                         and 5 (count only) = 0 and 3 with registers
+ *                         capped for 8 waves per SIMD, 7 and 9 = 6 and 8
+ *                         capped for 5; the autotuner tries 0 and 3-9
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between

@@ -1,6 +1,7 @@
 """CPU checks of the product library (no GPU calls): it loads, exports every
 entry point include/colo.h declares, and its host-side parts (CSV parser,
 candidate generation) agree with the oracle."""
+import ctypes as C
 import os
 import re
 
@@ -167,3 +168,37 @@ def test_csv_binary_cache(colo_mod, datagen, tmp_path):
     b = colo_mod.HostData(w["csv"], cache=cache)
     assert b.x[0] == np.float32(1.5) and b.y[0] == np.float32(2.5)
     _hd_equal(colo_mod.HostData(w["csv"], cache=cache), b)
+
+
+def test_ctypes_mirrors_match_header(colo_mod, tmp_path):
+    """The ctypes structures of the Python binding have the sizes and field
+    offsets of the C structures in include/colo.h (compiled with gcc)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if not gcc:
+        pytest.skip("gcc not available")
+    mirrors = {"colo_dataset": colo_mod.Dataset, "colo_grid": colo_mod.Grid, "colo_mine_opts": colo_mod.MineOpts,
+               "colo_tune": colo_mod.Tune, "colo_mine_stats": colo_mod.MineStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "colo.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run([gcc, "-std=c11", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            cname, field, val = line.split()
+            got[(cname, field)] = int(val)
+    for cname, py in mirrors.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

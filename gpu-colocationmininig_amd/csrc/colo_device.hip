@@ -1898,17 +1898,19 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 }
 
 // Level k >= 3 (KM1 = k - 1; 0 for any k).  VAR 0: one parent row per
-// thread, all of a run's candidates per row; VAR 2: one parent row per
-// thread, the item's candidates one after another (one neighbour-list
-// stream at a time); VAR 3: two adjacent rows per thread, their lists read
-// once when they share the anchor; VAR 1: wave-cooperative
-// (wave_rows_multi) over 64-row chunks taken by the block's waves in turn.
-// Rows per candidate are summed in LDS over the item and added to the
-// candidates' totals.
-// VAR 4 and 5 run the code of VAR 0 and 3 with the registers capped at 64
-// per lane, for 8 waves per SIMD; VAR 7 and 9 the code of VAR 6 and 8 at
-// 102 per lane, for 5 waves per SIMD (the compiler spills what does not
-// fit).
+// lane, all of a run's candidates per row, waves converged and member bits
+// combined over the wave; VAR 1: wave-cooperative (wave_rows_multi) over
+// 64-row chunks taken by the block's waves in turn; VAR 2: one parent row
+// per thread, the item's candidates one after another (one neighbour-list
+// stream at a time); VAR 3: two adjacent rows per thread, their list read
+// once when they share the anchor; VAR 6: four adjacent rows per thread,
+// one list pass when all four share the anchor; VAR 8: two adjacent rows
+// per lane, waves converged, member bits combined over the wave.  Rows per
+// candidate are summed in LDS over the item and added to the candidates'
+// totals.  VAR 4 and 5 run the code of VAR 0 and 3 with the registers
+// capped at 64 per lane, for 8 waves per SIMD; VAR 7 and 9 the code of
+// VAR 6 and 8 at 102 per lane, for 5 waves per SIMD (the compiler spills
+// what does not fit).
 constexpr int var_waves(int v) { return v == 4 || v == 5 ? 8 : v == 7 || v == 9 ? 5 : 1; }
 
 template <int KM1, int VAR>

@@ -699,7 +699,8 @@ __device__ inline float sqdist(float x1, float y1, float x2, float y2) {
     return sx + sy;
 }
 
-constexpr int kJoinUnroll = 4;  // list entries whose loads are issued together
+constexpr int kJoinUnroll = 4;   // list entries whose loads are issued together (counts)
+constexpr int kWriteUnroll = 8;  // the same for the level-k write, which runs fewer rows
 
 // A pointer read from a descriptor in memory, as a global-memory pointer
 // (global rather than flat loads).
@@ -2049,18 +2050,18 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
             }
         };
         uint64_t e = b0;
-        for (; e + kJoinUnroll <= b1 && w < nrows; e += kJoinUnroll) {
-            uint32_t q[kJoinUnroll];
-            float qx[kJoinUnroll], qy[kJoinUnroll];
+        for (; e + kWriteUnroll <= b1 && w < nrows; e += kWriteUnroll) {
+            uint32_t q[kWriteUnroll];
+            float qx[kWriteUnroll], qy[kWriteUnroll];
 #pragma unroll
-            for (int u = 0; u < kJoinUnroll; u++) q[u] = nbt[e + u];
+            for (int u = 0; u < kWriteUnroll; u++) q[u] = nbt[e + u];
 #pragma unroll
-            for (int u = 0; u < kJoinUnroll; u++) {
+            for (int u = 0; u < kWriteUnroll; u++) {
                 qx[u] = D.sxy[q[u]].x;
                 qy[u] = D.sxy[q[u]].y;
             }
 #pragma unroll
-            for (int u = 0; u < kJoinUnroll; u++) test(q[u], qx[u], qy[u]);
+            for (int u = 0; u < kWriteUnroll; u++) test(q[u], qx[u], qy[u]);
         }
         for (; e < b1 && w < nrows; e++) {
             const uint32_t q = nbt[e];

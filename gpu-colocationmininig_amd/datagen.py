@@ -82,13 +82,14 @@ def _clamp_cells(x, y, S, r):
     return x, y
 
 
-def training_config(rng, n_range=(100_000, 2_000_000)):
+def training_config(rng, n_range=(100_000, 2_000_000), f_range=(10, 40)):
     """Randomised parameters of the c4 training datasets (drawn around the
-    c1/c2 generator): features 10-40, instances in n_range, hotspot
-    fraction 0-0.6, sigma 0.1-2% of the side, r for 8-64 mean neighbours,
-    theta 0.05-0.5, hotspots shared by all features (40%) or hosting 2..F
-    features each.  Returns keyword overrides for generate("c2", ...)."""
-    F = int(rng.integers(10, 41))
+    c1/c2 generator): features in f_range (10-40), instances in n_range,
+    hotspot fraction 0-0.6, sigma 0.1-2% of the side, r for 8-64 mean
+    neighbours, theta 0.05-0.5, hotspots shared by all features (40%) or
+    hosting 2..F features each.  Returns keyword overrides for
+    generate("c2", ...)."""
+    F = int(rng.integers(f_range[0], f_range[1] + 1))
     lo = float(rng.uniform(100, 2000))
     hf = None if rng.random() < 0.4 else int(rng.integers(2, F + 1))
     return dict(F=F, N=int(rng.integers(n_range[0], n_range[1] + 1)), hot_frac=float(rng.uniform(0.0, 0.6)),

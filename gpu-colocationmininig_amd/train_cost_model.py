@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--datasets", type=int, default=200)
     ap.add_argument("--min-n", type=int, default=100_000)
     ap.add_argument("--max-n", type=int, default=2_000_000)
+    ap.add_argument("--max-f", type=int, default=40, help="features per training dataset: 10 .. max-f")
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--seed", type=int, default=4000)
     ap.add_argument("--eval-n", type=int, default=20_000_000)
@@ -59,7 +60,7 @@ def main():
     rng = np.random.default_rng(a.seed)
     jobs = []
     for i in range(a.datasets):
-        jobs.append((a.seed + i, datagen.training_config(rng, n_range=(a.min_n, a.max_n))))
+        jobs.append((a.seed + i, datagen.training_config(rng, n_range=(a.min_n, a.max_n), f_range=(10, a.max_f))))
     ecfg = datagen.training_config(np.random.default_rng(a.seed + 999), n_range=(a.eval_n, a.eval_n))
     ecfg["F"] = 40
     # generation runs in spawned CPU workers (no GPU use there), started

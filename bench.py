@@ -10,10 +10,11 @@ every prevalence threshold of the workload in one colo_mine_sweep call
 The default workload is c5: 50M instances, 60 features with Zipf(1.2)
 frequencies, 60% of the points in 2,000 Gaussian hotspots (sigma log-uniform
 in 100-2000 on a 1e5 square, 12 features per hotspot), 20 planted patterns of
-size 3-6, r for ~64 mean neighbours, patterns up to size 6, thresholds 0.1,
-0.2, 0.3 and 0.5.  The data is generated in memory with a fixed seed
-(float64 coordinates converted to float32, the reference's coordinate type)
-before the timed region.  c1/c2/c3 are available with --workload.
+size 3-6 (each a run of consecutive frequency ranks in 6..59, planted at 50%),
+r for ~64 mean neighbours, patterns up to size 6, thresholds 0.1, 0.2, 0.3 and
+0.5.  The data is generated in memory with a fixed seed (float64 coordinates
+converted to float32, the reference's coordinate type) before the timed
+region.  c1/c2/c3 are available with --workload.
 
 Prints one JSON line.  value = neighbour pairs per second of the level-2
 join (pairs counted by the level-2 count pass / wall time of the level-2
@@ -21,8 +22,12 @@ count + PI + scan + write); mining_s = wall time of a step.  The timed
 schedule is the learned one (persistent launches whose work queue is ordered
 and partitioned, and whose table arenas are pre-sized, by the trained cost
 model in gpu-colocationmininig_amd/models/cost_mlp.bin) when that model is
-present, else the persistent one; the persistent and plain (per-candidate)
-schedules also run on the same data and every report must be identical.
+present, else the persistent one.  The persistent schedule (autotuned for
+itself) and the plain per-candidate schedule also run on the same data; one
+more sweep of each schedule with table checksums must give identical reports.
+The CPU baseline is the oracle (the C/OpenMP restatement of the reference) on
+the host cores: level-2 joins of the same workload over a time budget, plus
+complete mining of c2 (16 threads and 1 thread) and c1 beside the GPU's.
 """
 import argparse
 import json
@@ -35,30 +40,62 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-colocationmininig_amd"))
 
 METRIC = "end-to-end mining s + neighbor pairs/s; learned-sched vs plain, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+T_START = time.perf_counter()
 
 
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T_START:6.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def describe(p):
     th = ", ".join(f"{t:g}" for t in p["thetas"])
-    return (f"{p['name']}: {p['N']} instances, {p['F']} features, r={p['r']:g}, theta in {{{th}}}, "
-            f"patterns up to size {p['max_k']}, seed {p['seed']}")
+    s = f"{p['name']}: {p['N']} instances, {p['F']} features"
+    if p.get("zipf"):
+        s += f" (Zipf {p['zipf']:g})"
+    if p.get("hotspots"):
+        lo, hi = p["sigma"]
+        s += (f", {p['hot_frac']:.0%} in {p['hotspots']} hotspots (sigma {lo:g}-{hi:g}"
+              + (f", {p['hot_feats']} features per hotspot" if p.get("hot_feats") else "") + ")")
+    sizes = sorted(set(len(x) for x in p["planted"]))
+    s += f", {len(p['planted'])} planted patterns of size {sizes[0]}-{sizes[-1]} at {p['plant_rate']:.0%}"
+    if p.get("plant_band"):
+        s += f" (runs of consecutive frequency ranks in {p['plant_band'][0]}..{p['plant_band'][1] - 1})"
+    s += f", r={p['r']:g}, theta in {{{th}}}, patterns up to size {p['max_k']}, seed {p['seed']}"
+    return s
 
 
-def prepare(args):
+def gen(name, N=None):
     import datagen
     import numpy as np
-    t0 = time.perf_counter()
-    labels, x, y, S, p = datagen.generate(args.workload, N=args.N)
+    labels, x, y, S, p = datagen.generate(name, N=N)
     fc = np.bincount(labels, minlength=p["F"]).astype(np.uint32)
-    x32, y32 = x.astype(np.float32), y.astype(np.float32)
-    log(f"generated {describe(p)} in {time.perf_counter() - t0:.1f} s")
-    return x32, y32, fc, np.array([0, 0, S, S], np.float32), p
+    return x.astype(np.float32), y.astype(np.float32), fc, np.array([0, 0, S, S], np.float32), p
 
 
-def cpu_baseline(x, y, fc, bounds, p, budget_s, threads):
+def cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads():
+    # the GPU box gives one GPU's process 16 host threads, whatever nproc says
+    return max(1, min(16, cpu_info()["affinity_cpus"]))
+
+
+def cpu_level2_sample(x, y, fc, bounds, p, budget_s, threads):
     """The oracle's level-2 joins (count + write) of the same workload on
     host cores, over as many candidate pairs as fit the time budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -68,8 +105,7 @@ def cpu_baseline(x, y, fc, bounds, p, budget_s, threads):
     o.bin(p["r"])
     F = o.F
     pairs = [(a, b) for a in range(F) for b in range(a + 1, F)]
-    # spread the sample over frequent and rare features
-    pairs.sort(key=lambda ab: (ab[0] + ab[1]) % 7)
+    pairs.sort(key=lambda ab: (ab[0] + ab[1]) % 7)  # spread over frequent and rare features
     rows = 0
     t_tot = 0.0
     used = 0
@@ -82,10 +118,46 @@ def cpu_baseline(x, y, fc, bounds, p, budget_s, threads):
         used += 1
         if t_tot > budget_s:
             break
-    return {"value": rows / t_tot if t_tot else 0.0, "unit": "neighbor pairs/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle level-2 joins (count + write) of {used} of {len(pairs)} feature pairs of "
-                      f"{p['name']} ({rows} pairs, {t_tot:.1f} s, OpenMP {threads} threads)"}
+    return {"value": rows / t_tot if t_tot else 0.0, "unit": "neighbor pairs/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C/OpenMP restatement of the reference) level-2 joins, count + write, of {used} of "
+                      f"{len(pairs)} feature pairs of {p['name']} ({rows} pairs in {t_tot:.1f} s on {threads} threads)",
+            "sample_s": t_tot, "sample_pairs": used, "all_pairs": len(pairs)}
+
+
+def cpu_vs_gpu_mining(colo, model, threads):
+    """Complete mining of c2 (oracle on `threads` threads and on 1 thread) and
+    c1 (1 thread), each beside the GPU learned schedule on the same data,
+    reports (with table checksums) compared."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import torch
+    out = {}
+    for name, tlist in (("c2", (threads, 1)), ("c1", (1,))):
+        x, y, fc, b, p = gen(name)
+        ctx = colo.Context(x, y, fc, b, p["r"])
+        sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
+        ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=model, hash_tables=False)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.rebin(p["r"])
+        rep, _ = ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=model, hash_tables=False)
+        torch.cuda.synchronize()
+        g_s = time.perf_counter() - t0
+        grep, _ = ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=model, hash_tables=True)
+        entry = {"workload": describe(p), "gpu_s": g_s}
+        for t in tlist:
+            oracle.set_threads(t)
+            t0 = time.perf_counter()
+            o = oracle.Oracle.from_arrays(x, y, fc, b)
+            orep = o.mine(p["r"], p["theta"], max_k=p["max_k"], hash_tables=True)
+            entry[f"cpu_s_{t}t"] = time.perf_counter() - t0
+            entry["identical"] = entry.get("identical", True) and orep == grep
+            del o
+        out[name] = entry
+        log(f"cpu vs gpu {name}: " + ", ".join(f"{k} {v:.3f}" for k, v in entry.items() if k.endswith("_s") or "_s_" in k)
+            + f", identical {entry['identical']}")
+        ctx.close()
+    return out
 
 
 def main():
@@ -96,7 +168,7 @@ def main():
     ap.add_argument("--workload", default="c5")
     ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the persistent and plain schedules")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--model", default=os.path.join(ROOT, "gpu-colocationmininig_amd", "models", "cost_mlp.bin"))
     ap.add_argument("--no-model", action="store_true")
@@ -119,13 +191,16 @@ def main():
                                    "note only)", "n_gpus": args.gpus}))
         return 2
 
-    x, y, fc, bounds, p = prepare(args)
+    x, y, fc, bounds, p = gen(args.workload, N=args.N)
+    log(f"generated {describe(p)}")
 
     import colo
     import torch
 
     ctx = colo.Context(x, y, fc, bounds, p["r"])
+    ctx.set_verbose(1)
     torch.cuda.synchronize()
+    log("dataset resident and binned")
 
     model = None
     if not args.no_model and os.path.exists(args.model):
@@ -135,21 +210,26 @@ def main():
                   colo.COLO_SCHED_PLAIN: "plain"}
     thetas = p["thetas"]
 
-    tune = ctx.tune
-    if not args.no_tune:
+    def tune_for(sched, m):
+        if args.no_tune:
+            return ctx.tune
         t0 = time.perf_counter()
         os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
-        tune = ctx.autotune(min(thetas), max_k=p["max_k"], sched=main_sched, model=model, cache=args.tune_cache)
-        log(f"autotuned launch shapes in {time.perf_counter() - t0:.1f} s: {tune}")
+        t = ctx.autotune(min(thetas), max_k=p["max_k"], sched=sched, model=m, cache=args.tune_cache)
+        log(f"autotuned {sched_name[sched]} launch shapes in {time.perf_counter() - t0:.1f} s: {t}")
+        return t
+
+    default_tune = ctx.tune
+    tune = tune_for(main_sched, model)
 
     bin_ms = []
 
-    def step(sched):
+    def step(sched, hash_tables=False):
         t0 = time.perf_counter()
         ctx.rebin(p["r"])
         torch.cuda.synchronize()
         bin_ms.append((time.perf_counter() - t0) * 1000.0)
-        out = ctx.mine_sweep(thetas, max_k=p["max_k"], sched=sched, hash_tables=False,
+        out = ctx.mine_sweep(thetas, max_k=p["max_k"], sched=sched, hash_tables=hash_tables,
                              model=model if sched == colo.COLO_SCHED_LEARNED else None)
         return [o[0] for o in out], [o[1] for o in out]
 
@@ -189,8 +269,14 @@ def main():
         "bin": sum(bin_ms[args.warmup:args.warmup + args.steps]) / args.steps,
         **{k: sum(s[k] for s in stats) / args.steps for k in ("filter_ms", "model_ms", "count_ms", "scan_ms",
                                                                "write_ms", "host_ms", "total_ms")}}
-    out["patterns"] = {f"{t:g}": s["patterns"] for t, s in zip(thetas, sts)}
-    out["levels"] = {f"{t:g}": s["levels"] for t, s in zip(thetas, sts)}
+    first = stats[:len(thetas)]
+    out["patterns"] = {f"{t:g}": s["patterns"] for t, s in zip(thetas, first)}
+    out["levels"] = {f"{t:g}": s["levels"] for t, s in zip(thetas, first)}
+    out["level_ms"] = {f"{t:g}": {str(k): round(v, 2) for k, v in enumerate(s["level_ms"]) if v > 0}
+                       for t, s in zip(thetas, first)}
+    out["memory"] = {"pool_peak_gb": max(s["pool_peak_bytes"] for s in stats) / 1e9,
+                     "slots_deferred": sum(s["slots_deferred"] for s in first),
+                     "recounts": sum(s["recounts"] for s in first)}
 
     # dominant kernel by HIP-event time: level-2 or level-k count or write
     def tot(key):
@@ -211,42 +297,68 @@ def main():
     tf = os.path.join(ROOT, "profiles", f"traffic_{name}_{args.workload}.json")
     if os.path.exists(tf) and args.N is None:
         with open(tf) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            tj = json.load(fh)
+        if tj.get("workload_key") == describe(p):
+            traffic = tj.get("hbm_bytes_per_launch")
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": name,
                        "launches": n, "avg_launch_ms": ms / n if n else None,
                        "algorithmic_bytes_per_launch": by / n if n else None}
-    cnt_ms, wr_ms = tot("count_kernel_ms"), tot("write_kernel_ms")
-    out["kernels_ms_per_step"] = {"count": cnt_ms / args.steps, "write": wr_ms / args.steps,
-                                  "filter": sum(s["filter_kernel_ms"] for s in stats) / args.steps,
-                                  "pi": sum(s["pi_kernel_ms"] for s in stats) / args.steps}
+    out["kernels_ms_per_step"] = {kn: v[0] / args.steps for kn, v in kern.items()}
+    out["kernels_ms_per_step"].update({"filter": tot("filter_kernel_ms") / args.steps,
+                                       "pi": tot("pi_kernel_ms") / args.steps})
 
     if not args.no_plain:
-        identical = True
-        others = [colo.COLO_SCHED_PLAIN] + ([colo.COLO_SCHED_PERSISTENT] if main_sched != colo.COLO_SCHED_PERSISTENT
-                                            else [])
-        for sc in others:
-            step(sc)  # warm
+        main_s = ms_per_step / 1000.0
+        # persistent with the launch shapes tuned for itself
+        if main_sched != colo.COLO_SCHED_PERSISTENT:
+            ctx.tune = default_tune
+            ptune = tune_for(colo.COLO_SCHED_PERSISTENT, None)
+            step(colo.COLO_SCHED_PERSISTENT)  # warm
+            reps = min(args.steps, 3)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            reps = 1 if sc == colo.COLO_SCHED_PLAIN else args.steps
             for _ in range(reps):
-                rep_o, _ = step(sc)
+                step(colo.COLO_SCHED_PERSISTENT)
             torch.cuda.synchronize()
-            out["mining_s"][sched_name[sc]] = (time.perf_counter() - t0) / reps
-            identical &= rep_o == rep
-            log(f"{sched_name[sc]}: {out['mining_s'][sched_name[sc]]:.3f} s per step")
-        main_s = ms_per_step / 1000.0
-        out["mining_s"][f"{sched_name[main_sched]}_speedup_vs_plain"] = out["mining_s"]["plain"] / main_s
-        if "persistent" in out["mining_s"] and main_sched != colo.COLO_SCHED_PERSISTENT:
+            out["mining_s"]["persistent"] = (time.perf_counter() - t0) / reps
+            out["mining_s"]["persistent_launch_shapes"] = ptune
             out["mining_s"]["learned_speedup_vs_persistent"] = out["mining_s"]["persistent"] / main_s
+            log(f"persistent (own shapes): {out['mining_s']['persistent']:.3f} s per step")
+            ctx.tune = tune
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step(colo.COLO_SCHED_PLAIN)
+        torch.cuda.synchronize()
+        out["mining_s"]["plain"] = time.perf_counter() - t0
+        out["mining_s"][f"{sched_name[main_sched]}_speedup_vs_plain"] = out["mining_s"]["plain"] / main_s
+        log(f"plain: {out['mining_s']['plain']:.3f} s per step")
+        # one sweep per schedule with table checksums: reports must agree
+        ref, _ = step(main_sched, hash_tables=True)
+        identical = True
+        for sc in (colo.COLO_SCHED_PERSISTENT, colo.COLO_SCHED_PLAIN):
+            if sc == main_sched:
+                continue
+            r2, _ = step(sc, hash_tables=True)
+            identical &= r2 == ref
         out["reports_identical"] = identical
+        out["reports_checked"] = "learned, persistent and plain sweeps with instance-table checksums"
+        log(f"schedules identical (with table checksums): {identical}")
         if not identical:
             log("ERROR: schedules produced different reports")
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(x, y, fc, bounds, p, args.cpu_budget, threads)
+        threads = cpu_threads()
+        log(f"cpu baseline on {threads} threads")
+        info = cpu_info()
+        cb = cpu_level2_sample(x, y, fc, bounds, p, args.cpu_budget, threads)
+        cb.update(info)
+        # the GPU's whole step against the CPU's level-2 joins of a fraction
+        # of the pairs: a lower bound on the CPU's mining time of the workload
+        cb["cpu_mining_s_lower_bound"] = cb["sample_s"]
+        cb["gpu_step_s"] = ms_per_step / 1000.0
+        out["cpu_baseline"] = cb
+        out["cpu_vs_gpu_mining"] = cpu_vs_gpu_mining(colo, model, threads)
+    log("done")
     print(json.dumps(out), flush=True)
     return 0
 

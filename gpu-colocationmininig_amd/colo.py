@@ -51,7 +51,7 @@ TUNE_UNSET = 0xFFFFFFFF
 
 class Tune(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("count_var", "write_var", "blocks_per_cu", "item_rows")] + \
-               [("count_var_k", C.c_uint32 * 16), ("item_rows_k", C.c_uint32 * 16)]
+               [("count_var_k", C.c_uint32 * 16), ("item_rows_k", C.c_uint32 * 16), ("slots_budget_kb", C.c_uint32)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -71,7 +71,8 @@ class MineStats(C.Structure):
                                           "l2_write_bytes", "model_abs_err", "model_n", "arena_bytes")] + \
                [(n, C.c_uint64) for n in ("overflows", "arenas")] + \
                [(n, C.c_double) for n in ("model_ms", "alloc_ms", "alloc_bytes")] + [("level_ms", C.c_double * 16)] + \
-               [("level_count_kernel_ms", C.c_double * 16)]
+               [("level_count_kernel_ms", C.c_double * 16), ("pool_peak_bytes", C.c_double),
+                ("slots_deferred", C.c_uint64), ("recounts", C.c_uint64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -83,7 +84,7 @@ class MineStats(C.Structure):
 _lib = None
 
 SYMBOLS = [
-    "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_get_tune", "colo_set_tune", "colo_autotune", "colo_set_graphs", "colo_graph_info", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
+    "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_set_verbose", "colo_get_tune", "colo_set_tune", "colo_autotune", "colo_set_graphs", "colo_graph_info", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
     "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_mine_sweep", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_load_csv_opts", "colo_host_dataset", "colo_host_label", "colo_host_free",
@@ -104,6 +105,7 @@ def lib():
         L.colo_ctx_destroy.argtypes = [vp]
         L.colo_trim.argtypes = [vp]
         L.colo_set_graphs.argtypes = [vp, C.c_int]
+        L.colo_set_verbose.argtypes = [vp, C.c_int]
         L.colo_graph_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.colo_get_tune.argtypes = [vp, C.POINTER(Tune)]
         L.colo_set_tune.argtypes = [vp, C.POINTER(Tune)]
@@ -256,6 +258,10 @@ class Context:
         if getattr(self, "h", None):
             lib().colo_ctx_destroy(self.h)
             self.h = None
+
+    def set_verbose(self, level):
+        """level > 0: progress lines on stderr (autotuner measurements)."""
+        self._check(lib().colo_set_verbose(self.h, int(level)))
 
     def set_graphs(self, on):
         """Binning replayed from a captured HIP graph (on) or launched eagerly."""

@@ -935,7 +935,7 @@ __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32
         },
         [] { return true; });
     acc.flush();
-    slots[t] = cnt;
+    if (slots) slots[t] = cnt;
     if (cnt) set_bit(bits_a, t);
     return cnt;
 }
@@ -1365,7 +1365,7 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
             test(q, D.sxy[q].x, D.sxy[q].y);
         }
         acc.flush();
-        if (act) slots[cd.slots_off + i] = cnt;
+        if (act && cd.slots_off != kNoSlots) slots[cd.slots_off + i] = cnt;
         if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
         if (WAVE) {
             if (__ballot(cnt != 0)) wave_member_bits<kArr>(bt, cd.seg_off, rk, km1, cnt != 0);
@@ -1446,8 +1446,10 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
             test(q, D.sxy[q].x, D.sxy[q].y);
         }
         acc.flush();
-        slots[cd.slots_off + i] = cnt0;
-        slots[cd.slots_off + i + 1] = cnt1;
+        if (cd.slots_off != kNoSlots) {
+            slots[cd.slots_off + i] = cnt0;
+            slots[cd.slots_off + i + 1] = cnt1;
+        }
         if (cnt0 + cnt1) atomicAdd(&s_rows[j], (unsigned long long)(cnt0 + cnt1));
         // the two rows share the anchor: its bit is set once
         if (cnt0) set_member_bits<kArr>(bt, cd.seg_off, rk0, km1);
@@ -1562,8 +1564,10 @@ __device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *desc
         walk(a0, a1, act0, shared);
         walk(e0, e1, false, own1);
         acc.flush();
-        if (act0) slots[cd.slots_off + i] = cnt0;
-        if (act1) slots[cd.slots_off + i + 1] = cnt1;
+        if (cd.slots_off != kNoSlots) {
+            if (act0) slots[cd.slots_off + i] = cnt0;
+            if (act1) slots[cd.slots_off + i + 1] = cnt1;
+        }
         if (cnt0 + cnt1) atomicAdd(&s_rows[j], (unsigned long long)(cnt0 + cnt1));
         if (__ballot(cnt0 + cnt1 != 0)) {
             wave_member_bits<kArr>(bt, cd.seg_off, rk0, km1, cnt0 != 0);
@@ -1653,7 +1657,7 @@ __device__ inline void joink_count_rowsR(const DevData &D, const CandDesc *descs
         uint32_t tot = 0;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            slots[cd.slots_off + i + r] = cnt[r];
+            if (cd.slots_off != kNoSlots) slots[cd.slots_off + i + r] = cnt[r];
             tot += cnt[r];
         }
         if (tot) atomicAdd(&s_rows[j], (unsigned long long)tot);
@@ -1731,7 +1735,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
             uint32_t *bt = WRITE ? nullptr : bits + cd.bits_off;
             const uint32_t *wsl = WRITE ? (cd.wslots ? cd.wslots : slots + cd.slots_off) : nullptr;
             if (m == 0) {
-                if (!WRITE && lane < nrows) slots[cd.slots_off + r0 + gs + lane] = 0;
+                if (!WRITE && lane < nrows && cd.slots_off != kNoSlots) slots[cd.slots_off + r0 + gs + lane] = 0;
                 continue;
             }
             if (m <= 32) {
@@ -1762,7 +1766,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                     if (!WRITE) {
                         const uint32_t cnt = (uint32_t)__popcll(bal & segm);
                         if (valid && e == 0) {
-                            slots[cd.slots_off + row] = cnt;
+                            if (cd.slots_off != kNoSlots) slots[cd.slots_off + row] = cnt;
                             if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
                             if (cnt)
                                 for (uint32_t mm = 0; mm < km1; mm++)
@@ -1862,7 +1866,7 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
                 }
                 if (!WRITE) {
                     if (lane == 0) {
-                        slots[cd.slots_off + row] = cnt;
+                        if (cd.slots_off != kNoSlots) slots[cd.slots_off + row] = cnt;
                         if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
                     }
                     if (cnt && lane < km1) set_bit(bt + cd.seg_off[lane], rp[lane] - D.fstart[cd.f[lane]]);
@@ -1886,7 +1890,7 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         const CandDesc &cd = desc[wi.cand];
-        uint32_t *sl = slots + cd.slots_off;
+        uint32_t *sl = cd.slots_off != kNoSlots ? slots + cd.slots_off : nullptr;
         uint32_t *bt = bits + cd.bits_off;
         const uint32_t a = cd.f[0], b = cd.f[1];
         uint64_t sum = 0;
@@ -2519,6 +2523,8 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
     }
     if (!rc && tot != c->n) rc = set_error(c, COLO_EARG, "sum of fcount != n");
     c->minX = ds->minX; c->minY = ds->minY; c->maxX = ds->maxX; c->maxY = ds->maxY;
+    using Clock = std::chrono::steady_clock;
+    auto t_up = Clock::now();
     if (!rc) {
         const uint64_t n = c->n;
         if (!rc) rc = alloc_dev(c, (void **)&c->d_x, 4 * n);
@@ -2544,7 +2550,11 @@ int colo_ctx_create(const colo_dataset *ds, float r, void *stream, colo_ctx **ou
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) rc = set_error(c, COLO_EHIP, hipGetErrorString(e));
         }
+        c->upload_ms = std::chrono::duration<double, std::milli>(Clock::now() - t_up).count();
+        auto t_bin = Clock::now();
         if (!rc) rc = colo_bin(c, r, nullptr);
+        if (!rc) rc = hipStreamSynchronize(s) == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "binning");
+        c->bin_ms = std::chrono::duration<double, std::milli>(Clock::now() - t_bin).count();
     }
     *out = c;  // returned even on error so colo_last_error can be read
     return rc;
@@ -2563,6 +2573,12 @@ int colo_ctx_destroy(colo_ctx *c) {
     c->pool.release();
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return COLO_OK;
+}
+
+int colo_set_verbose(colo_ctx *c, int level) {
+    if (!c) return COLO_EARG;
+    c->verbose = level;
     return COLO_OK;
 }
 
@@ -2589,6 +2605,7 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
     o->write_var = c->tune.write_var;
     o->blocks_per_cu = c->tune.blocks_per_cu;
     o->item_rows = c->tune.item_rows;
+    o->slots_budget_kb = c->tune.slots_budget_kb;
     return COLO_OK;
 }
 
@@ -2609,6 +2626,7 @@ int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     c->tune.write_var = t->write_var;
     c->tune.blocks_per_cu = t->blocks_per_cu;
     c->tune.item_rows = t->item_rows;
+    c->tune.slots_budget_kb = t->slots_budget_kb;
     return COLO_OK;
 }
 

@@ -40,13 +40,18 @@ struct DevData {
     const uint32_t *fcount;        // F
 };
 
+// slots_off of a candidate whose count pass keeps no per-row slots (its
+// rows and participation bits are still counted; a table built for it later
+// is recounted with slots).
+constexpr uint64_t kNoSlots = ~0ull;
+
 // One candidate of a batched (persistent) level launch.
 struct CandDesc {
     uint32_t k;
     uint32_t f[kMaxK];
     uint64_t n_anchor;             // N_{p0} at level 2, parent rows at level k
     const uint32_t *parent;        // level k: parent table (n_anchor x (k-1))
-    uint64_t slots_off;            // offset of this candidate's slots in the arena
+    uint64_t slots_off;            // offset of this candidate's slots in the arena, or kNoSlots
     uint64_t bits_off;             // word offset of the bitmap in the bits arena
     uint64_t seg_off[kMaxK];       // word offset of segment j inside the bitmap
     const uint64_t *offsets;       // write pass: row offsets (n_anchor), global
@@ -94,6 +99,7 @@ struct SegOut {
 struct DevPool {
     std::multimap<size_t, void *> free_blocks;
     size_t cached = 0;
+    size_t in_use = 0, peak = 0;  // bytes handed out and not yet returned; their maximum
     double sys_ms = 0, sys_bytes = 0;  // time in hipMalloc/hipFree, bytes allocated
     struct Timer {
         double &acc;
@@ -101,17 +107,21 @@ struct DevPool {
         explicit Timer(double &a) : acc(a) {}
         ~Timer() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
     };
-    size_t cap = 0;  // bytes kept cached at most: 3/4 of the device's memory
+    size_t cap = 0, total = 0;  // bytes kept cached at most (3/4 of the device's memory); device memory
     size_t cap_bytes() {
-        if (!cap) {
+        if (!cap) cap = total_bytes() / 4 * 3;
+        return cap;
+    }
+    size_t total_bytes() {
+        if (!total) {
             size_t fr = 0, tot = 0;
             if (hipMemGetInfo(&fr, &tot) != hipSuccess || !tot) {
                 (void)hipGetLastError();
                 tot = 64ull << 30;
             }
-            cap = tot / 4 * 3;
+            total = tot;
         }
-        return cap;
+        return total;
     }
     static size_t round(size_t b) {
         if (b >= (2u << 20)) return (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
@@ -127,6 +137,7 @@ struct DevPool {
             *got = it->first;
             cached -= it->first;
             free_blocks.erase(it);
+            note_out(*got);
             return p;
         }
         void *p = nullptr;
@@ -148,10 +159,16 @@ struct DevPool {
         }
         sys_bytes += (double)want;
         *got = want;
+        note_out(want);
         return p;
+    }
+    void note_out(size_t b) {
+        in_use += b;
+        if (in_use > peak) peak = in_use;
     }
     void free(void *p, size_t bytes) {
         if (!p) return;
+        in_use -= bytes < in_use ? bytes : in_use;
         if (cached + bytes > cap_bytes()) {
             Timer tm(sys_ms);
             (void)hipFree(p);
@@ -192,6 +209,7 @@ struct TuneParams {
     uint32_t write_var = 0;       // level-k write: same choice
     uint32_t blocks_per_cu = 8;   // persistent grid: workgroups per CU
     uint32_t item_rows = 1024;    // rows (anchors) per work item without the cost model
+    uint32_t slots_budget_kb = 0; // slots arena limit per level pass, KiB (0 = 1/16 of device memory)
 };
 
 struct colo_ctx {
@@ -234,6 +252,8 @@ struct colo_ctx {
     uint32_t graph_captures = 0, graph_replays = 0;
     std::vector<uint64_t> pairwork; // F x F 3x3 join work per (first, last), filled on first use
     colo::DevPool pool;             // miner buffers (tables, arenas, slots, offsets)
+    double upload_ms = 0, bin_ms = 0; // colo_ctx_create: host-to-device copy; first binning
+    int verbose = 0;                // progress lines on stderr (colo_set_verbose)
     std::string err;
     colo::DevData dev() const;
 };

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <thread>
@@ -246,14 +247,16 @@ struct Miner {
 
     // Algorithmic bytes (each datum once): level 2 count reads both features'
     // coordinates (8 B each), writes 4 B of slots per anchor and one bit per
-    // instance; level-k count reads the parent rows (4(k-1) B), the last
-    // feature's coordinates and writes slots and bits; writes add the
-    // offsets (8 B) and the table (4k B per row).
-    double count_bytes(uint32_t k, const uint32_t *cand, uint64_t n_anchor) const {
+    // instance; level-k count reads the last feature's coordinates and
+    // writes slots (when kept) and bits -- its parent rows (4(k-1) B per
+    // row) are charged once per run of candidates sharing them, by the
+    // caller; writes add the offsets (8 B) and the table (4k B per row).
+    double count_bytes(uint32_t k, const uint32_t *cand, uint64_t n_anchor, bool slots_kept = true) const {
         double Nsum = 0;
         for (uint32_t j = 0; j < k; j++) Nsum += c->fcount[cand[j]];
-        if (k == 2) return 8.0 * Nsum + 4.0 * n_anchor + Nsum / 8.0;
-        return 4.0 * (k - 1) * n_anchor + 8.0 * c->fcount[cand[k - 1]] + 4.0 * n_anchor + Nsum / 8.0;
+        const double sl = slots_kept ? 4.0 * n_anchor : 0.0;
+        if (k == 2) return 8.0 * Nsum + sl + Nsum / 8.0;
+        return 8.0 * c->fcount[cand[k - 1]] + sl + Nsum / 8.0;
     }
     double write_bytes(uint32_t k, const uint32_t *cand, uint64_t n_anchor, uint64_t rows) const {
         if (k == 2) return 8.0 * (c->fcount[cand[0]] + c->fcount[cand[1]]) + 12.0 * n_anchor + 8.0 * rows;
@@ -301,10 +304,25 @@ struct Miner {
         fclose(fp);
     }
 
-    static int64_t find(const std::vector<Prev> &L, const uint32_t *v, uint32_t w) {
-        for (size_t i = 0; i < L.size(); i++)
-            if (std::equal(v, v + w, L[i].pat.begin())) return (int64_t)i;
-        return -1;
+    // getIndex (cpp:703-718) as a hash lookup: position of pattern v (w ids)
+    // in the previous level's prevalent list, -1 if absent.  The index is
+    // rebuilt whenever the run moves to the next level (index_prev).
+    std::unordered_map<std::vector<uint32_t>, int64_t, PatSet::H> prev_index_{16, PatSet::H{0}};
+    const std::vector<Prev> *prev_indexed_ = nullptr;
+    void index_prev(const std::vector<Prev> &L) {
+        prev_index_.clear();
+        prev_index_.reserve(L.size() * 2 + 1);
+        for (size_t i = 0; i < L.size(); i++) prev_index_.emplace(L[i].pat, (int64_t)i);
+        prev_indexed_ = &L;
+    }
+    int64_t find(const std::vector<Prev> &L, const uint32_t *v, uint32_t w) const {
+        if (prev_indexed_ != &L || prev_index_.size() != L.size()) {  // not the indexed list
+            for (size_t i = 0; i < L.size(); i++)
+                if (std::equal(v, v + w, L[i].pat.begin())) return (int64_t)i;
+            return -1;
+        }
+        auto it = prev_index_.find(std::vector<uint32_t>(v, v + w));
+        return it == prev_index_.end() ? -1 : it->second;
     }
 
     // Per-candidate result of the count pass.
@@ -456,7 +474,7 @@ struct Miner {
     int upload_items(const std::vector<WorkItem> &items, const std::vector<double> *w) {
         std::vector<uint64_t> seg = xcd_segments(items, w);
         size_t ib = items.size() * sizeof(WorkItem);
-        if (!d_items.ensure(ib + 9 * 8)) return oom("work items", ib + 9 * 8);
+        if (!ensure_buf(d_items, ib + 9 * 8)) return oom("work items", ib + 9 * 8);
         COLO_HIP(hipMemcpyAsync(d_items.p, items.data(), ib, hipMemcpyHostToDevice, s));
         COLO_HIP(hipMemcpyAsync((char *)d_items.p + ib, seg.data(), 9 * 8, hipMemcpyHostToDevice, s));
         d_seg_ = (const uint64_t *)((const char *)d_items.p + ib);
@@ -516,6 +534,35 @@ struct Miner {
             d.bits_off = bits_tot;
             bits_tot += words;
         }
+        // The slots arena holds one u32 per parent row and candidate, but
+        // only the tables built afterwards read slots.  Above the budget the
+        // largest candidates keep none (their rows and bits are still
+        // counted); a table built for one of them is recounted with slots
+        // (recount_missing_slots).
+        const uint64_t budget = force_slots_ ? ~0ull : slots_budget_words();
+        if (slots_tot > budget) {
+            std::vector<uint64_t> ord(m);
+            for (uint64_t u = 0; u < m; u++) ord[u] = u;
+            std::stable_sort(ord.begin(), ord.end(),
+                             [&](uint64_t a, uint64_t b) { return desc[a].n_anchor < desc[b].n_anchor; });
+            std::vector<uint8_t> keep(m, 0);
+            uint64_t kept = 0;
+            for (uint64_t u : ord)
+                if (kept + desc[u].n_anchor <= budget) {
+                    kept += desc[u].n_anchor;
+                    keep[u] = 1;
+                }
+            slots_tot = 0;
+            for (uint64_t u = 0; u < m; u++) {
+                if (keep[u]) {
+                    desc[u].slots_off = slots_tot;
+                    slots_tot += desc[u].n_anchor;
+                } else {
+                    desc[u].slots_off = kNoSlots;
+                    st.slots_deferred++;
+                }
+            }
+        }
         // work queue: candidates sharing a parent table form one item stream;
         // costliest runs first so the tail of the launch is short.  Without
         // a model a run costs its rows x candidates and is cut into
@@ -536,37 +583,52 @@ struct Miner {
                                                        const std::pair<uint64_t, uint32_t> &b) {
             return cost[a.first] > cost[b.first];
         });
+        // work items of rows [begin, end) x one run, for an item size of
+        // item_rows; with the model, cut at equal predicted cost
         std::vector<WorkItem> items;
         std::vector<double> iw;  // predicted cost of each item (learned schedule)
-        if (pred) {
-            double total = 0;
-            for (auto &r : runs) total += cost[r.first];
-            // about 8 items per workgroup at equal predicted cost, each
-            // between 1/4 and 8x the tuned item size
-            double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 8));
-            const double lo = std::max(64.0, c->tune.item_rows_of(k) / 4.0), hi = c->tune.item_rows_of(k) * 8.0;
-            for (auto &r : runs) {
-                uint64_t n = desc[r.first].n_anchor;
-                double per_row = n ? cost[r.first] / (double)n : 1.0;
-                uint64_t chunk = (uint64_t)std::min(hi, std::max(lo, target / std::max(per_row, 1e-9)));
-                size_t before = items.size();
-                make_items(r.first, n, chunk, items, r.second);
-                for (size_t i = before; i < items.size(); i++) iw.push_back(per_row * (items[i].end - items[i].begin));
+        auto build_items = [&](uint32_t item_rows) {
+            items.clear();
+            iw.clear();
+            if (pred) {
+                double total = 0;
+                for (auto &r : runs) total += cost[r.first];
+                // about 8 items per workgroup at equal predicted cost, each
+                // between 1/4 and 8x the tuned item size
+                double target = std::max(1.0, total / (double)(num_cus() * c->tune.blocks_per_cu * 8));
+                const double lo = std::max(64.0, item_rows / 4.0), hi = item_rows * 8.0;
+                for (auto &r : runs) {
+                    uint64_t n = desc[r.first].n_anchor;
+                    double per_row = n ? cost[r.first] / (double)n : 1.0;
+                    uint64_t chunk = (uint64_t)std::min(hi, std::max(lo, target / std::max(per_row, 1e-9)));
+                    size_t before = items.size();
+                    make_items(r.first, n, chunk, items, r.second);
+                    for (size_t i = before; i < items.size(); i++) iw.push_back(per_row * (items[i].end - items[i].begin));
+                }
+            } else {
+                for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, item_rows, items, r.second);
             }
-        } else {
-            for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows_of(k), items, r.second);
-        }
+        };
+        build_items(c->tune.item_rows_of(k));
         slots_arena = std::make_shared<DevBuf>(&c->pool);
-        if (!slots_arena->ensure(std::max<uint64_t>(slots_tot, 1) * 4)) return oom("slots arena", std::max<uint64_t>(slots_tot, 1) * 4);
-        if (!arena_bits.ensure(std::max<uint64_t>(bits_tot, 1) * 4)) return oom("bits arena", std::max<uint64_t>(bits_tot, 1) * 4);
-        if (!d_desc.ensure(m * sizeof(CandDesc)) || !d_queue.ensure(64))
+        if (!ensure_buf(*slots_arena, std::max<uint64_t>(slots_tot, 1) * 4))
+            return oom("slots arena", std::max<uint64_t>(slots_tot, 1) * 4);
+        if (!ensure_buf(arena_bits, std::max<uint64_t>(bits_tot, 1) * 4))
+            return oom("bits arena", std::max<uint64_t>(bits_tot, 1) * 4);
+        if (!ensure_buf(d_desc, m * sizeof(CandDesc)) || !ensure_buf(d_queue, 64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), m * sizeof(CandDesc), hipMemcpyHostToDevice, s));
         {
             int rc0 = upload_items(items, pred ? &iw : nullptr);
             if (rc0) return rc0;
         }
-        if (!d_rowsel.ensure((m + 1) * 8)) return oom("row totals", (m + 1) * 8);
+        if (!ensure_buf(d_rowsel, (m + 1) * 8)) return oom("row totals", (m + 1) * 8);
+        if (tuning_ && !force_slots_) {
+            int rc0 = tune_level(k, items, iw, pred != nullptr, build_items, slots_arena->as<uint32_t>(), bits_tot, m);
+            if (rc0) return rc0;
+            build_items(c->tune.item_rows_of(k));
+            if ((rc0 = upload_items(items, pred ? &iw : nullptr))) return rc0;
+        }
         auto t0 = Clock::now();
         int rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
         if (!rc) rc = fill_u32(d_rowsel.as<uint32_t>(), 2 * m, 0, s);
@@ -590,8 +652,12 @@ struct Miner {
             }
         }
         {
+            // a run's candidates read their shared parent rows once
             double by = 0;
-            for (uint64_t u = 0; u < m; u++) by += count_bytes(k, &C[idx[u] * k], desc[u].n_anchor);
+            for (uint64_t u = 0; u < m; u++)
+                by += count_bytes(k, &C[idx[u] * k], desc[u].n_anchor, desc[u].slots_off != kNoSlots);
+            if (k > 2)
+                for (auto &r : runs) by += 4.0 * (k - 1) * desc[r.first].n_anchor;
             st.count_bytes += by;
             if (k == 2) st.l2_count_bytes += by;
             st.count_launches++;
@@ -604,7 +670,7 @@ struct Miner {
         // PI segments of every candidate, one launch
         std::vector<SegDesc> segs(m * k);
         for (uint64_t u = 0; u < m; u++) seg_descs(c, k, &C[idx[u] * k], desc[u].bits_off, &segs[u * k], true);
-        if (!d_segs.ensure(segs.size() * sizeof(SegDesc)) || !d_segout.ensure(segs.size() * sizeof(SegOut)))
+        if (!ensure_buf(d_segs, segs.size() * sizeof(SegDesc)) || !ensure_buf(d_segout, segs.size() * sizeof(SegOut)))
             return set_error(c, COLO_ENOMEM, "segments");
         COLO_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
         size_t e1 = mark();
@@ -628,7 +694,7 @@ struct Miner {
             if (r.prevalent) {
                 Prev p;
                 p.pat.assign(cand, cand + k);
-                p.slots = slots_arena;
+                if (desc[u].slots_off != kNoSlots) p.slots = slots_arena;  // else recounted if its table is built
                 p.slots_off = desc[u].slots_off;
                 p.nslots = desc[u].n_anchor;
                 p.rows = r.rows;
@@ -659,7 +725,8 @@ struct Miner {
         for (size_t i = 0; i < L.size(); i++)
             if (build[i]) owords += L[i].nslots + 1;
         arena_offs = std::make_shared<DevBuf>(&c->pool);
-        if (!arena_offs->ensure(std::max<uint64_t>(owords, 1) * 8)) return oom("offsets arena", std::max<uint64_t>(owords, 1) * 8);
+        if (!ensure_buf(*arena_offs, std::max<uint64_t>(owords, 1) * 8))
+            return oom("offsets arena", std::max<uint64_t>(owords, 1) * 8);
         std::vector<uint64_t> offs_at(L.size(), 0);
         {
             std::vector<ScanSeg> segs;
@@ -679,6 +746,7 @@ struct Miner {
             arena = arena_reserved_;      // pre-sized before the count pass
         } else {
             if (arena_reserved_) st.overflows++;  // prediction too small: exact size
+            arena_reserved_.reset();               // given back before the exact allocation
             arena = std::make_shared<DevBuf>(&c->pool);
             if (!arena->ensure(std::max<uint64_t>(words, 1) * 4)) return oom("table arena", std::max<uint64_t>(words, 1) * 4);
         }
@@ -723,7 +791,7 @@ struct Miner {
         if (desc.empty()) return COLO_OK;
         for (auto &r : parent_runs(desc, k))
             make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows_of(k), items, r.second);
-        if (!d_desc.ensure(desc.size() * sizeof(CandDesc)) || !d_queue.ensure(64))
+        if (!ensure_buf(d_desc, desc.size() * sizeof(CandDesc)) || !ensure_buf(d_queue, 64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
         {
@@ -750,8 +818,136 @@ struct Miner {
         return COLO_OK;
     }
 
+    // Online tuning of a level's count pass (the autotuner's per-level
+    // step): every (count variant, item size) pair is timed with HIP events
+    // on a sample of the level's work items -- blocks of 16 consecutive
+    // items, one block in 8, so neighbouring items still share their L2 --
+    // and the fastest pair is kept for the level in c->tune.  The sample
+    // launches write into the level's slots, bits and row totals, which the
+    // real pass clears and rewrites: results do not depend on the tuning.
+    bool tuning_ = false;
+    int tune_level(uint32_t k, std::vector<WorkItem> &items, std::vector<double> &iw, bool has_pred,
+                   const std::function<void(uint32_t)> &build, uint32_t *d_slots, uint64_t bits_tot, uint64_t m) {
+        if (k >= 16) return COLO_OK;
+        static const uint32_t vars[] = {0, 3, 4, 5, 6, 7, 8, 9}, sizes[] = {1024, 4096, 16384};
+        const uint32_t nv = k == 2 ? 1 : 8;  // level 2 has one count kernel
+        hipEvent_t ev[2];
+        COLO_HIP(hipEventCreate(&ev[0]));
+        if (hipEventCreate(&ev[1]) != hipSuccess) {
+            (void)hipEventDestroy(ev[0]);
+            return set_error(c, COLO_EHIP, "event");
+        }
+        double best = 1e300;
+        uint32_t bv = c->tune.count_var_of(k), bs = c->tune.item_rows_of(k);
+        int rc = COLO_OK;
+        for (uint32_t si = 0; si < 3 && !rc; si++) {
+            build(sizes[si]);
+            std::vector<WorkItem> sub;
+            std::vector<double> subw;
+            for (size_t i = 0; i < items.size(); i++)
+                if ((i / 16) % 8 == 0) {
+                    sub.push_back(items[i]);
+                    if (has_pred) subw.push_back(iw[i]);
+                }
+            if ((rc = upload_items(sub, has_pred ? &subw : nullptr))) break;
+            for (uint32_t vi = 0; vi < nv && !rc; vi++) {
+                if (k > 2) c->tune.count_var_k[k] = vars[vi];
+                rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
+                if (!rc) rc = fill_u32(d_rowsel.as<uint32_t>(), 2 * m, 0, s);
+                if (rc) break;
+                float ms = 0;
+                if (hipEventRecord(ev[0], s) != hipSuccess) { rc = set_error(c, COLO_EHIP, "event"); break; }
+                rc = count_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, sub.size(), d_slots,
+                                 arena_bits.as<uint32_t>(), d_rowsel.as<uint64_t>(), d_queue.as<uint32_t>(), s);
+                if (rc) break;
+                if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+                    rc = set_error(c, COLO_EHIP, "tuning launch");
+                    break;
+                }
+                if (c->verbose)
+                    fprintf(stderr, "[colo tune] level %u count var %u, %u-row items: %.2f ms on %zu of %zu items\n", k,
+                            k > 2 ? vars[vi] : 0u, sizes[si], ms, sub.size(), items.size());
+                if (ms < best) {
+                    best = ms;
+                    bv = vars[vi];
+                    bs = sizes[si];
+                }
+            }
+        }
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        if (rc) return rc;
+        if (k > 2) c->tune.count_var_k[k] = bv;
+        c->tune.item_rows_k[k] = bs;
+        return COLO_OK;
+    }
+
+    // Recounts, with slots, the candidates whose tables are to be built but
+    // whose count pass kept no slots (deferred above the slots budget, or
+    // results remembered from an earlier threshold of a sweep).  The
+    // recount's results equal the first count's; only its slots are used.
+    int recount_missing_slots(uint32_t k, const std::vector<uint32_t> &C, std::vector<Prev> &L,
+                              const std::vector<uint8_t> &build, const std::vector<Prev> &Lprev) {
+        const uint64_t nC = C.size() / k;
+        std::vector<uint8_t> again(nC, 0);
+        bool any = false;
+        for (size_t i = 0; i < L.size() && !any; i++) any = build[i] && !L[i].slots;
+        if (!any) return COLO_OK;
+        std::unordered_map<std::vector<uint32_t>, uint64_t, PatSet::H> at(nC * 2 + 1, PatSet::H{k});
+        for (uint64_t ci = 0; ci < nC; ci++) at.emplace(std::vector<uint32_t>(&C[ci * k], &C[ci * k] + k), ci);
+        for (size_t i = 0; i < L.size(); i++)
+            if (build[i] && !L[i].slots) {
+                auto it = at.find(L[i].pat);
+                if (it == at.end()) return set_error(c, COLO_EARG, "internal: table without a candidate");
+                again[it->second] = 1;
+                st.recounts++;
+            }
+        std::vector<CandRes> res2(nC);
+        std::vector<Prev> L2;
+        std::shared_ptr<DevBuf> arena2;
+        force_slots_ = true;
+        int rc = o.sched == COLO_SCHED_PLAIN ? run_level_plain(k, C, again, Lprev, res2, L2)
+                                             : run_level_persistent(k, C, again, Lprev, res2, L2, arena2, nullptr);
+        force_slots_ = false;
+        if (rc) return rc;
+        std::unordered_map<std::vector<uint32_t>, size_t, PatSet::H> li(L.size() * 2 + 1, PatSet::H{k});
+        for (size_t i = 0; i < L.size(); i++) li.emplace(L[i].pat, i);
+        for (Prev &q : L2) {
+            auto it = li.find(q.pat);
+            if (it == li.end()) continue;
+            Prev &p = L[it->second];
+            p.slots = q.slots;
+            p.slots_off = q.slots_off;
+            p.nslots = q.nslots;
+            p.parent = q.parent;
+        }
+        for (size_t i = 0; i < L.size(); i++)
+            if (build[i] && !L[i].slots) return set_error(c, COLO_EARG, "internal: recount kept no slots");
+        return COLO_OK;
+    }
+
     // ---------------------------------------------------- cost model
+    // The learned schedule's table arena, reserved from the predicted rows
+    // while a level's count pass runs; taken by build_tables_persistent when
+    // large enough.
     std::shared_ptr<DevBuf> arena_reserved_;
+    // Allocation that gives the reservation back before it reports a
+    // failure: the reservation only saves an allocation, so it must never
+    // make a run fail that the exact sizes fit.
+    // Slots arena budget of a level's count pass in u32 words: 1/16 of the
+    // device's memory, or the tuned limit (KiB) when one is set.
+    bool force_slots_ = false;
+    uint64_t slots_budget_words() {
+        if (c->tune.slots_budget_kb) return (uint64_t)c->tune.slots_budget_kb << 8;
+        return (uint64_t)(c->pool.total_bytes() / 16 / 4);
+    }
+    bool ensure_buf(DevBuf &b, size_t bytes) {
+        if (b.ensure(bytes)) return true;
+        if (!arena_reserved_) return false;
+        arena_reserved_.reset();
+        return b.ensure(bytes);
+    }
 
     // Features of a candidate (COLO_MLP_IN floats): size, instance counts,
     // anchors / parent rows, filter counts and bound, non-empty cells, the
@@ -849,6 +1045,7 @@ struct Miner {
     int run(char **report) {
         auto T0 = Clock::now();
         const double sys_ms0 = c->pool.sys_ms, sys_bytes0 = c->pool.sys_bytes;
+        c->pool.peak = c->pool.in_use;
         const uint32_t F = c->F;
         rep.printf("colo-report v1\n");
         rep.printf("grid gx=%u gy=%u cells=%u n=%u F=%u r2=%08x theta=%08x\n", c->gx, c->gy, c->cells, c->n, F,
@@ -1059,36 +1256,10 @@ struct Miner {
                         L[i] = it->second;
                         build[i] = 0;
                     }
-                // tables to build whose slots were not counted in this run
-                std::vector<uint8_t> again(nC, 0);
-                bool any = false;
-                for (size_t i = 0; i < L.size(); i++)
-                    if (build[i] && !L[i].slots) {
-                        for (uint64_t ci = 0; ci < nC; ci++)
-                            if (std::equal(L[i].pat.begin(), L[i].pat.end(), &C[ci * k])) {
-                                again[ci] = 1;
-                                any = true;
-                                break;
-                            }
-                    }
-                if (any) {
-                    std::vector<CandRes> res2(nC);
-                    std::vector<Prev> L2;
-                    std::shared_ptr<DevBuf> arena2;
-                    rc = o.sched == COLO_SCHED_PLAIN ? run_level_plain(k, C, again, Lprev, res2, L2)
-                                                     : run_level_persistent(k, C, again, Lprev, res2, L2, arena2, nullptr);
-                    if (rc) return rc;
-                    for (Prev &q : L2)
-                        for (size_t i = 0; i < L.size(); i++)
-                            if (L[i].pat == q.pat) {
-                                L[i].slots = q.slots;
-                                L[i].slots_off = q.slots_off;
-                                L[i].nslots = q.nslots;
-                                L[i].parent = q.parent;
-                                break;
-                            }
-                }
             }
+            // tables to build whose slots were not kept (deferred above the
+            // slots budget, or remembered from an earlier threshold)
+            if ((rc = recount_missing_slots(k, C, L, build, Lprev))) return rc;
             rc = o.sched == COLO_SCHED_PLAIN ? build_tables_plain(k, L, build, Lprev)
                                              : build_tables_persistent(k, L, build, Lprev, C);
             if (rc) return rc;
@@ -1122,6 +1293,7 @@ struct Miner {
                 for (Prev &p : L) p.offs.reset();
             }
             Lprev = std::move(L);
+            index_prev(Lprev);
             C = std::move(N);
             if (k < 16) st.level_ms[k] = ms_since(TL);
             k++;
@@ -1132,6 +1304,7 @@ struct Miner {
         st.total_ms = ms_since(T0);
         st.alloc_ms = c->pool.sys_ms - sys_ms0;
         st.alloc_bytes = c->pool.sys_bytes - sys_bytes0;
+        st.pool_peak_bytes = (double)c->pool.peak;
         if (report) {
             *report = (char *)malloc(rep.s.size() + 1);
             memcpy(*report, rep.s.c_str(), rep.s.size() + 1);
@@ -1502,7 +1675,7 @@ static std::string tune_key(const colo_ctx *c, const colo_mine_opts &o) {
     for (char &ch : arch)
         if (ch == ' ') ch = '_';
     char b[256];
-    snprintf(b, sizeof b, "v3/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
+    snprintf(b, sizeof b, "v4/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
              fbits(o.theta), o.max_k, o.sched);
     return b;
 }
@@ -1555,7 +1728,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
     if (!(cache_path && tune_load(cache_path, key, &best))) {
         // kernel time of one run: {level-k count, level-k write, all batched
         // passes}; lvl: wall time per level
-        auto measure = [&](const TuneParams &t, double *m3, double *lvl) -> int {
+        auto measure = [&](const TuneParams &t, double *m3) -> int {
             c->tune = t;
             Miner M(c, o);
             int rc = M.run(nullptr);
@@ -1564,7 +1737,9 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             m3[0] = M.st.count_kernel_ms - M.st.l2_count_kernel_ms;
             m3[1] = M.st.write_kernel_ms - M.st.l2_write_kernel_ms;
             m3[2] = M.st.count_kernel_ms + M.st.write_kernel_ms;
-            if (lvl) memcpy(lvl, M.st.level_ms, sizeof M.st.level_ms);
+            if (c->verbose)
+                fprintf(stderr, "[colo tune] run: write var %u, %u workgroups per CU: count %.1f ms, write %.1f ms\n",
+                        t.write_var, t.blocks_per_cu, M.st.count_kernel_ms, M.st.write_kernel_ms);
             return COLO_OK;
         };
         auto fail = [&](int rc) {
@@ -1573,59 +1748,46 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
         };
         for (int k = 0; k < 16; k++) best.count_var_k[k] = best.item_rows_k[k] = COLO_TUNE_UNSET;
         double cur[3], m[3];
-        int rc = measure(best, cur, nullptr);  // warm-up (pool, cached features)
-        if (!rc) rc = measure(best, cur, nullptr);
+        int rc = measure(best, cur);  // warm-up (pool, cached features)
         if (rc) return fail(rc);
-        // whole-run parameters, one at a time
+        // 1. per level: the count variant and item size, timed online inside
+        //    one mining run (Miner::tune_level)
+        {
+            c->tune = best;
+            Miner M(c, o);
+            M.tuning_ = true;
+            rc = M.run(nullptr);
+            (void)hipStreamSynchronize(c->stream);
+            if (rc) return fail(rc);
+            best = c->tune;
+            // levels not seen here take the deepest level's variant
+            for (int k = 15; k >= 3; k--)
+                if (best.count_var_k[k] != COLO_TUNE_UNSET) {
+                    best.count_var = best.count_var_k[k];
+                    break;
+                }
+        }
+        if ((rc = measure(best, cur))) return fail(rc);
+        // 2. whole-run parameters, one at a time, by full runs
         struct Axis {
             uint32_t TuneParams::*field;
             std::vector<uint32_t> values;
             int metric;
         };
         const std::vector<Axis> axes = {{&TuneParams::write_var, {0, 1, 2}, 1},
-                                        {&TuneParams::blocks_per_cu, {4, 8, 16}, 2},
-                                        {&TuneParams::item_rows, {1024, 4096, 16384}, 2}};
+                                        {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
         for (const Axis &ax : axes) {
             const uint32_t start = best.*(ax.field);
             for (uint32_t v : ax.values) {
                 if (v == start) continue;
                 TuneParams t = best;
                 t.*(ax.field) = v;
-                if ((rc = measure(t, m, nullptr))) return fail(rc);
+                if ((rc = measure(t, m))) return fail(rc);
                 if (m[ax.metric] < cur[ax.metric]) {
                     best = t;
                     memcpy(cur, m, sizeof cur);
                 }
             }
-        }
-        // per level: every (count variant, item size) pair, judged by the
-        // level's wall time (count, scan and write of that level); the
-        // wave-cooperative and candidate-major counts (1, 2) are left out,
-        // slower on every workload measured
-        const uint32_t vars[] = {0, 3, 4, 5, 6, 7, 8, 9}, sizes[] = {1024, 4096, 16384};
-        constexpr int nv = 8, ns = 3;
-        double lv[nv][ns][16], tot[nv] = {};
-        for (int si = 0; si < ns; si++)
-            for (int vi = 0; vi < nv; vi++) {
-                TuneParams t = best;
-                t.count_var = vars[vi];
-                t.item_rows = sizes[si];
-                if ((rc = measure(t, m, lv[vi][si]))) return fail(rc);
-                if (sizes[si] == best.item_rows)
-                    for (int k = 3; k < 16; k++) tot[vi] += lv[vi][si][k];
-            }
-        int bv = 0;
-        for (int vi = 1; vi < nv; vi++)
-            if (tot[vi] < tot[bv]) bv = vi;
-        best.count_var = vars[bv];  // levels not seen here
-        for (int k = 2; k < 16; k++) {
-            if (lv[0][0][k] <= 0) continue;
-            int kv = 0, ks = 0;
-            for (int si = 0; si < ns; si++)
-                for (int vi = 0; vi < nv; vi++)
-                    if (lv[vi][si][k] < lv[kv][ks][k]) kv = vi, ks = si;
-            if (k >= 3) best.count_var_k[k] = vars[kv];
-            best.item_rows_k[k] = sizes[ks];
         }
         if (cache_path) {
             FILE *fp = fopen(cache_path, "ab");
@@ -1678,10 +1840,8 @@ int colo_run_config(const char *config_path, const colo_mine_opts *opts, char **
     double io_ms = std::chrono::duration<double, std::milli>(Clock::now() - tio).count();
     colo_dataset ds;
     colo_host_dataset(h, &ds);
-    auto tg = Clock::now();
     colo_ctx *ctx = nullptr;
     rc = colo_ctx_create(&ds, r, nullptr, &ctx);
-    double grid_ms = std::chrono::duration<double, std::milli>(Clock::now() - tg).count();
     if (rc) {
         if (ctx) fprintf(stderr, "colo: %s\n", colo_last_error(ctx));
         colo_ctx_destroy(ctx);
@@ -1698,12 +1858,18 @@ int colo_run_config(const char *config_path, const colo_mine_opts *opts, char **
     if (rc) fprintf(stderr, "colo: %s\n", colo_last_error(ctx));
     double e2e = std::chrono::duration<double, std::milli>(Clock::now() - T0).count();
     if (timing_line && !rc) {
-        // the reference's summary line (cpp:1144): dataset, program, theta,
-        // io, transfer, grid, filter, refine, table generation, end to end (ms)
+        // the reference's summary line (cpp:1144), fields as its comment at
+        // cpp:1143 names them, each as cout prints a float (%g): dataset,
+        // executable, theta, io (CSV parse, cpp:203), transfer (host-to-device
+        // copy of the dataset, cpp:240), grid (binning, cpp:251), filter (all
+        // levels, cpp:307, 770), refinement (count passes and PI, cpp:422,
+        // 850), table and candidate generation (t1, cpp:1129), end to end
+        // (ms).  The leading newline of cpp:1144 is the caller's to print.
+        auto g = [](double v) { return (double)(float)v; };  // the reference's times are float
         char b[1024];
-        snprintf(b, sizeof b, "%s , MI355X_colo , %g , %.3f , %.3f , %.3f , %.3f , %.3f , %.3f , %.3f", dataset.c_str(),
-                 (double)theta, io_ms, 0.0, grid_ms, st.filter_ms, st.count_ms + st.pi_ms, st.scan_ms + st.write_ms,
-                 e2e);
+        snprintf(b, sizeof b, "%s , colo_miner , %g , %g , %g , %g , %g , %g , %g , %g", dataset.c_str(),
+                 (double)theta, g(io_ms), g(ctx->upload_ms), g(ctx->bin_ms), g(st.filter_ms), g(st.count_ms + st.pi_ms),
+                 g(st.scan_ms + st.write_ms + st.host_ms), g(e2e));
         *timing_line = strdup(b);
     }
     colo_ctx_destroy(ctx);

@@ -18,8 +18,9 @@ Workloads (all on a square of side S, coordinates clamped into [0, S)):
       mean of ~32 neighbours; theta=0.2, max size 5.
   c5  50M points, 60 features Zipf(1.2); 60% in 2,000 hotspots (sigma
       log-uniform in [100, 2000]), each hotspot hosting 12 features; 20
-      planted patterns of size 3-6; r for ~64 neighbours; max size 6;
-      thetas 0.1, 0.2, 0.3, 0.5.
+      planted patterns of size 3-6, each a run of consecutive frequency
+      ranks in 6..59 (members of similar frequency); r for ~64 neighbours;
+      max size 6; thetas 0.1, 0.2, 0.3, 0.5.
 
 Hotspots are shared by all features unless hot_feats is set, in which case
 each hotspot hosts hot_feats features drawn at random and a feature's hotspot
@@ -41,7 +42,7 @@ CONFIGS = {
                planted=[3, 3, 3, 4, 4, 4, 5, 5, 3, 4], plant_rate=0.5, r=None, mean_nbrs=32, theta=0.2, max_k=5),
     "c5": dict(seed=5, S=1e5, F=60, N=50_000_000, zipf=1.2, hot_frac=0.6, hotspots=2000, sigma=(100, 2000),
                planted=[3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6, 3, 4, 5, 6], plant_rate=0.5, r=None,
-               mean_nbrs=64, theta=0.1, thetas=[0.1, 0.2, 0.3, 0.5], max_k=6, hot_feats=12),
+               mean_nbrs=64, theta=0.1, thetas=[0.1, 0.2, 0.3, 0.5], max_k=6, hot_feats=12, plant_band=(6, 60)),
 }
 
 
@@ -129,19 +130,35 @@ def generate(name, N=None, seed=None, **over):
     r = cfg["r"]
     if r is None:
         r = estimate_r(np.concatenate(xs), np.concatenate(ys), cfg["mean_nbrs"], rng)
-    # planted patterns: disjoint feature sets drawn by rank; for plant_rate of
-    # the least frequent member's instances, all members go within r/2 of a
-    # shared seed point
+    # planted patterns: for plant_rate of the least frequent member's
+    # instances, all members go within r/2 of a shared seed point.  Members
+    # are disjoint feature sets drawn at random, or, with plant_band = (lo,
+    # hi), runs of consecutive frequency ranks inside [lo, hi): members of
+    # similar frequency, so that a planted set's participation index stays
+    # near plant_rate for every member (with Zipf frequencies, a set of
+    # random members is limited by its most frequent one)
     used = np.zeros(F, np.int64)
     order = rng.permutation(F)
     pos = 0
     planted = []
+    band = cfg.get("plant_band")
+    taken = np.zeros(F, bool)
     for size in cfg["planted"]:
-        if pos + size > F:
-            order = rng.permutation(F)
-            pos = 0
-        feats = np.sort(order[pos:pos + size])
-        pos += size
+        if band:
+            # a window of free ranks; the band is reused once no window fits
+            free = [st for st in range(band[0], band[1] - size + 1) if not taken[st:st + size].any()]
+            if not free:
+                taken[:] = False
+                free = list(range(band[0], band[1] - size + 1))
+            start = int(free[int(rng.integers(0, len(free)))])
+            taken[start:start + size] = True
+            feats = np.arange(start, start + size)
+        else:
+            if pos + size > F:
+                order = rng.permutation(F)
+                pos = 0
+            feats = np.sort(order[pos:pos + size])
+            pos += size
         planted.append(feats.tolist())
         m = int(cfg["plant_rate"] * min(counts[f] - used[f] for f in feats))
         if m <= 0:
@@ -157,22 +174,52 @@ def generate(name, N=None, seed=None, **over):
     labels = np.repeat(np.arange(F), counts)
     x, y = _clamp_cells(np.concatenate(xs), np.concatenate(ys), S, r)
     params = dict(r=float(r), theta=cfg["theta"], max_k=cfg["max_k"], S=S, planted=planted,
-                  thetas=cfg.get("thetas", [cfg["theta"]]), name=name, N=int(cfg["N"]), F=F, seed=cfg["seed"])
+                  thetas=cfg.get("thetas", [cfg["theta"]]), name=name, N=int(cfg["N"]), F=F, seed=cfg["seed"],
+                  zipf=cfg["zipf"], hot_frac=cfg["hot_frac"], hotspots=nh, sigma=tuple(cfg["sigma"]),
+                  hot_feats=cfg.get("hot_feats"), plant_rate=cfg["plant_rate"], plant_band=band)
     return labels, x, y, S, params
 
 
 def estimate_r(x, y, mean_nbrs, rng, sample=20000):
     """r (rounded to 3 significant digits) giving about mean_nbrs neighbours
-    per point (including itself) on a random sample."""
+    per point (including itself) on a random sample.
+
+    Bisection on the radius over the sample's exact mean neighbour count.
+    Counts are taken from a k-d tree over only the points near the sample: a
+    radius R whose mean count already reaches mean_nbrs is found on a grid of
+    R-sized cells, and the tree holds the points of the sample's 3x3 cells.
+    Every radius below R is counted exactly; every radius above R has a mean
+    count of at least the one at R, so the bisection takes the same branch
+    as with a tree over all points."""
     from scipy.spatial import cKDTree
     n = len(x)
-    tree = cKDTree(np.column_stack([x, y]))
     idx = rng.choice(n, size=min(sample, n), replace=False)
     q = np.column_stack([x[idx], y[idx]])
+    x0, y0 = float(x.min()), float(y.min())
+    area = max((float(x.max()) - x0) * (float(y.max()) - y0), 1e-12)
+    R = float(np.sqrt(mean_nbrs * area / (np.pi * n)))  # uniform-density guess
+    while True:
+        nx = int((float(x.max()) - x0) / R) + 3
+        ny = int((float(y.max()) - y0) / R) + 3
+        qc = np.floor((q[:, 0] - x0) / R).astype(np.int64) + 1, np.floor((q[:, 1] - y0) / R).astype(np.int64) + 1
+        need = np.zeros((ny, nx), bool)
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                need[np.clip(qc[1] + dy, 0, ny - 1), np.clip(qc[0] + dx, 0, nx - 1)] = True
+        cx = np.floor((x - x0) / R).astype(np.int64) + 1
+        cy = np.floor((y - y0) / R).astype(np.int64) + 1
+        sel = need[np.clip(cy, 0, ny - 1), np.clip(cx, 0, nx - 1)]
+        del cx, cy
+        tree = cKDTree(np.column_stack([x[sel], y[sel]]))
+        del sel
+        exact = 0.99 * R  # points within this of a sample point lie in its 3x3 cells
+        if tree.query_ball_point(q, exact, return_length=True).mean() >= mean_nbrs or R > 1e5:
+            break
+        R *= 2.0
     lo, hi = 1e-6, 1e5
     for _ in range(40):
         mid = np.sqrt(lo * hi)
-        m = tree.query_ball_point(q, mid, return_length=True).mean()
+        m = tree.query_ball_point(q, mid, return_length=True).mean() if mid <= exact else mean_nbrs
         if m < mean_nbrs:
             lo = mid
         else:

@@ -89,6 +89,9 @@ int colo_bin(colo_ctx *ctx, float r, void *stream);
 int colo_get_grid(const colo_ctx *ctx, colo_grid *out);
 /* Human-readable message for the last error on this context. */
 const char *colo_last_error(const colo_ctx *ctx);
+/* level > 0: progress lines on stderr (one per autotuner measurement).  No
+ * counterpart in the reference. */
+int colo_set_verbose(colo_ctx *ctx, int level);
 
 /* Copies the binning out (device -> device, input order for cell_of,
  * sorted order for sorted_idx; dir = F x (cells+1) first sorted position of
@@ -225,6 +228,11 @@ typedef struct colo_mine_stats {
     double alloc_bytes;     /* bytes obtained from hipMalloc during the run   */
     double level_ms[16];    /* wall time of level k (index k)                 */
     double level_count_kernel_ms[16]; /* count-pass kernel time of level k    */
+    double pool_peak_bytes; /* most device bytes the run held at once (tables,
+                               arenas, slots, reservations)                */
+    uint64_t slots_deferred;/* candidates counted without per-row slots
+                               (slots arena above its budget)              */
+    uint64_t recounts;      /* candidates recounted for their table's slots */
 } colo_mine_stats;
 
 /* HIP graphs: the binning (colo_bin, colo_ctx_create) is captured once per
@@ -265,19 +273,25 @@ typedef struct colo_tune {
     uint32_t count_var_k[16];
     /* item rows of level k (index k, 2..15); COLO_TUNE_UNSET = item_rows */
     uint32_t item_rows_k[16];
+    /* largest slots arena of a level's count pass in KiB; 0 = 1/16 of the
+     * device's memory.  Candidates above it are counted without per-row
+     * slots and recounted when their table is built (results unchanged). */
+    uint32_t slots_budget_kb;
 } colo_tune;
 #define COLO_TUNE_UNSET 0xffffffffu
 
 int colo_get_tune(const colo_ctx *ctx, colo_tune *out);
 int colo_set_tune(colo_ctx *ctx, const colo_tune *t);
-/* Autotuner: mines the context's data with opts under candidate launch
- * shapes, one parameter at a time (coordinate descent), and keeps the one
- * with the least HIP-event kernel time in the passes the parameter
- * affects; the count variant and the item size are chosen per level, by
- * each level's time.  The context then uses the result.  With cache_path, results are
- * stored under a key of device and workload (instances, features, r, theta,
- * max_k, schedule) and a cached entry is taken without measuring.  Every
- * shape gives the same results; only time differs. */
+/* Autotuner: mines the context's data with opts and keeps the launch
+ * shapes with the least HIP-event kernel time.  Per level, the count
+ * variant and item size are chosen inside one mining run: each pair is
+ * timed on a sample of that level's work items (one block of 16 items in
+ * 8).  The write variant and the workgroups per CU are then chosen one at a
+ * time by full runs (coordinate descent).  The context then uses the
+ * result.  With cache_path, results are stored under a key of device and
+ * workload (instances, features, r, theta, max_k, schedule) and a cached
+ * entry is taken without measuring.  Every shape gives the same results;
+ * only time differs. */
 int colo_autotune(colo_ctx *ctx, const colo_mine_opts *opts, const char *cache_path, colo_tune *out);
 
 /* ------------------------------------------------------- cost model */

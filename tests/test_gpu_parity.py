@@ -289,17 +289,22 @@ def test_threshold_sweep(c2small, colo_mod, sched):
                                   dict(item_rows=1024, item_rows_k=[0xFFFFFFFF] * 2 + [64, 256, 16384] + [0xFFFFFFFF] * 11),
                                   dict(count_var=4, item_rows=256), dict(count_var=0, item_rows=64),
                                   dict(count_var=6), dict(count_var=7, item_rows=256),
-                                  dict(count_var=8), dict(count_var=9, item_rows=64)])
+                                  dict(count_var=8), dict(count_var=9, item_rows=64),
+                                  dict(slots_budget_kb=64), dict(count_var=7, slots_budget_kb=16, item_rows=256)])
 def test_launch_shapes_identical(c2small, tune):
-    """Every launch shape of the batched passes gives the oracle's report."""
+    """Every launch shape of the batched passes gives the oracle's report;
+    so does a slots budget small enough that the largest candidates are
+    counted without per-row slots and recounted for their tables."""
     ctx, orc, p = c2small
     _say(f"launch shape {tune}")
     old = ctx.tune
     try:
         ctx.tune = tune
         for sched in (1, 2):
-            rep, _ = ctx.mine(0.15, max_k=4, sched=sched)
+            rep, st = ctx.mine(0.15, max_k=4, sched=sched)
             assert rep == orc.mine(p["r"], 0.15, max_k=4), (tune, sched)
+            if tune.get("slots_budget_kb"):
+                assert st["slots_deferred"] > 0 and st["recounts"] > 0, st
     finally:
         ctx.tune = old
 
@@ -390,7 +395,16 @@ def test_cli_drop_in(datagen, colo_mod, oracle_mod, tmp_path, sched):
     r = subprocess.run([exe, w["config"], "--sched", sched, "--max-k", "4"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
-    assert r.stdout.strip(), "no timing line"
+    # timing line (cpp:1144): a newline, then dataset, executable, theta and
+    # seven times in ms, " , "-separated, numbers as cout prints a float
+    assert r.stdout.startswith("\n"), repr(r.stdout)
+    fields = r.stdout[1:].split(" , ")
+    assert len(fields) == 10, fields
+    assert fields[0] == w["csv"] and fields[1] == "colo_miner"
+    assert fields[2] == "%g" % np.float32(w["params"]["theta"]), fields[2]
+    times = [float(v) for v in fields[3:]]
+    assert all(t >= 0 for t in times) and times[1] > 0, times  # transfer measured
+    assert times[-1] >= max(times[:-1]), times  # end to end spans the others
     orc = oracle_mod.Oracle.from_csv(w["csv"])
     o = str(tmp_path / "orc_")
     orc.mine(w["params"]["r"], w["params"]["theta"], max_k=4, out_prefix=o)

@@ -27,7 +27,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-SOURCES = ["colo_device.hip", "colo_mlp.hip", "colo_miner.cpp"]
+SOURCES = ["colo_device.hip", "colo_mlp.hip", "colo_noalloc.hip", "colo_miner.cpp"]
 
 
 def _run(cmd):

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libcolo.so")
 COLO_SCHED_PLAIN = 0
 COLO_SCHED_PERSISTENT = 1
 COLO_SCHED_LEARNED = 2
+SCAN_NOALLOC_MAX_N = 1 << 28  # COLO_SCAN_NOALLOC_MAX_N
 
 ERRORS = {0: "ok", 1: "file", 2: "parse", 3: "bounds", 4: "nomem", 5: "hip", 6: "arg", 7: "unsupported"}
 
@@ -85,7 +86,8 @@ _lib = None
 
 SYMBOLS = [
     "colo_ctx_create", "colo_ctx_destroy", "colo_trim", "colo_set_verbose", "colo_get_tune", "colo_set_tune", "colo_autotune", "colo_set_graphs", "colo_graph_info", "colo_bin", "colo_get_grid", "colo_last_error", "colo_binning_export",
-    "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_join2_write",
+    "colo_filter", "colo_bits_words", "colo_join2_count", "colo_scan_slots", "colo_scan_slots_noalloc",
+    "colo_join2_write",
     "colo_joink_count", "colo_joink_write", "colo_joink_count_nbr", "colo_joink_write_nbr", "colo_pi", "colo_mine_sweep", "colo_participation", "colo_mine", "colo_free_str",
     "colo_run_config", "colo_load_csv", "colo_load_csv_opts", "colo_host_dataset", "colo_host_label", "colo_host_free",
     "colo_candidates", "colo_mlp_create", "colo_mlp_destroy", "colo_mlp_set_params", "colo_mlp_get_params",
@@ -120,6 +122,7 @@ def lib():
         L.colo_bits_words.argtypes = [vp, u32, vp]
         L.colo_join2_count.argtypes = [vp, u32, u32, vp, vp, C.POINTER(u64), vp]
         L.colo_scan_slots.argtypes = [vp, vp, u64, vp, C.POINTER(u64), vp]
+        L.colo_scan_slots_noalloc.argtypes = [vp, vp, u64, vp, C.POINTER(u64), vp]
         L.colo_join2_write.argtypes = [vp, u32, u32, vp, vp, vp, vp, vp]
         L.colo_joink_count.argtypes = [vp, vp, u64, u32, vp, vp, vp, C.POINTER(u64), vp]
         L.colo_joink_write.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp, vp]

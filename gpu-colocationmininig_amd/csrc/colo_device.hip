@@ -267,6 +267,14 @@ int colo::scan_u32_u64(colo_ctx *c, const uint32_t *in, uint64_t n, uint64_t *ou
     return hipGetLastError() == hipSuccess ? COLO_OK : COLO_EHIP;
 }
 
+uint64_t colo::scan_ws_words(uint64_t n) { return scan_tmp_words(n); }
+
+int colo::scan_slots_ws(const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *ws, hipStream_t s) {
+    if (n == 0) return fill_u32((uint32_t *)out, 2, 0, s);
+    scan_rec<uint32_t, uint64_t>(in, n, out, out + n, ws, s);
+    return hipGetLastError() == hipSuccess ? COLO_OK : COLO_EHIP;
+}
+
 // ------------------------------------------------------------- binning
 
 // getCellID (K:615-625) in binary32; key = fid << cellbits | cell.

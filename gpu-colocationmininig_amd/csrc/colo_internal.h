@@ -266,6 +266,11 @@ int scratch(colo_ctx *c, size_t bytes, void **p);
 int scan_u32_u64(colo_ctx *c, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *total_dev,
                  hipStream_t s);
 int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+// Exclusive scan of n u32 slot counts into out[0..n) (u64), the total at
+// out[n], with a caller-provided workspace of scan_ws_words(n) u64: launches
+// only, no allocation.
+uint64_t scan_ws_words(uint64_t n);
+int scan_slots_ws(const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *ws, hipStream_t s);
 int filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
                  uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
 // The filter's pass flags (threshold_Kernel, K:545-573) from counts already

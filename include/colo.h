@@ -129,6 +129,17 @@ int colo_join2_count(colo_ctx *ctx, uint32_t a, uint32_t b, uint32_t *d_slots, u
 int colo_scan_slots(colo_ctx *ctx, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets,
                     uint64_t *total, void *stream);
 
+/* colo_scan_slots without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scan's workspace is a static device array of about 1 MiB per GPU, held
+ * until the process ends, so n is limited to COLO_SCAN_NOALLOC_MAX_N
+ * (COLO_EARG above it).  Calls on one GPU share that array: calls that
+ * overlap on one GPU race.  Results equal colo_scan_slots's. */
+#define COLO_SCAN_NOALLOC_MAX_N (1ull << 28)
+int colo_scan_slots_noalloc(colo_ctx *ctx, const uint32_t *d_slots, uint64_t n, uint64_t *d_offsets,
+                            uint64_t *total, void *stream);
+
 /* Level-2 write pass: rows (g, q) in a-instance order, neighbours in scan
  * order, into d_table (2 * rows u32).  d_nbr (rows u32, may be NULL)
  * receives the same neighbours as positions in the cell-sorted order of

@@ -96,9 +96,17 @@ SYMBOLS = [
 
 
 def lib():
-    """Loads libcolo.so; raises when it has not been built."""
+    """Loads libcolo.so; raises when it has not been built.
+
+    torch is imported first: torch ships its own HIP runtime under the same
+    soname (libamdhip64.so.7) as the one libcolo.so links, and the first one
+    loaded serves the whole process.  Loading torch's first keeps torch and
+    the kernels of libcolo.so on one runtime, whatever the caller touches
+    first (a library loaded before torch put torch on the other runtime and
+    made the library's first hipMalloc fail)."""
     global _lib
     if _lib is None:
+        import torch  # noqa: F401  (see above)
         if not os.path.exists(LIB_PATH):
             raise ColoError(5, f"{LIB_PATH} is missing: build it first (python gpu-colocationmininig_amd/build_lib.py)")
         L = C.CDLL(LIB_PATH)

@@ -2286,9 +2286,11 @@ __global__ void k_participation(const uint32_t *bits, SegDesc sd, uint32_t N, ui
 // ------------------------------------------------------------ C ABI
 
 static int alloc_dev(colo_ctx *c, void **p, size_t bytes) {
-    if (hipMalloc(p, bytes ? bytes : 4) != hipSuccess) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e != hipSuccess) {
         *p = nullptr;
-        return set_error(c, COLO_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return set_error(c, COLO_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                             hipGetErrorName(e));
     }
     return COLO_OK;
 }

@@ -377,6 +377,58 @@ static inline float d2_of(const orc_ctx *c, uint32_t p, uint32_t q) {
     return sx + sy;
 }
 
+/* The one rounding question the reference's source leaves open (U:31): how
+ * nvcc evaluated pow(x2-x1, 2) + pow(y2-y1, 2) (a float or a double pow,
+ * and whether the sum was contracted into a fused multiply-add).  Over
+ * every distance test of the level-2 scan (K:24-41: each instance against
+ * every instance of a later feature in its 3x3 cells), counts the tests
+ * where another evaluation decides d2 <= r2 differently from the canonical
+ * binary32 form fl(fl(dx*dx) + fl(dy*dy)):
+ *   out[0] tests, out[1] neighbours under the canonical form,
+ *   out[2] fmaf(dx, dx, fl(dy*dy)) decides differently,
+ *   out[3] fmaf(dy, dy, fl(dx*dx)) decides differently,
+ *   out[4] fl32(double(dx)*dx + double(dy)*dy) decides differently,
+ *   out[5] at least one of the three does.
+ * CUDA's pow is not reproduced: each form is an IEEE operation sequence. */
+void orc_boundary_pairs(const orc_ctx *c, uint64_t *out) {
+    uint64_t t = 0, nb = 0, fa = 0, fb = 0, db = 0, any = 0;
+    const uint32_t F = c->F;
+    #pragma omp parallel for schedule(dynamic, 1024) reduction(+:t,nb,fa,fb,db,any)
+    for (int64_t pi = 0; pi < (int64_t)c->n; pi++) {
+        const uint32_t p = (uint32_t)pi, fp = c->fid[p];
+        if (fp + 1 >= F) continue;
+        const uint32_t cur = c->cellId[p];
+        for (int32_t j = -1; j <= 1; j++)
+            for (int32_t i = -1; i <= 1; i++) {
+                int32_t ncl = orc_nbr_cell(c->gx, c->cells, cur, i, j);
+                if (ncl < 0) continue;
+                const uint64_t base = (uint64_t)ncl * F;
+                const int32_t b = c->start[base + fp + 1];
+                const int32_t e = c->start[base + F - 1] + c->cnt[base + F - 1];
+                for (int32_t pos = b; pos < e; pos++) {
+                    const uint32_t q = c->sortedIdx[pos];
+                    volatile float dx = c->x[q] - c->x[p];
+                    volatile float dy = c->y[q] - c->y[p];
+                    volatile float sx = dx * dx;
+                    volatile float sy = dy * dy;
+                    volatile float d2 = sx + sy;
+                    const int canon = d2 <= c->r2;
+                    const int ra = fmaf(dx, dx, sy) <= c->r2;
+                    const int rb = fmaf(dy, dy, sx) <= c->r2;
+                    volatile double dd = (double)dx * (double)dx + (double)dy * (double)dy;
+                    const int rd = (float)dd <= c->r2;
+                    t++;
+                    nb += (uint64_t)canon;
+                    fa += (uint64_t)(ra != canon);
+                    fb += (uint64_t)(rb != canon);
+                    db += (uint64_t)(rd != canon);
+                    any += (uint64_t)(ra != canon || rb != canon || rd != canon);
+                }
+            }
+    }
+    out[0] = t; out[1] = nb; out[2] = fa; out[3] = fb; out[4] = db; out[5] = any;
+}
+
 /* ---------------------------------------------------------------- level 2 */
 
 /* degree2TableSlotCounter_Kernel (K:12-55) for candidate (a, b): one pass

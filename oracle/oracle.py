@@ -81,6 +81,7 @@ def lib():
         L.orc_mine.argtypes = [vp, C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                C.c_char_p, C.POINTER(C.c_int)]
         L.orc_free_str.argtypes = [C.c_void_p]
+        L.orc_boundary_pairs.argtypes = [vp, _u64p]
         L.orc_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -220,6 +221,15 @@ class Oracle:
         fn = lib().orc_filter_literal if literal else lib().orc_filter
         fn(self.h, cands, Cn, k, theta, counts, passed)
         return counts[: Cn * k].reshape(Cn, k), passed[:Cn].astype(bool)
+
+    def boundary_pairs(self):
+        """Distance tests of the level-2 scan where a fused or a
+        double-accumulated squared distance decides d2 <= r2 differently from
+        the canonical binary32 form (orc_boundary_pairs)."""
+        out = np.zeros(6, np.uint64)
+        lib().orc_boundary_pairs(self.h, out)
+        keys = ("tests", "neighbours", "fma_dx_first", "fma_dy_first", "double_sum", "any_form")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def mine(self, r, theta, max_k=0, pi_mode=0, filter_on=True, hash_tables=True, out_prefix=None):
         err = C.c_int(0)

@@ -414,3 +414,80 @@ def test_cli_drop_in(datagen, colo_mod, oracle_mod, tmp_path, sched):
         assert os.path.exists(fg) == os.path.exists(fo), k
         if os.path.exists(fo):
             assert open(fg).read() == open(fo).read(), k
+
+
+def _run_cli(colo_mod, config, cwd, *extra):
+    import subprocess
+    exe = os.path.join(os.path.dirname(colo_mod.__file__), "colo_miner")
+    return subprocess.run([exe, config, *extra], capture_output=True, text=True, timeout=600, cwd=cwd)
+
+
+@pytest.mark.parametrize("sched", ["plain", "learned"])
+def test_cli_hand_case(colo_mod, tmp_path, sched):
+    """The drop-in executable writes the hand-worked pattern files (labels
+    7, 3, 7, 12, 5; compat numerators with two instances per feature) byte
+    for byte."""
+    from test_oracle import HAND_CSV, HAND_FILES
+    _say(f"cli hand case {sched}")
+    (tmp_path / "hand.csv").write_text(HAND_CSV)
+    (tmp_path / "parameter.txt").write_text("2\n0.5\n1\n")
+    (tmp_path / "config.txt").write_text(f"{tmp_path}/\n{tmp_path}/parameter.txt\n{tmp_path}/hand.csv\n./\nlog.csv\n")
+    r = _run_cli(colo_mod, str(tmp_path / "config.txt"), str(tmp_path), "--sched", sched)
+    assert r.returncode == 0, r.stderr
+    for k, text in HAND_FILES.items():
+        assert (tmp_path / f"ColocationDegree_{k}.txt").read_text() == text, k
+    assert not (tmp_path / "ColocationDegree_5.txt").exists()
+
+
+def test_seattle_testcase_full_depth(colo_mod, tmp_path):
+    """The reference's TestCase as it ships: config.txt with relative paths
+    (TestCase/config.txt:1-5), Data/parameter.txt (r = 10, theta = 0.3) and
+    Data/Seattle2012.csv, run from the TestCase directory with no size cap.
+    The pattern files equal the oracle's (tests/golden/seattle_*), and the
+    report with table checksums equals the oracle's complete report."""
+    import gzip
+    import shutil
+    _say("seattle testcase, full depth, drop-in executable")
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    (tmp_path / "Data").mkdir()
+    with gzip.open(os.path.join(g, "seattle2012.csv.gz"), "rb") as fi, open(tmp_path / "Data" / "Seattle2012.csv", "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    (tmp_path / "Data" / "parameter.txt").write_text("10\n0.3\n1\n")
+    (tmp_path / "config.txt").write_text("./\n./Data/parameter.txt\n./Data/Seattle2012.csv\n./ExperimentLog/\nExperimentLog1.csv\n")
+    r = _run_cli(colo_mod, "config.txt", str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split(" , ")[0] == "\n./Data/Seattle2012.csv"
+    want = sorted(f for f in os.listdir(g) if f.startswith("seattle_ColocationDegree_"))
+    assert want, "golden pattern files missing: run tests/golden/make_golden.py seattle"
+    got = sorted(f for f in os.listdir(tmp_path) if f.startswith("ColocationDegree_"))
+    assert got == [f[len("seattle_"):] for f in want]
+    for f in want:
+        assert (tmp_path / f[len("seattle_"):]).read_text() == open(os.path.join(g, f)).read(), f
+    _say("seattle testcase, full depth, report")
+    with gzip.open(os.path.join(g, "seattle_report.txt.gz"), "rt") as fh:
+        fh.readline()
+        golden = fh.read()
+    hd = colo_mod.HostData(str(tmp_path / "Data" / "Seattle2012.csv"))
+    ctx = colo_mod.Context.from_host(hd, 10.0)
+    for sched in (colo_mod.COLO_SCHED_PLAIN, colo_mod.COLO_SCHED_PERSISTENT):
+        rep, _ = ctx.mine(0.3, max_k=0, sched=sched, hash_tables=True)
+        assert rep == golden, sched
+
+
+@pytest.mark.parametrize("fx", ["c1", "c2small"])
+def test_filter_threshold_on_a_bound(fx, request):
+    """theta exactly on a candidate's bound passes it, one float32 step
+    above fails it, as in the oracle (threshold_Kernel, K:545-573)."""
+    ctx, orc, p = request.getfixturevalue(fx)
+    _say(f"filter threshold on a bound {fx}")
+    F = ctx.F
+    cands = np.array([[i, j] for i in range(F) for j in range(i + 1, F)], np.uint32)
+    oc, _ = orc.filter(cands, 2, 0.0)
+    ub = [min(np.float32(np.float64(c) / np.float64(np.float32(ctx.fcount[f]))) for c, f in zip(row, cand))
+          for row, cand in zip(oc, cands)]
+    u = sorted(ub)[len(ub) // 2]
+    for th in (u, np.nextafter(np.float32(u), np.float32(2))):
+        gc, gp = ctx.filter(cands, 2, float(th))
+        oc2, op2 = orc.filter(cands, 2, float(th))
+        np.testing.assert_array_equal(gc, oc2)
+        np.testing.assert_array_equal(gp, op2)

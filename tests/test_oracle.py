@@ -336,3 +336,173 @@ def test_table_hash_properties():
     assert oracle.table_hash(u) != h
     assert oracle.table_hash(t[:, [1, 0, 2]]) != h
     assert oracle.table_hash(np.vstack([t, t[:1]])) != h
+
+
+def filter_bruteforce(x, y, fc, b, r, cands, theta):
+    """The cell-aggregate bound (K:469-509, 545-573) restated on arrays,
+    independently of the oracle's code: per-(cell, feature) counts on the
+    gy x gx grid; a 2x2 quadrant qualifies when every member occurs in one
+    of its four cells; every cell of the union of qualifying quadrants counts
+    once; ub_t = fl32(double(cnt_t) / double(fl32(N_t))); pass iff
+    min_t ub_t >= theta.  Returns (counts, pass, min_t ub_t)."""
+    cid, gx = _cells32(x, y, b, r)
+    r32 = np.float32(r)
+    gy = int(np.ceil(np.float32(np.float32(b[3]) + r32 - np.float32(b[1])) / r32))
+    F = len(fc)
+    fid = np.repeat(np.arange(F), fc)
+    cnt = np.zeros((gy * gx, F), np.int64)
+    np.add.at(cnt, (cid.astype(np.int64), fid), 1)
+    cnt = cnt.reshape(gy, gx, F)
+    out_c, out_p, out_u = [], [], []
+    for cand in cands:
+        q = np.ones((max(gy - 1, 0), max(gx - 1, 0)), bool)
+        for f in cand:
+            pr = cnt[:, :, f] > 0
+            q &= pr[:-1, :-1] | pr[1:, :-1] | pr[:-1, 1:] | pr[1:, 1:]
+        U = np.zeros((gy, gx), bool)
+        U[:-1, :-1] |= q
+        U[1:, :-1] |= q
+        U[:-1, 1:] |= q
+        U[1:, 1:] |= q
+        c = [int(cnt[:, :, f][U].sum()) for f in cand]
+        ub = min(np.float32(np.float64(ci) / np.float64(np.float32(fc[f]))) for ci, f in zip(c, cand))
+        out_c.append(c)
+        out_p.append(bool(ub >= np.float32(theta)))
+        out_u.append(ub)
+    return np.array(out_c, np.uint32), np.array(out_p), out_u
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_filter_bruteforce(oracle_mod, seed):
+    """The oracle's filter counts and pass flags equal the array restatement
+    on small random grids (clustered and uniform points, pairs and triples),
+    with theta also placed exactly on a bound (passes) and one float32 step
+    above it (fails)."""
+    rng = np.random.default_rng(1000 + seed)
+    F = int(rng.integers(3, 6))
+    S = float(rng.choice([100.0, 150.0, 250.0]))
+    r = float(rng.choice([20.0, 30.0, 45.0]))
+    x, y, fc, b = _dataset(rng, int(rng.integers(15, 60)), F, S, hot=float(rng.choice([0.0, 0.5])))
+    o = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    o.bin(r)
+    for k in (2, 3):
+        cands = np.array([c for c in __import__("itertools").combinations(range(F), k)], np.uint32)
+        ec, ep, eu = filter_bruteforce(x, y, fc, b, r, cands, 0.3)
+        oc, op = o.filter(cands, k, 0.3)
+        np.testing.assert_array_equal(oc, ec)
+        np.testing.assert_array_equal(op, ep)
+        lit_c, lit_p = o.filter(cands, k, 0.3, literal=True)
+        np.testing.assert_array_equal(lit_c, ec)
+        u = max(eu)  # theta exactly on a bound, and one step above it
+        if u > 0:
+            for th, want in ((u, True), (np.nextafter(np.float32(u), np.float32(2)), False)):
+                _, p_at = o.filter(cands, k, float(th))
+                _, e_at, _ = filter_bruteforce(x, y, fc, b, r, cands, th)
+                np.testing.assert_array_equal(p_at, e_at)
+                assert p_at[int(np.argmax(eu))] == want
+
+
+HAND_CSV = """10
+7,10.0,10.0
+7,50.0,50.0
+3,11.0,10.0
+3,51.0,50.0
+7,10.0,11.0
+7,50.0,51.0
+12,90.0,90.0
+12,90.5,10.0
+5,10.5,10.5
+5,95.0,95.0
+
+0,0,100,100"""
+
+# Worked by hand from the reference's rules for r = 2, theta = 0.5.  Features
+# in row order: f0 "7", f1 "3", f2 "7" (a reappearing label is a new feature,
+# cpp:168-183), f3 "12", f4 "5".  f0, f1, f2 meet at (10, 10) and (50, 50),
+# f4 only at (10, 10), f3 nowhere.  With two instances per feature the
+# reference's windows (cu:116-124) give the first member 1 of 2 and every
+# other member 1 or 2 of 2, so each co-located set has PI 0.5 and is
+# prevalent (cpp:335: !(PI < theta)); sets with f3 have PI 0.  Level 3 from
+# the level-2 list by the Apriori join with its first-failure break
+# (cpp:517-557): (0,1,2), (0,1,4), (0,2,4), (1,2,4); level 4: (0,1,2,4).
+# Lines per savetoFile2 / savetoFileGen (cpp:960-997).
+HAND_FILES = {
+    2: "( 7 | 3 )\n( 7 | 7 )\n( 7 | 5 )\n( 3 | 7 )\n( 3 | 5 )\n( 7 | 5 )\n",
+    3: "( 7 | 3 | 7 )\n( 7 | 3 | 5 )\n( 7 | 7 | 5 )\n( 3 | 7 | 5 )\n",
+    4: "( 7 | 3 | 7 | 5 )\n",
+}
+
+
+def test_pattern_files_hand(oracle_mod, tmp_path):
+    """The oracle's ColocationDegree_<k>.txt files equal the hand-worked
+    expectation byte for byte, and nothing is written past size 4."""
+    csv = tmp_path / "hand.csv"
+    csv.write_text(HAND_CSV)
+    o = oracle_mod.Oracle.from_csv(str(csv))
+    assert [o.label(f) for f in range(o.F)] == ["7", "3", "7", "12", "5"]
+    prefix = str(tmp_path) + "/"
+    rep = o.mine(2.0, 0.5, max_k=0, out_prefix=prefix)
+    for k, text in HAND_FILES.items():
+        assert (tmp_path / f"ColocationDegree_{k}.txt").read_text() == text, k
+    assert not (tmp_path / "ColocationDegree_5.txt").exists()
+    assert rep.splitlines()[-1] == "end levels 3 patterns 11"
+
+
+def _round_f32(fr):
+    """Fraction -> nearest binary32 (ties to even), exactly."""
+    c = np.float32(float(fr))
+    best = None
+    for v in (np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))):
+        d = abs(Fraction(float(v)) - fr)
+        if best is None or d < best[0] or (d == best[0] and int(v.view(np.uint32)) % 2 == 0):
+            best = (d, v)
+    return best[1]
+
+
+def test_boundary_pair_forms(oracle_mod):
+    """orc_boundary_pairs against exact rational arithmetic: pairs placed
+    within a few ulps of r^2 (far apart from each other, so each instance's
+    3x3 cells hold only its partner); for each pair the canonical form
+    fl(fl(dx*dx) + fl(dy*dy)), both fused forms (exact product plus the
+    other rounded square, rounded once) and the double-accumulated form
+    (exact sum rounded to binary64, then to binary32) are evaluated here and
+    their <= r^2 decisions compared."""
+    rng = np.random.default_rng(21)
+    r = np.float32(7.3)
+    r2 = r * r
+    xs, ys, qx, qy = [], [], [], []
+    while len(xs) < 3000:
+        k = len(xs)
+        ax, ay = np.float32(50.0 * (k % 60) + 11.0), np.float32(50.0 * (k // 60) + 11.0)
+        t = rng.uniform(0, 2 * np.pi)
+        d = float(r) * (1 + rng.uniform(-4e-7, 4e-7))
+        bx, by = np.float32(float(ax) + d * np.cos(t)), np.float32(float(ay) + d * np.sin(t))
+        dx, dy = bx - ax, by - ay
+        if abs(float(dx * dx + dy * dy) - float(r2)) > 16 * float(np.spacing(r2)):
+            continue
+        xs.append(ax); ys.append(ay); qx.append(bx); qy.append(by)
+    n = len(xs)
+    x = np.array(xs + qx, np.float32)
+    y = np.array(ys + qy, np.float32)
+    S = np.float32(50.0 * 61)
+    o = oracle_mod.Oracle.from_arrays(x, y, np.array([n, n], np.uint32), np.array([0, 0, S, S], np.float32))
+    o.bin(float(r))
+    got = o.boundary_pairs()
+    want = dict(tests=n, neighbours=0, fma_dx_first=0, fma_dy_first=0, double_sum=0, any_form=0)
+    R2 = Fraction(float(r2))
+    for i in range(n):
+        dx, dy = np.float32(qx[i]) - np.float32(xs[i]), np.float32(qy[i]) - np.float32(ys[i])
+        sx, sy = dx * dx, dy * dy
+        canon = bool(sx + sy <= r2)
+        fa = Fraction(float(dx)) ** 2 + Fraction(float(sy))
+        fb = Fraction(float(dy)) ** 2 + Fraction(float(sx))
+        ra = bool(_round_f32(fa) <= r2)
+        rb = bool(_round_f32(fb) <= r2)
+        rd = bool(np.float32(float(Fraction(float(dx)) ** 2 + Fraction(float(dy)) ** 2)) <= r2)
+        want["neighbours"] += canon
+        want["fma_dx_first"] += ra != canon
+        want["fma_dy_first"] += rb != canon
+        want["double_sum"] += rd != canon
+        want["any_form"] += ra != canon or rb != canon or rd != canon
+    assert got == want
+    assert want["any_form"] > 0 and want["neighbours"] > 0  # the boundary band is exercised

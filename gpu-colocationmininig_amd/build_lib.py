@@ -27,7 +27,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-SOURCES = ["colo_device.hip", "colo_mlp.hip", "colo_noalloc.hip", "colo_miner.cpp"]
+SOURCES = ["colo_device.hip", "colo_mlp.hip", "colo_noalloc.hip", "colo_miner.cpp", "colo_host.cpp"]
+HOST_ONLY = {"colo_host.cpp"}  # plain C++, no device code
 
 
 def _run(cmd):
@@ -42,7 +43,7 @@ def build(verbose=False):
     objs = []
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        lang = ["-x", "hip"] if src.endswith(".cpp") and src not in HOST_ONLY else []
         out = _run([HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         if verbose and out.strip():
             print(out)

@@ -5,6 +5,7 @@
       Markdown table of per-kernel calls and durations (microseconds); the
       instantiations of a kernel template are summed unless --templates.
   traffic  --fetch F.csv --write W.csv --cfetch CF.csv --cwrite CW.csv --kernel NAME [--copy-bytes B]
+           [--workload-key TEXT]
       HBM bytes per launch of NAME from two `rocprofv3 --pmc FETCH_SIZE` /
       `--pmc WRITE_SIZE` runs (KiB per dispatch), each scaled by what the same
       counter reads for a plain device copy of B bytes (bench.py --copy-probe,
@@ -70,7 +71,8 @@ def traffic(a):
     return {"kernel": a.kernel, "dispatches": nd, "fetch_kib_per_dispatch_raw": f, "write_kib_per_dispatch_raw": w,
             "fetch_scale": fs, "write_scale": ws, "hbm_bytes_per_launch": f * 1024.0 * fs + w * 1024.0 * ws,
             "calibration": f"plain copy of {a.copy_bytes} B: FETCH_SIZE {cf:.1f} KiB, WRITE_SIZE {cw:.1f} KiB "
-                           f"per dispatch", "sources": [os.path.basename(p) for p in (a.fetch, a.write)]}
+                           f"per dispatch", "sources": [os.path.basename(p) for p in (a.fetch, a.write)],
+            "workload_key": a.workload_key}
 
 
 def main():
@@ -84,6 +86,7 @@ def main():
     for n in ("fetch", "write", "cfetch", "cwrite", "kernel"):
         t.add_argument(f"--{n}", required=True)
     t.add_argument("--copy-bytes", type=float, default=256 * 1024 * 1024)
+    t.add_argument("--workload-key", default=None, help="the workload description bench.py prints in config.workload")
     a = ap.parse_args()
     if a.cmd == "kernels":
         print(kernels(a.path, a.top, a.templates))

@@ -59,23 +59,25 @@ int main(int argc, char **argv) {
         fc[f] = PER;
         for (uint32_t i = 0; i < PER; i++) {
             uint32_t p = f * PER + i;
-            double cx = i % 3 == 0 ? 200.0 : urand() * 950.0, cy = i % 3 == 0 ? 300.0 : urand() * 950.0;
-            x[p] = (float)(cx + (i % 3 == 0 ? urand() * 40.0 : 0.0));
-            y[p] = (float)(cy + (i % 3 == 0 ? urand() * 40.0 : 0.0));
+            /* one instance in eight in a 60 x 60 cluster, the rest uniform */
+            const int hot = i % 8 == 0;
+            double cx = hot ? 200.0 : urand() * 950.0, cy = hot ? 300.0 : urand() * 950.0;
+            x[p] = (float)(cx + (hot ? urand() * 60.0 : 0.0));
+            y[p] = (float)(cy + (hot ? urand() * 60.0 : 0.0));
         }
     }
     const float bounds[4] = {0, 0, 1000, 1000};
     orc_ctx *c = orc_from_arrays(n, F, x, y, fc, bounds);
-    CHECK(c && orc_bin(c, 25.0f) == 0);
+    CHECK(c && orc_bin(c, 12.0f) == 0);
     uint64_t bp[6];
     orc_boundary_pairs(c, bp);
     CHECK(bp[0] > 0 && bp[1] <= bp[0]);
     char prefix[512];
     snprintf(prefix, sizeof prefix, "%s/san_", tmp);
     int err = 0;
-    char *rep = orc_mine(c, 25.0f, 0.2f, 0, 0, 1, 1, prefix, &err);
-    CHECK(rep && !err && strstr(rep, "end levels"));
-    char *rep2 = orc_mine(c, 25.0f, 0.2f, 0, 1, 0, 1, NULL, &err);
+    char *rep = orc_mine(c, 12.0f, 0.05f, 0, 0, 1, 1, prefix, &err);
+    CHECK(rep && !err && strstr(rep, "end levels 4"));
+    char *rep2 = orc_mine(c, 12.0f, 0.05f, 0, 1, 0, 1, NULL, &err);
     CHECK(rep2 && !err);
     orc_free_str(rep);
     orc_free_str(rep2);

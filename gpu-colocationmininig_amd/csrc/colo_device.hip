@@ -1279,6 +1279,177 @@ __device__ inline void set_member_bits(uint32_t *bt, const uint64_t *seg_off, co
     }
 }
 
+// Participation bits of a work item staged in LDS (count variants 10-13).
+// Every member and every last-feature instance of a row lies within r of
+// the row's anchor, hence in the anchor's 3x3 cells, whose cell ids lie
+// within gx + 1 of the anchor's; anchors come in cell order.  So the ranks
+// an item can touch in column c lie in a window: the item's anchor ranks
+// for the anchor column, and for the others the instances of the column's
+// feature in cells cell(first anchor) - gx - 1 .. cell(last anchor) + gx + 1
+// (two directory reads).  Each (candidate, column) whose window fits gets
+// words of an LDS bitmap; hits set bits there with LDS atomics (no return
+// value, no wait) and the item ends with one check and one global atomic
+// per non-zero word, instead of a check round trip per hit word.  Windows
+// that do not fit, and any rank outside its window, take the global path.
+constexpr uint32_t kLdsBitWords = 3072;  // 12 KiB of bits per workgroup
+constexpr uint32_t kWinCols = 8;         // columns (k <= 8: the specialised kernels)
+struct LdsWin {
+    uint32_t base_n;  // LDS word base (low 16 bits) and words (high 16 bits); 0 words = global path
+    uint32_t gword;   // segment word of LDS word 0
+};
+__device__ inline uint32_t win_base(const LdsWin &w) { return w.base_n & 0xffffu; }
+__device__ inline uint32_t win_words(const LdsWin &w) { return w.base_n >> 16; }
+
+// Sets bit `rank` of column c: in the LDS window when it holds the rank,
+// else in global memory (check, then atomic).
+__device__ inline void lds_or_bit(uint32_t *s_bits, const LdsWin &w, uint32_t *gseg, uint32_t rank) {
+    const uint32_t wi = (rank >> 5) - w.gword;
+    if (wi < win_words(w))
+        atomicOr(&s_bits[win_base(w) + wi], 1u << (rank & 31));
+    else
+        set_bit(gseg, rank);
+}
+
+// Windows of the item's rows [begin, end) for its ncand candidates (k
+// columns each), LDS bases assigned in candidate order while they fit;
+// clears the words in use.  Workgroup-wide (all threads call).  Anchor
+// column: the item's anchor ranks; other member columns: the exact rank
+// range of the item's rows (one min/max pass over them, shared by the run's
+// candidates); last column: from the first entry of the first anchor's
+// neighbour list to the last entry of the last anchor's (lists follow the
+// anchors' cell order), or the cell-range window above when either list is
+// empty.  A rank outside its window takes the global path, so a window only
+// has to be a good guess.
+__device__ inline void lds_setup(const DevData &D, const CandDesc *descs, uint32_t ncand, uint32_t k, uint64_t begin,
+                                 uint64_t end, uint32_t *s_bits, LdsWin (*s_win)[kWinCols], uint32_t *s_used,
+                                 uint32_t *s_mlo, uint32_t *s_mhi) {
+    const CandDesc &c0 = descs[0];
+    const uint32_t km1 = k - 1;
+    if (threadIdx.x < kWinCols) {
+        s_mlo[threadIdx.x] = ~0u;
+        s_mhi[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    {
+        uint32_t lo[kWinCols], hi[kWinCols];
+#pragma unroll
+        for (uint32_t m = 0; m < kWinCols; m++) {
+            lo[m] = ~0u;
+            hi[m] = 0u;
+        }
+        for (uint64_t i = begin + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+            for (uint32_t m = 1; m < kWinCols; m++)
+                if (m < km1) {
+                    const uint32_t v = c0.parent[i * km1 + m];
+                    lo[m] = v < lo[m] ? v : lo[m];
+                    hi[m] = v > hi[m] ? v : hi[m];
+                }
+#pragma unroll
+        for (uint32_t m = 1; m < kWinCols; m++)
+            if (m < km1) {
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) {
+                    const uint32_t ol = __shfl_xor(lo[m], d, 64), oh = __shfl_xor(hi[m], d, 64);
+                    lo[m] = ol < lo[m] ? ol : lo[m];
+                    hi[m] = oh > hi[m] ? oh : hi[m];
+                }
+                if (__lane_id() == 0) {
+                    atomicMin(&s_mlo[m], lo[m]);
+                    atomicMax(&s_mhi[m], hi[m]);
+                }
+            }
+    }
+    __syncthreads();
+    const uint32_t a_first = c0.parent[begin * km1], a_last = c0.parent[(end - 1) * km1];
+    const uint32_t f0s = D.fstart[c0.f[0]], t_first = a_first - f0s, t_last = a_last - f0s;
+    const uint32_t cf = D.scell[a_first], cl = D.scell[a_last];
+    const uint32_t lo_cell = cf > D.g.gx + 1 ? cf - D.g.gx - 1 : 0;
+    const uint32_t hi_cell = (cl + D.g.gx + 1 < D.g.cells ? cl + D.g.gx + 1 : D.g.cells - 1) + 1;
+    for (uint32_t e = threadIdx.x; e < ncand * k; e += kBlock) {
+        const uint32_t j = e / k, c = e - j * k;
+        const CandDesc &cd = descs[j];
+        const uint32_t f = cd.f[c], fs = D.fstart[f];
+        uint32_t lo, hi;
+        if (c == 0) {
+            lo = t_first;
+            hi = t_last + 1;
+        } else if (c < km1) {
+            lo = s_mlo[c] - fs;
+            hi = s_mhi[c] - fs + 1;
+        } else {
+            const uint64_t b0 = cd.nb_off[t_first] - cd.nb_base, b1 = cd.nb_off[t_first + 1] - cd.nb_base;
+            const uint64_t e0 = cd.nb_off[t_last] - cd.nb_base, e1 = cd.nb_off[t_last + 1] - cd.nb_base;
+            if (b1 > b0 && e1 > e0) {
+                lo = cd.nb_table[b0] - fs;
+                hi = cd.nb_table[e1 - 1] - fs + 1;
+            } else {
+                const uint32_t *dirf = D.dir + (uint64_t)f * (D.g.cells + 1);
+                lo = dirf[lo_cell] - fs;
+                hi = dirf[hi_cell] - fs;
+            }
+        }
+        const uint32_t lw = lo >> 5, hw = hi > lo ? (hi + 31) >> 5 : lw;
+        const uint32_t n = hw - lw < 0xffffu ? hw - lw : 0u;  // a window that large stays global
+        s_win[j][c] = LdsWin{n << 16, lw};
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t used = 0;
+        for (uint32_t j = 0; j < ncand; j++) {
+            uint32_t tot = 0;
+            for (uint32_t c = 0; c < k; c++) tot += win_words(s_win[j][c]);
+            const bool fits = used + tot <= kLdsBitWords;
+            for (uint32_t c = 0; c < k; c++) {
+                const uint32_t n = fits ? win_words(s_win[j][c]) : 0u;
+                s_win[j][c].base_n = (n << 16) | (fits ? used : 0u);
+                if (fits) used += n;
+            }
+        }
+        *s_used = used;
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < *s_used; w += kBlock) s_bits[w] = 0;
+    __syncthreads();
+}
+
+// Writes the item's LDS bits to the candidates' bitmaps: one check and one
+// atomic per non-zero word.  Workgroup-wide, after a barrier.
+__device__ inline void lds_flush(const CandDesc *descs, uint32_t ncand, uint32_t k, uint32_t *bits,
+                                 const uint32_t *s_bits, const LdsWin (*s_win)[kWinCols]) {
+    for (uint32_t j = 0; j < ncand; j++)
+        for (uint32_t c = 0; c < k; c++) {
+            const LdsWin w = s_win[j][c];
+            const uint32_t n = win_words(w);
+            if (!n) continue;
+            uint32_t *g = bits + descs[j].bits_off + descs[j].seg_off[c] + w.gword;
+            for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+                const uint32_t v = s_bits[win_base(w) + i];
+                if (v && (__hip_atomic_load(g + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & v) != v)
+                    atomicOr(g + i, v);
+            }
+        }
+}
+
+// LDS-staged participation bits of one candidate j for the rows of a wave
+// (row_multi and rows2w with LDSB): the members of a row with an
+// extension.  The anchor bit is set by the first lane of each run of lanes
+// with the same anchor and an extension; all 64 lanes must take part.
+template <int kArr>
+__device__ inline void lds_member_bits(uint32_t *s_bits, const LdsWin *win, uint32_t *bt, const uint64_t *seg_off,
+                                       const uint32_t *rk, uint32_t km1, bool hit) {
+    const uint32_t key = hit ? rk[0] : ~0u;
+    const uint32_t pk = __shfl_up(key, 1, 64);
+    if (!hit) return;
+    const bool lead = __lane_id() == 0 || pk != key;
+#pragma unroll
+    for (uint32_t m = 0; m < (uint32_t)kArr; m++) {
+        if (m >= km1) break;
+        if (m == 0 && !lead) continue;
+        lds_or_bit(s_bits, win[m], bt + seg_off[m], rk[m]);
+    }
+}
+
 // Level-k count over the rows of one parent table for ncand candidates that
 // extend it: the row and its members' coordinates are loaded once, and the
 // members' ranks too (every candidate extends the same parent features).
@@ -1287,10 +1458,11 @@ __device__ inline void set_member_bits(uint32_t *bt, const uint64_t *seg_off, co
 // loaded before this candidate's stores, one step ahead.  WAVE: all 64
 // lanes of the wave call together (act false for lanes without a row) and
 // the members' bits are combined over the wave.
-template <int KM1, bool WAVE = false>
+template <int KM1, bool WAVE = false, bool LDSB = false>
 __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
                                              uint32_t *slots, uint32_t *bits, unsigned long long *s_rows,
-                                             bool act = true) {
+                                             bool act = true, uint32_t *s_bits = nullptr,
+                                             const LdsWin (*s_win)[kWinCols] = nullptr) {
     const CandDesc &c0 = descs[0];
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
@@ -1332,6 +1504,7 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         uint32_t *bt = bits + cd.bits_off;
         uint32_t cnt = 0;
         BitAcc acc(bt + cd.seg_off[km1]);
+        const LdsWin wl = LDSB ? s_win[j][km1] : LdsWin{0, 0};
         auto test = [&](uint32_t q, float qx, float qy) {
             bool ok = true;
 #pragma unroll
@@ -1341,7 +1514,15 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
             }
             if (ok) {
                 cnt++;
-                acc.set(q - sl);
+                if (LDSB && win_words(wl)) {
+                    const uint32_t r = q - sl, w = (r >> 5) - wl.gword;
+                    if (w < win_words(wl))
+                        atomicOr(&s_bits[win_base(wl) + w], 1u << (r & 31));
+                    else
+                        acc.set(r);
+                } else {
+                    acc.set(q - sl);
+                }
             }
         };
         uint64_t e = b0;
@@ -1375,7 +1556,9 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
         acc.flush();
         if (act && cd.slots_off != kNoSlots) slots[cd.slots_off + i] = cnt;
         if (cnt) atomicAdd(&s_rows[j], (unsigned long long)cnt);
-        if (WAVE) {
+        if (LDSB) {
+            if (__ballot(cnt != 0)) lds_member_bits<kArr>(s_bits, s_win[j], bt, cd.seg_off, rk, km1, act && cnt != 0);
+        } else if (WAVE) {
             if (__ballot(cnt != 0)) wave_member_bits<kArr>(bt, cd.seg_off, rk, km1, cnt != 0);
         }
         else if (cnt)
@@ -1471,10 +1654,11 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
 // otherwise each row's list is walked in turn.  The members' bits are
 // combined over the wave as in joink_count_row_multi.  Results equal two
 // calls of joink_count_row_multi.
-template <int KM1>
+template <int KM1, bool LDSB = false>
 __device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
                                           uint64_t end, uint32_t *slots, uint32_t *bits,
-                                          unsigned long long *s_rows) {
+                                          unsigned long long *s_rows, uint32_t *s_bits = nullptr,
+                                          const LdsWin (*s_win)[kWinCols] = nullptr) {
     const CandDesc &c0 = descs[0];
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
@@ -1535,6 +1719,7 @@ __device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *desc
         uint32_t *bt = bits + cd.bits_off;
         uint32_t cnt0 = 0, cnt1 = 0;
         BitAcc acc(bt + cd.seg_off[km1]);
+        const LdsWin wl = LDSB ? s_win[j][km1] : LdsWin{0, 0};
         // pass 1: the anchor list of row i, tested against row i and, when
         // shared, row i + 1; pass 2: row i + 1's own list
         auto walk = [&](uint64_t b0, uint64_t b1, bool use0, bool use1) {
@@ -1548,7 +1733,17 @@ __device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *desc
                 }
                 cnt0 += ok0;
                 cnt1 += ok1;
-                if (ok0 || ok1) acc.set(q - sl);
+                if (ok0 || ok1) {
+                    if (LDSB && win_words(wl)) {
+                        const uint32_t r = q - sl, w = (r >> 5) - wl.gword;
+                        if (w < win_words(wl))
+                            atomicOr(&s_bits[win_base(wl) + w], 1u << (r & 31));
+                        else
+                            acc.set(r);
+                    } else {
+                        acc.set(q - sl);
+                    }
+                }
             };
             uint64_t e = b0;
             for (; e + kJoinUnroll <= b1; e += kJoinUnroll) {
@@ -1577,7 +1772,12 @@ __device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *desc
             if (act1) slots[cd.slots_off + i + 1] = cnt1;
         }
         if (cnt0 + cnt1) atomicAdd(&s_rows[j], (unsigned long long)(cnt0 + cnt1));
-        if (__ballot(cnt0 + cnt1 != 0)) {
+        if (LDSB) {
+            if (__ballot(cnt0 + cnt1 != 0)) {
+                lds_member_bits<kArr>(s_bits, s_win[j], bt, cd.seg_off, rk0, km1, act0 && cnt0 != 0);
+                lds_member_bits<kArr>(s_bits, s_win[j], bt, cd.seg_off, rk1, km1, act1 && cnt1 != 0);
+            }
+        } else if (__ballot(cnt0 + cnt1 != 0)) {
             wave_member_bits<kArr>(bt, cd.seg_off, rk0, km1, cnt0 != 0);
             wave_member_bits<kArr>(bt, cd.seg_off, rk1, km1, cnt1 != 0);
         }
@@ -1924,34 +2124,44 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
 // capped at 64 per lane, for 8 waves per SIMD; VAR 7 and 9 the code of
 // VAR 6 and 8 at 102 per lane, for 5 waves per SIMD (the compiler spills
 // what does not fit).
-constexpr int var_waves(int v) { return v == 4 || v == 5 ? 8 : v == 7 || v == 9 ? 5 : 1; }
+constexpr int var_waves(int v) { return v == 4 || v == 5 || v == 10 ? 8 : v == 7 || v == 9 || v == 11 ? 5 : 1; }
+// VAR 10-13: the code of VAR 4, 9, 0 and 8 with each item's participation
+// bits staged in LDS (lds_setup / lds_flush), for k = 3..8.
+constexpr bool var_lds(int v) { return v >= 10 && v <= 13; }
 
 template <int KM1, int VAR>
 __global__ __launch_bounds__(kBlock, var_waves(VAR)) void k_countk_batch(
     DevData D, const CandDesc *desc, const WorkItem *items, const uint64_t *seg, uint32_t *slots, uint32_t *bits,
     unsigned long long *rows, uint32_t *queue) {
     __shared__ unsigned long long s_rows[kMaxRun];
+    constexpr bool LDS = var_lds(VAR) && KM1 >= 2 && KM1 + 1 <= (int)kWinCols;
+    __shared__ uint32_t s_bits[LDS ? kLdsBitWords : 1];
+    __shared__ LdsWin s_win[LDS ? kMaxRun : 1][kWinCols];
+    __shared__ uint32_t s_used, s_mlo[kWinCols], s_mhi[kWinCols];
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         if (threadIdx.x < kMaxRun) s_rows[threadIdx.x] = 0;
         __syncthreads();
-        if (VAR == 0 || VAR == 4) {
+        if (LDS)
+            lds_setup(D, desc + wi.cand, wi.ncand, KM1 + 1, wi.begin, wi.end, s_bits, s_win, &s_used, s_mlo, s_mhi);
+        if (VAR == 0 || VAR == 4 || VAR == 10 || VAR == 12) {
             // waves stay converged over the rows so the member bits can be
             // combined over the wave
             for (uint64_t i0 = wi.begin + (threadIdx.x & ~63u); i0 < wi.end; i0 += kBlock) {
                 const uint64_t i = i0 + __lane_id();
-                joink_count_row_multi<KM1, true>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows, i < wi.end);
+                joink_count_row_multi<KM1, true, LDS>(D, desc + wi.cand, wi.ncand, i, slots, bits, s_rows,
+                                                      i < wi.end, s_bits, s_win);
             }
         } else if (VAR == 2) {
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                     joink_count_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots, bits, s_rows + j);
-        } else if (VAR == 8 || VAR == 9) {
+        } else if (VAR == 8 || VAR == 9 || VAR == 11 || VAR == 13) {
             // two adjacent rows per lane, waves converged
             for (uint64_t i0 = wi.begin + 2 * (uint64_t)(threadIdx.x & ~63u); i0 < wi.end; i0 += 2 * kBlock)
-                joink_count_rows2w<KM1>(D, desc + wi.cand, wi.ncand, i0 + 2 * __lane_id(), wi.end, slots, bits,
-                                        s_rows);
+                joink_count_rows2w<KM1, LDS>(D, desc + wi.cand, wi.ncand, i0 + 2 * __lane_id(), wi.end, slots, bits,
+                                             s_rows, s_bits, s_win);
         } else if (VAR == 6 || VAR == 7) {
             // four adjacent rows per thread, one list pass when they share the
             // anchor, else pairs as in VAR 3
@@ -1992,6 +2202,7 @@ __global__ __launch_bounds__(kBlock, var_waves(VAR)) void k_countk_batch(
             }
         }
         __syncthreads();
+        if (LDS) lds_flush(desc + wi.cand, wi.ncand, KM1 + 1, bits, s_bits, s_win);
         if (threadIdx.x < wi.ncand && s_rows[threadIdx.x]) atomicAdd(&rows[wi.cand + threadIdx.x], s_rows[threadIdx.x]);
     }
 }
@@ -2149,6 +2360,10 @@ template <int KM1> struct PickCount6 { static auto get() { return &k_countk_batc
 template <int KM1> struct PickCount7 { static auto get() { return &k_countk_batch<KM1, 7>; } };
 template <int KM1> struct PickCount8 { static auto get() { return &k_countk_batch<KM1, 8>; } };
 template <int KM1> struct PickCount9 { static auto get() { return &k_countk_batch<KM1, 9>; } };
+template <int KM1> struct PickCount10 { static auto get() { return &k_countk_batch<KM1, 10>; } };
+template <int KM1> struct PickCount11 { static auto get() { return &k_countk_batch<KM1, 11>; } };
+template <int KM1> struct PickCount12 { static auto get() { return &k_countk_batch<KM1, 12>; } };
+template <int KM1> struct PickCount13 { static auto get() { return &k_countk_batch<KM1, 13>; } };
 template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
 template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
 template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
@@ -2164,7 +2379,11 @@ int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
                            d_items, d_seg, d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
     else {
         const uint32_t var = c->tune.count_var_of(k);
-        hipLaunchKernelGGL(var == 9   ? pick_k<PickCount9>(k)
+        hipLaunchKernelGGL(var == 13   ? pick_k<PickCount13>(k)
+                           : var == 12 ? pick_k<PickCount12>(k)
+                           : var == 11 ? pick_k<PickCount11>(k)
+                           : var == 10 ? pick_k<PickCount10>(k)
+                           : var == 9 ? pick_k<PickCount9>(k)
                            : var == 8 ? pick_k<PickCount8>(k)
                            : var == 7 ? pick_k<PickCount7>(k)
                            : var == 6 ? pick_k<PickCount6>(k)
@@ -2622,10 +2841,10 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
 int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
     for (int k = 0; k < 16; k++)
-        if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 9) ||
+        if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 13) ||
             (t->item_rows_k[k] != COLO_TUNE_UNSET && (t->item_rows_k[k] < 64 || t->item_rows_k[k] > 65536)))
             return set_error(c, COLO_EARG, "launch shape out of range");
-    if (t->count_var > 9 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+    if (t->count_var > 13 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;

@@ -781,8 +781,8 @@ struct Miner {
     int tune_level(uint32_t k, std::vector<WorkItem> &items, std::vector<double> &iw, bool has_pred,
                    const std::function<void(uint32_t)> &build, uint32_t *d_slots, uint64_t bits_tot, uint64_t m) {
         if (k >= 16) return COLO_OK;
-        static const uint32_t vars[] = {0, 3, 4, 5, 6, 7, 8, 9}, sizes[] = {1024, 4096, 16384};
-        const uint32_t nv = k == 2 ? 1 : 8;  // level 2 has one count kernel
+        static const uint32_t vars[] = {0, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13}, sizes[] = {1024, 4096, 16384};
+        const uint32_t nv = k == 2 ? 1 : 12;  // level 2 has one count kernel
         hipEvent_t ev[2];
         COLO_HIP(hipEventCreate(&ev[0]));
         if (hipEventCreate(&ev[1]) != hipSuccess) {

@@ -273,7 +273,9 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         anchor, member bits combined over the wave; 4
  *                         and 5 (count only) = 0 and 3 with registers
  *                         capped for 8 waves per SIMD, 7 and 9 = 6 and 8
- *                         capped for 5; the autotuner tries 0 and 3-9
+ *                         capped for 5; 10-13 (count only) = 4, 9, 0 and 8
+ *                         with each work item's participation bits staged
+ *                         in LDS (k = 3..8); the autotuner tries 0 and 3-13
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between

@@ -290,7 +290,9 @@ def test_threshold_sweep(c2small, colo_mod, sched):
                                   dict(count_var=4, item_rows=256), dict(count_var=0, item_rows=64),
                                   dict(count_var=6), dict(count_var=7, item_rows=256),
                                   dict(count_var=8), dict(count_var=9, item_rows=64),
-                                  dict(slots_budget_kb=64), dict(count_var=7, slots_budget_kb=16, item_rows=256)])
+                                  dict(slots_budget_kb=64), dict(count_var=7, slots_budget_kb=16, item_rows=256),
+                                  dict(count_var=10), dict(count_var=11, item_rows=256), dict(count_var=12, item_rows=64),
+                                  dict(count_var=13, item_rows=16384)])
 def test_launch_shapes_identical(c2small, tune):
     """Every launch shape of the batched passes gives the oracle's report;
     so does a slots budget small enough that the largest candidates are
@@ -319,7 +321,7 @@ def test_autotune_and_cache(c2small, tmp_path):
     try:
         t1 = ctx.autotune(0.15, max_k=4, cache=cache)
         assert os.path.getsize(cache) > 0
-        assert t1["count_var_k"][3] <= 9 and all(v <= 9 or v == 0xFFFFFFFF for v in t1["count_var_k"]), t1
+        assert t1["count_var_k"][3] <= 13 and all(v <= 13 or v == 0xFFFFFFFF for v in t1["count_var_k"]), t1
         assert t1["item_rows_k"][2] in (1024, 4096, 16384), t1
         ctx.tune = old
         t2 = ctx.autotune(0.15, max_k=4, cache=cache)

@@ -513,13 +513,14 @@ class Context:
         return out
 
 
-MLP_IN, MLP_HIDDEN = 16, 64
-MLP_NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+MLP_IN, MLP_HIDDEN, MLP_OUT = 16, 64, 2
+MLP_NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2
 FEATURE_NAMES = ["k", "log1p N_first", "log1p N_last", "log1p N_min", "log1p N_max", "log1p anchors",
                  "log1p filter_first", "log1p filter_last", "filter bound", "log1p cells_first", "log1p cells_last",
                  "log1p 3x3 work(first,last)", "log1p R2(first,last)", "log1p R2(prev,last)",
                  "log occupancy_first", "log occupancy_last"]
-RECORD_FLOATS = MLP_IN + 2  # features, log1p(rows), k
+OUTPUT_NAMES = ["log1p rows", "log1p refinement tests"]
+RECORD_FLOATS = MLP_IN + MLP_OUT + 1  # features, log1p(rows), log1p(tests), k
 
 
 def _opts(theta, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, log_path):
@@ -528,16 +529,20 @@ def _opts(theta, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, mode
                     log_path.encode() if log_path else None)
 
 
-def read_log(path):
-    """Training records written by the miner's log_path hook."""
+def read_log(path, with_k=False):
+    """Training records written by the miner's log_path hook: features
+    (n x MLP_IN), targets (n x MLP_OUT: log1p rows, log1p refinement tests)
+    and, with with_k, the pattern size of each record."""
     a = np.fromfile(path, dtype=np.float32)
     a = a[: len(a) // RECORD_FLOATS * RECORD_FLOATS].reshape(-1, RECORD_FLOATS)
-    return a[:, :MLP_IN].copy(), a[:, MLP_IN].copy()
+    X, T = a[:, :MLP_IN].copy(), a[:, MLP_IN:MLP_IN + MLP_OUT].copy()
+    return (X, T, a[:, -1].astype(np.int64)) if with_k else (X, T)
 
 
 class CostModel:
-    """FP32 MLP 16 -> 64 -> 64 -> 64 -> 1 on the GPU (HIP forward, backward
-    and Adam kernels in colo_mlp.hip) predicting log1p(table rows)."""
+    """FP32 MLP 16 -> 64 -> 64 -> 64 -> 2 on the GPU (HIP forward, backward
+    and Adam kernels in colo_mlp.hip) predicting log1p(table rows) and
+    log1p(refinement tests) of a candidate."""
 
     def __init__(self, seed=0, h=None):
         _torch()
@@ -591,7 +596,7 @@ class CostModel:
 
     def predict(self, X):
         X = np.ascontiguousarray(X, np.float32)
-        y = np.zeros(len(X), np.float32)
+        y = np.zeros((len(X), MLP_OUT), np.float32)
         rc = lib().colo_mlp_predict(self.h, X.ctypes.data, len(X), y.ctypes.data)
         if rc:
             raise ColoError(rc, "predict")
@@ -599,7 +604,7 @@ class CostModel:
 
     def grad(self, X, T):
         X = np.ascontiguousarray(X, np.float32)
-        T = np.ascontiguousarray(T, np.float32)
+        T = np.ascontiguousarray(T, np.float32).reshape(len(X), MLP_OUT)
         g = np.zeros(MLP_NPARAMS, np.float32)
         loss = C.c_float()
         rc = lib().colo_mlp_grad(self.h, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data, C.byref(loss))
@@ -609,7 +614,7 @@ class CostModel:
 
     def train(self, X, T, steps=2000, batch=256, lr=1e-3, seed=0, normalise=True):
         X = np.ascontiguousarray(X, np.float32)
-        T = np.ascontiguousarray(T, np.float32)
+        T = np.ascontiguousarray(T, np.float32).reshape(len(X), MLP_OUT)
         if normalise:
             self.set_norm(X.mean(0), X.std(0) + 1e-6)
         loss = C.c_float()

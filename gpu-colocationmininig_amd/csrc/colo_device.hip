@@ -777,6 +777,59 @@ __global__ __launch_bounds__(kBlock) void k_pairwork(const FilterItem *items, co
     if (threadIdx.x == 0 && tot) atomicAdd(&out[it.cand], (unsigned long long)tot);
 }
 
+// Refinement tests of size-k candidates: one workgroup per chunk of
+// kFilterRuns parent rows of one candidate; each row adds the length of
+// its first instance's neighbour list.
+__global__ __launch_bounds__(kBlock) void k_refine_tests(const colo::RefineItem *items, const uint64_t *chunk_prefix,
+                                                         uint64_t n_items, unsigned long long *out) {
+    uint64_t b = blockIdx.x;
+    uint64_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (chunk_prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    const colo::RefineItem it = items[lo];
+    const uint64_t r0 = (b - chunk_prefix[lo]) * kFilterRuns;
+    const uint64_t r1 = r0 + kFilterRuns < it.rows ? r0 + kFilterRuns : it.rows;
+    uint64_t sum = 0;
+    for (uint64_t h = r0 + threadIdx.x; h < r1; h += kBlock) {
+        const uint32_t t = it.parent[h * it.km1] - it.fstart0;
+        sum += it.nb_off[t + 1] - it.nb_off[t];
+    }
+    uint64_t ex;
+    uint64_t tot = block_excl_scan(sum, ex);
+    if (threadIdx.x == 0 && tot) atomicAdd(&out[lo], (unsigned long long)tot);
+}
+
+int colo::refine_tests_batch(colo_ctx *c, const RefineItem *items, uint32_t n, uint64_t *out, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (!n) return COLO_OK;
+    std::vector<uint64_t> prefix(n);
+    uint64_t chunks = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        prefix[i] = chunks;
+        chunks += (items[i].rows + kFilterRuns - 1) / kFilterRuns;
+    }
+    const size_t ib = ((size_t)n * sizeof(RefineItem) + 7) & ~(size_t)7;
+    void *tmp;
+    int rc = scratch(c, ib + (size_t)n * 16, &tmp);
+    if (rc) return rc;
+    RefineItem *d_items = (RefineItem *)tmp;
+    uint64_t *d_prefix = (uint64_t *)((char *)tmp + ib);
+    uint64_t *d_out = d_prefix + n;
+    if ((rc = fill_u32((uint32_t *)d_out, 2ull * n, 0, s))) return rc;
+    COLO_HIP(hipMemcpyAsync(d_items, items, (size_t)n * sizeof(RefineItem), hipMemcpyHostToDevice, s));
+    COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+    if (chunks) {
+        hipLaunchKernelGGL(k_refine_tests, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_items, d_prefix, (uint64_t)n,
+                           (unsigned long long *)d_out);
+        COLO_HIP(hipGetLastError());
+    }
+    COLO_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    return COLO_OK;
+}
+
 int colo::pairwork_batch(colo_ctx *c, const uint32_t *pairs, uint32_t P, uint64_t *out, hipStream_t s) {
     colo_ctx *ctx_for_err = c;
     if (!P) return COLO_OK;

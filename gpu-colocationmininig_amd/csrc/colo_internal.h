@@ -314,6 +314,19 @@ void pi_from_segs(const colo_ctx *c, uint32_t k, const uint32_t *cand, const Seg
 int num_cus();
 // 3x3 join work of (first, last) pairs (P pairs, host in/out).
 int pairwork_batch(colo_ctx *c, const uint32_t *pairs, uint32_t P, uint64_t *out, hipStream_t s);
+// One size-k candidate of refine_tests_batch: its parent table (rows x km1
+// instance ids), the neighbour-list offsets of its (first, last) features
+// indexed by the first feature's sorted rank, and that feature's fstart.
+struct RefineItem {
+    const uint32_t *parent;
+    const uint64_t *nb_off;
+    uint64_t rows;
+    uint32_t km1, fstart0;
+};
+// Refinement tests of n size-k candidates (host in/out): the sum over each
+// one's parent rows of the neighbour-list length of the row's first
+// instance, i.e. the last-feature instances its count pass tests.
+int refine_tests_batch(colo_ctx *c, const RefineItem *items, uint32_t n, uint64_t *out, hipStream_t s);
 int mlp_predict_host(colo_mlp *m, const float *X, uint32_t n, float *y);
 }  // namespace colo
 

@@ -519,15 +519,18 @@ struct Miner {
         // costliest runs first so the tail of the launch is short.  Without
         // a model a run costs its rows x candidates and is cut into
         // 1024-row items; with the model (learned schedule) a run costs its
-        // rows plus the predicted rows of its tables, and items are cut to
-        // an equal predicted cost.
+        // parent rows plus the predicted refinement tests and table rows of
+        // its candidates, and items are cut to an equal predicted cost.
         auto runs = parent_runs(desc, k);
         std::vector<double> cost(m, 0.0);
         for (auto &r : runs) {
             double cst = (double)desc[r.first].n_anchor * r.second;
             if (pred) {
                 cst = (double)desc[r.first].n_anchor;
-                for (uint32_t j = 0; j < r.second; j++) cst += std::expm1((double)(*pred)[r.first + j]);
+                for (uint32_t j = 0; j < r.second; j++) {
+                    const float *y = &(*pred)[(r.first + j) * COLO_MLP_OUT];
+                    cst += std::expm1((double)std::max(0.f, y[0])) + std::expm1((double)std::max(0.f, y[1]));
+                }
             }
             cost[r.first] = cst;
         }
@@ -593,11 +596,23 @@ struct Miner {
         span(e0, mark(), k == 2 ? kL2Count : kCount, k);
         if (pred) {
             // pre-size the level's table arena from the predicted rows while
-            // the count pass runs (the exact sizes are known only after it)
+            // the count pass runs (the exact sizes are known only after it).
+            // Every row is a passed test, so a prediction is capped by the
+            // predicted tests and, for a pair, by its exact 3x3 join work.
+            // A reservation is a guess: it takes at most half of the pool's
+            // headroom, so that an over-prediction cannot push the cached
+            // blocks of earlier levels out of the pool.
             double words = 0;
-            for (uint64_t u = 0; u < m; u++) words += std::expm1((double)std::max(0.f, (*pred)[u])) * table_words(k);
+            for (uint64_t u = 0; u < m; u++) {
+                const float *y = &(*pred)[u * COLO_MLP_OUT];
+                double rows = std::expm1((double)std::max(0.f, std::min(y[0], y[1])));
+                if (k == 2 && !c->pairwork.empty())
+                    rows = std::min(rows, (double)c->pairwork[(size_t)desc[u].f[0] * c->F + desc[u].f[1]]);
+                words += rows * table_words(k);
+            }
             words *= 1.10;
-            if (words < 4e10) {
+            const double headroom = (double)c->pool.cap_bytes() - (double)c->pool.in_use;
+            if (words * 4 <= headroom / 2) {
                 arena_reserved_ = std::make_shared<DevBuf>(&c->pool);
                 if (!arena_reserved_->ensure((size_t)std::max(words, 1.0) * 4)) arena_reserved_.reset();
                 else st.arena_bytes += (double)arena_reserved_->bytes;
@@ -979,19 +994,48 @@ struct Miner {
         return COLO_OK;
     }
 
-    void log_records(uint32_t k, const std::vector<uint64_t> &idx, const std::vector<float> &X,
-                     const std::vector<CandRes> &res) {
-        if (!o.log_path || idx.empty()) return;
+    // Refinement tests of the counted candidates idx (the cost model's
+    // second target, measured for the log only): at size 2 the second
+    // feature's instances in the 3x3 cells of each anchor (the pair's join
+    // work), at size k the neighbours in the last feature of each parent
+    // row's first instance.
+    int refine_tests(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint64_t> &idx,
+                     const std::vector<Prev> &Lprev, std::vector<uint64_t> &out) {
+        out.assign(idx.size(), 0);
+        if (k == 2) {
+            for (size_t u = 0; u < idx.size(); u++)
+                out[u] = c->pairwork[(size_t)C[idx[u] * 2] * c->F + C[idx[u] * 2 + 1]];
+            return COLO_OK;
+        }
+        std::vector<RefineItem> items(idx.size());
+        for (size_t u = 0; u < idx.size(); u++) {
+            const uint32_t *cand = &C[idx[u] * k];
+            int64_t pi = find(Lprev, cand, k - 1);
+            const Prev *nb = nbr_list(cand[0], cand[k - 1]);
+            if (pi < 0 || !Lprev[pi].tptr || !nb) return set_error(c, COLO_EARG, "internal: refinement tests");
+            items[u] = RefineItem{Lprev[pi].tptr, nb->offs->as<uint64_t>() + nb->offs_at, Lprev[pi].rows, k - 1,
+                                  c->fstart[cand[0]]};
+        }
+        return refine_tests_batch(c, items.data(), (uint32_t)items.size(), out.data(), s);
+    }
+
+    int log_records(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint64_t> &idx,
+                    const std::vector<float> &X, const std::vector<CandRes> &res, const std::vector<Prev> &Lprev) {
+        if (!o.log_path || idx.empty()) return COLO_OK;
+        std::vector<uint64_t> tests;
+        int rc = refine_tests(k, C, idx, Lprev, tests);
+        if (rc) return rc;
         FILE *fp = fopen(o.log_path, "ab");
-        if (!fp) return;
-        std::vector<float> rec(COLO_MLP_IN + 2);
+        if (!fp) return set_error(c, COLO_EFILE, "cannot open the cost-model log");
+        std::vector<float> rec(COLO_MLP_RECORD);
         for (size_t u = 0; u < idx.size(); u++) {
             memcpy(rec.data(), &X[u * COLO_MLP_IN], COLO_MLP_IN * 4);
             rec[COLO_MLP_IN] = (float)std::log1p((double)res[idx[u]].rows);
-            rec[COLO_MLP_IN + 1] = (float)k;
+            rec[COLO_MLP_IN + 1] = (float)std::log1p((double)tests[u]);
+            rec[COLO_MLP_IN + 2] = (float)k;
             fwrite(rec.data(), 4, rec.size(), fp);
         }
-        fclose(fp);
+        return fclose(fp) == 0 ? COLO_OK : set_error(c, COLO_EFILE, "cannot write the cost-model log");
     }
 
     int run(char **report) {
@@ -1097,7 +1141,7 @@ struct Miner {
                 if (rc0) return rc0;
             }
             if (learned && !fidx.empty()) {
-                pred.resize(fidx.size());
+                pred.resize(fidx.size() * COLO_MLP_OUT);
                 int rc0 = mlp_predict_host(o.model, FX.data(), (uint32_t)fidx.size(), pred.data());
                 if (rc0) return set_error(c, rc0, "cost model prediction");
             }
@@ -1144,10 +1188,11 @@ struct Miner {
             double level_count_ms = ms_since(tl);
             if (learned)
                 for (size_t u = 0; u < fidx.size(); u++) {
-                    st.model_abs_err += std::fabs((double)pred[u] - std::log1p((double)res[fidx[u]].rows));
+                    st.model_abs_err +=
+                        std::fabs((double)pred[u * COLO_MLP_OUT] - std::log1p((double)res[fidx[u]].rows));
                     st.model_n += 1;
                 }
-            log_records(k, fidx, FX, res);
+            if ((rc = log_records(k, C, fidx, FX, res, Lprev))) return rc;
             rep.printf("level %u cand %llu pass %llu prev %llu\n", k, (unsigned long long)nC,
                        (unsigned long long)npass, (unsigned long long)L.size());
             for (uint64_t i = 0; i < nC; i++) {

@@ -1,16 +1,19 @@
-// colo_mlp.hip -- FP32 MLP cost model (16 -> 64 -> 64 -> 64 -> 1, ReLU)
+// colo_mlp.hip -- FP32 MLP cost model (16 -> 64 -> 64 -> 64 -> 2, ReLU)
 // with hand-written forward, backward and Adam kernels for gfx950.
 //
-// It predicts log1p(rows) of a candidate's instance table from 16 features
-// logged by the miner (pattern size, instance counts, parent rows,
-// filter counts and bound, cell occupancy; see colo_miner.cpp
-// cand_features).  The schedule uses it to order and partition the
-// persistent work queue and to pre-size the level's table arena; it never
-// decides a pattern, so mining results do not depend on it.
+// From 16 features logged by the miner (pattern size, instance counts,
+// parent rows, filter counts and bound, cell occupancy; see colo_miner.cpp
+// cand_features) it predicts two costs of a candidate: log1p(rows) of its
+// instance table and log1p(refinement tests) of its count pass (instances
+// of the last feature tested against its parent rows).  The schedule uses
+// them to order and partition the persistent work queue (tests and rows)
+// and to pre-size the level's table arena (rows); it never decides a
+// pattern, so mining results do not depend on it.
 //
 // Parameter layout (row-major, out x in): W1[64][16] b1[64] W2[64][64] b2[64]
-// W3[64][64] b3[64] W4[1][64] b4[1].  Inputs are standardised with the
-// model's mean/std before the first layer.  Loss: mean squared error.
+// W3[64][64] b3[64] W4[2][64] b4[2].  Inputs are standardised with the
+// model's mean/std before the first layer.  Loss: squared error summed over
+// the two outputs, averaged over the samples.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -26,14 +29,17 @@
 
 namespace {
 
-constexpr int kIn = COLO_MLP_IN, kH = COLO_MLP_HIDDEN;
+constexpr int kIn = COLO_MLP_IN, kH = COLO_MLP_HIDDEN, kOut = COLO_MLP_OUT;
 constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
-              oB3 = oW3 + kH * kH, oW4 = oB3 + kH, oB4 = oW4 + kH, kNP = oB4 + 1;
+              oB3 = oW3 + kH * kH, oW4 = oB3 + kH, oB4 = oW4 + kOut * kH, kNP = oB4 + kOut;
 static_assert(kNP == COLO_MLP_NPARAMS, "parameter count");
 constexpr int kTile = 32;  // samples per workgroup
+// the output layer maps one thread to each (sample, output) of a tile
+static_assert(kTile * kOut == kH, "output layer: one thread per sample and output");
 
 // Forward for a tile of samples; one thread per hidden neuron (64 threads).
-// Writes the three hidden activations (post-ReLU) and the output.
+// Writes the three hidden activations (post-ReLU) and the outputs
+// (Y[sample * kOut + o]).
 __global__ __launch_bounds__(kH) void k_mlp_forward(const float *P, const float *X, const float *mean,
                                                     const float *stdv, uint32_t n, float *H1, float *H2, float *H3,
                                                     float *Y) {
@@ -76,12 +82,13 @@ __global__ __launch_bounds__(kH) void k_mlp_forward(const float *P, const float 
             H3[(uint64_t)g * kH + j] = a3[s][j];
         }
     }
-    if (j < kTile) {
-        uint32_t g = s0 + j;
+    {
+        const int s = j / kOut, o = j - s * kOut;
+        uint32_t g = s0 + s;
         if (g < n) {
-            float acc = P[oB4];
-            for (int i = 0; i < kH; i++) acc += P[oW4 + i] * a3[j][i];
-            Y[g] = acc;
+            float acc = P[oB4 + o];
+            for (int i = 0; i < kH; i++) acc += P[oW4 + o * kH + i] * a3[s][i];
+            Y[(uint64_t)g * kOut + o] = acc;
         }
     }
 }
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(kH) void k_mlp_backward(const float *P, const float
     __shared__ float h1[kTile][kH + 1];
     __shared__ float h2[kTile][kH + 1];
     __shared__ float h3[kTile][kH + 1];
-    __shared__ float dy[kTile];
+    __shared__ float dy[kTile][kOut];
     const int j = threadIdx.x;
     const uint32_t s0 = blockIdx.x * kTile;
     const float inv = 1.0f / (float)n;
@@ -115,25 +122,33 @@ __global__ __launch_bounds__(kH) void k_mlp_backward(const float *P, const float
         h2[s][j] = v ? H2[(uint64_t)g * kH + j] : 0.f;
         h3[s][j] = v ? H3[(uint64_t)g * kH + j] : 0.f;
     }
-    if (j < kTile) {
-        uint32_t g = s0 + j;
-        float e = g < n ? Y[g] - T[g] : 0.f;
-        dy[j] = 2.f * e * inv;
+    {
+        const int s = j / kOut, o = j - s * kOut;
+        uint32_t g = s0 + s;
+        float e = g < n ? Y[(uint64_t)g * kOut + o] - T[(uint64_t)g * kOut + o] : 0.f;
+        dy[s][o] = 2.f * e * inv;
         if (g < n) atomicAdd(loss, e * e * inv);
     }
     __syncthreads();
-    // output layer: dW4[j] = sum_s dy * h3; d3 = dy * W4[j] * relu'(h3)
+    // output layer: dW4[o][j] = sum_s dy[s][o] h3[s][j];
+    // d3[s][j] = relu'(h3) sum_o dy[s][o] W4[o][j]
     {
-        float gw = 0.f, w4 = P[oW4 + j];
-        for (int s = 0; s < kTile; s++) {
-            gw += dy[s] * h3[s][j];
-            d3[s][j] = h3[s][j] > 0.f ? dy[s] * w4 : 0.f;
+        float w4[kOut];
+        for (int o = 0; o < kOut; o++) {
+            w4[o] = P[oW4 + o * kH + j];
+            float gw = 0.f;
+            for (int s = 0; s < kTile; s++) gw += dy[s][o] * h3[s][j];
+            atomicAdd(&G[oW4 + o * kH + j], gw);
         }
-        atomicAdd(&G[oW4 + j], gw);
-        if (j == 0) {
+        for (int s = 0; s < kTile; s++) {
+            float acc = 0.f;
+            for (int o = 0; o < kOut; o++) acc += dy[s][o] * w4[o];
+            d3[s][j] = h3[s][j] > 0.f ? acc : 0.f;
+        }
+        if (j < kOut) {
             float gb = 0.f;
-            for (int s = 0; s < kTile; s++) gb += dy[s];
-            atomicAdd(&G[oB4], gb);
+            for (int s = 0; s < kTile; s++) gb += dy[s][j];
+            atomicAdd(&G[oB4 + j], gb);
         }
     }
     __syncthreads();
@@ -219,8 +234,9 @@ static int mlp_cap(colo_mlp *m, uint32_t n) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     uint32_t c = std::max<uint32_t>(n, 256);
-    if (hipMalloc(&m->X, (size_t)c * kIn * 4) != hipSuccess || hipMalloc(&m->T, (size_t)c * 4) != hipSuccess ||
-        hipMalloc(&m->Y, (size_t)c * 4) != hipSuccess || hipMalloc(&m->H, (size_t)c * kH * 3 * 4) != hipSuccess) {
+    if (hipMalloc(&m->X, (size_t)c * kIn * 4) != hipSuccess || hipMalloc(&m->T, (size_t)c * kOut * 4) != hipSuccess ||
+        hipMalloc(&m->Y, (size_t)c * kOut * 4) != hipSuccess ||
+        hipMalloc(&m->H, (size_t)c * kH * 3 * 4) != hipSuccess) {
         m->cap = 0;
         return COLO_ENOMEM;
     }
@@ -254,7 +270,7 @@ int mlp_predict_host(colo_mlp *m, const float *X, uint32_t n, float *y) {
     if (rc) return rc;
     if (hipMemcpyAsync(m->X, X, (size_t)n * kIn * 4, hipMemcpyHostToDevice, m->s) != hipSuccess) return COLO_EHIP;
     if ((rc = mlp_forward(m, n, false))) return rc;
-    if (hipMemcpyAsync(y, m->Y, (size_t)n * 4, hipMemcpyDeviceToHost, m->s) != hipSuccess) return COLO_EHIP;
+    if (hipMemcpyAsync(y, m->Y, (size_t)n * kOut * 4, hipMemcpyDeviceToHost, m->s) != hipSuccess) return COLO_EHIP;
     return hipStreamSynchronize(m->s) == hipSuccess ? COLO_OK : COLO_EHIP;
 }
 }  // namespace colo
@@ -285,7 +301,7 @@ int colo_mlp_create(uint32_t seed, colo_mlp **out) {
     fill(oW1, kIn, kH * kIn);
     fill(oW2, kH, kH * kH);
     fill(oW3, kH, kH * kH);
-    fill(oW4, kH, kH);
+    fill(oW4, kH, kOut * kH);
     m->h_mean.assign(kIn, 0.f);
     m->h_std.assign(kIn, 1.f);
     std::vector<float> z(kNP, 0.f);
@@ -345,7 +361,7 @@ int colo_mlp_grad(colo_mlp *m, const float *X, const float *T, uint32_t n, float
     int rc = mlp_cap(m, n);
     if (rc) return rc;
     if (hipMemcpyAsync(m->X, X, (size_t)n * kIn * 4, hipMemcpyHostToDevice, m->s) != hipSuccess ||
-        hipMemcpyAsync(m->T, T, (size_t)n * 4, hipMemcpyHostToDevice, m->s) != hipSuccess)
+        hipMemcpyAsync(m->T, T, (size_t)n * kOut * 4, hipMemcpyHostToDevice, m->s) != hipSuccess)
         return COLO_EHIP;
     if ((rc = mlp_forward(m, n, true)) || (rc = mlp_backward(m, n))) return rc;
     if (grad && hipMemcpyAsync(grad, m->G, kNP * 4, hipMemcpyDeviceToHost, m->s) != hipSuccess) return COLO_EHIP;
@@ -362,7 +378,7 @@ int colo_mlp_train(colo_mlp *m, const float *X, const float *T, uint32_t n, uint
     std::mt19937 rng(seed);
     std::vector<uint32_t> perm(n);
     for (uint32_t i = 0; i < n; i++) perm[i] = i;
-    std::vector<float> bx((size_t)batch * kIn), bt(batch);
+    std::vector<float> bx((size_t)batch * kIn), bt((size_t)batch * kOut);
     uint32_t pos = n;
     float last = 0.f;
     for (uint32_t st = 0; st < steps; st++) {
@@ -373,7 +389,7 @@ int colo_mlp_train(colo_mlp *m, const float *X, const float *T, uint32_t n, uint
         for (uint32_t b = 0; b < batch; b++) {
             uint32_t i = perm[pos + b];
             memcpy(&bx[(size_t)b * kIn], X + (size_t)i * kIn, kIn * 4);
-            bt[b] = T[i];
+            memcpy(&bt[(size_t)b * kOut], T + (size_t)i * kOut, kOut * 4);
         }
         pos += batch;
         // the host batch arrays are rewritten next step: copy synchronously
@@ -402,7 +418,7 @@ int colo_mlp_save(colo_mlp *m, const char *path) {
     if (rc) return rc;
     FILE *fp = fopen(path, "wb");
     if (!fp) return COLO_EFILE;
-    const char magic[8] = {'c', 'o', 'l', 'o', 'm', 'l', 'p', '1'};
+    const char magic[8] = {'c', 'o', 'l', 'o', 'm', 'l', 'p', '2'};
     bool ok = fwrite(magic, 1, 8, fp) == 8 && fwrite(m->h_mean.data(), 4, kIn, fp) == (size_t)kIn &&
               fwrite(m->h_std.data(), 4, kIn, fp) == (size_t)kIn && fwrite(p.data(), 4, kNP, fp) == (size_t)kNP;
     fclose(fp);
@@ -415,7 +431,7 @@ int colo_mlp_load(const char *path, colo_mlp **out) {
     if (!fp) return COLO_EFILE;
     char magic[8];
     std::vector<float> mean(kIn), stdv(kIn), p(kNP);
-    bool ok = fread(magic, 1, 8, fp) == 8 && !memcmp(magic, "colomlp1", 8) &&
+    bool ok = fread(magic, 1, 8, fp) == 8 && !memcmp(magic, "colomlp2", 8) &&
               fread(mean.data(), 4, kIn, fp) == (size_t)kIn && fread(stdv.data(), 4, kIn, fp) == (size_t)kIn &&
               fread(p.data(), 4, kNP, fp) == (size_t)kNP;
     fclose(fp);

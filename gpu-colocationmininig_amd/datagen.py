@@ -83,21 +83,37 @@ def _clamp_cells(x, y, S, r):
     return x, y
 
 
-def training_config(rng, n_range=(100_000, 2_000_000), f_range=(10, 40)):
+def training_config(rng, n_range=(100_000, 2_000_000), f_range=(10, 40), wide=False):
     """Randomised parameters of the c4 training datasets (drawn around the
     c1/c2 generator): features in f_range (10-40), instances in n_range,
     hotspot fraction 0-0.6, sigma 0.1-2% of the side, r for 8-64 mean
     neighbours, theta 0.05-0.5, hotspots shared by all features (40%) or
     hosting 2..F features each.  Returns keyword overrides for
-    generate("c2", ...)."""
+    generate("c2", ...).
+
+    wide=True also draws (after the draws above, which it leaves as they
+    are) the regimes of the c3/c5 workloads: hotspot fraction 0-0.8, a
+    sigma range up to 2% of the side per hotspot, patterns planted up to
+    size 6, mined to sizes 4-6 (theta at least 0.1 above size 4), and half
+    of the time planted in runs of similar frequency rank (plant_band)."""
     F = int(rng.integers(f_range[0], f_range[1] + 1))
     lo = float(rng.uniform(100, 2000))
     hf = None if rng.random() < 0.4 else int(rng.integers(2, F + 1))
-    return dict(F=F, N=int(rng.integers(n_range[0], n_range[1] + 1)), hot_frac=float(rng.uniform(0.0, 0.6)),
-                hotspots=int(rng.integers(50, 2001)), sigma=(lo, lo), zipf=float(rng.uniform(0.5, 1.3)),
-                planted=[int(v) for v in rng.integers(3, 6, size=int(rng.integers(2, 8)))],
-                r=None, mean_nbrs=float(rng.uniform(8, 64)), theta=float(rng.uniform(0.05, 0.5)), max_k=4,
-                hot_feats=hf)
+    cfg = dict(F=F, N=int(rng.integers(n_range[0], n_range[1] + 1)), hot_frac=float(rng.uniform(0.0, 0.6)),
+               hotspots=int(rng.integers(50, 2001)), sigma=(lo, lo), zipf=float(rng.uniform(0.5, 1.3)),
+               planted=[int(v) for v in rng.integers(3, 6, size=int(rng.integers(2, 8)))],
+               r=None, mean_nbrs=float(rng.uniform(8, 64)), theta=float(rng.uniform(0.05, 0.5)), max_k=4,
+               hot_feats=hf)
+    if wide:
+        cfg["hot_frac"] = float(rng.uniform(0.0, 0.8))
+        cfg["sigma"] = (lo, float(rng.uniform(lo, 2000)))
+        cfg["max_k"] = int(rng.integers(4, 7))
+        if cfg["max_k"] > 4:  # bounds the candidate count of the deeper levels
+            cfg["theta"] = max(cfg["theta"], 0.1)
+        cfg["planted"] = [int(v) for v in rng.integers(3, 7, size=int(rng.integers(2, 21)))]
+        if rng.random() < 0.5 and F >= 12:
+            cfg["plant_band"] = (min(6, F // 4), F)
+    return cfg
 
 
 def generate(name, N=None, seed=None, **over):

@@ -210,8 +210,9 @@ typedef struct colo_mine_opts {
     int32_t hash_tables;    /* 1 = report a position-sensitive table checksum */
     const char *out_prefix; /* ColocationDegree_<k>.txt prefix, NULL = none   */
     colo_mlp *model;        /* cost model of COLO_SCHED_LEARNED              */
-    const char *log_path;   /* append (features, rows) records of every
-                               counted candidate (training data), or NULL   */
+    const char *log_path;   /* append (features, rows, refinement tests)
+                               records of every counted candidate (training
+                               data of the cost model), or NULL             */
 } colo_mine_opts;
 
 /* Per-run measurements (milliseconds unless stated).  The *_kernel_ms
@@ -311,23 +312,34 @@ int colo_autotune(colo_ctx *ctx, const colo_mine_opts *opts, const char *cache_p
 
 /* Candidate features the miner logs and the cost model reads (16 floats per
  * candidate, see COLO_FEATURE_NAMES in colo.py) and the model's shape:
- * 16 -> 64 -> 64 -> 64 -> 1, ReLU, FP32, predicting log1p(table rows).
+ * 16 -> 64 -> 64 -> 64 -> 2, ReLU, FP32.  Output 0 predicts log1p(table
+ * rows), output 1 log1p(refinement tests): the instances of the last
+ * feature the count pass tests against the candidate's parent rows (at
+ * size 2 the instances of the second feature in the 3x3 cells of each
+ * anchor, at size k > 2 the neighbours of each parent row's first instance
+ * in the last feature).  A log record is COLO_MLP_IN features, then
+ * log1p(rows), log1p(tests) and k.
  * No counterpart in the reference: these serve the learned schedule. */
 #define COLO_MLP_IN 16
 #define COLO_MLP_HIDDEN 64
-#define COLO_MLP_NPARAMS (64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+#define COLO_MLP_OUT 2
+#define COLO_MLP_NPARAMS (64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2)
+#define COLO_MLP_RECORD (COLO_MLP_IN + COLO_MLP_OUT + 1)
 
 int colo_mlp_create(uint32_t seed, colo_mlp **out);
 int colo_mlp_destroy(colo_mlp *m);
 int colo_mlp_set_params(colo_mlp *m, const float *params, size_t n);
 int colo_mlp_get_params(colo_mlp *m, float *params, size_t n);
 int colo_mlp_set_norm(colo_mlp *m, const float *mean, const float *stdv);
-/* y[i] = model(X[i]) for n rows of COLO_MLP_IN host floats. */
+/* y[i*COLO_MLP_OUT + o] = output o of model(X[i]) for n rows of
+ * COLO_MLP_IN host floats. */
 int colo_mlp_predict(colo_mlp *m, const float *X, uint32_t n, float *y);
-/* Gradient of the mean squared error over the n rows (HIP forward and
- * backward kernels); grad has COLO_MLP_NPARAMS floats. */
+/* Gradient of the loss (1/n) sum_i sum_o (y[i][o] - T[i][o])^2 over the n
+ * rows (HIP forward and backward kernels); T has n * COLO_MLP_OUT floats,
+ * grad COLO_MLP_NPARAMS. */
 int colo_mlp_grad(colo_mlp *m, const float *X, const float *T, uint32_t n, float *grad, float *loss);
-/* Adam training on the GPU: steps minibatches of `batch` rows. */
+/* Adam training on the GPU: steps minibatches of `batch` rows (T as in
+ * colo_mlp_grad). */
 int colo_mlp_train(colo_mlp *m, const float *X, const float *T, uint32_t n, uint32_t steps, uint32_t batch,
                    float lr, uint32_t seed, float *final_loss);
 int colo_mlp_save(colo_mlp *m, const char *path);

@@ -160,6 +160,41 @@ def cpu_vs_gpu_mining(colo, model, threads):
     return out
 
 
+def cpu_full(colo, model, threads, name):
+    """Complete mining of one workload by the oracle on `threads` host
+    threads and by the GPU (learned schedule, best of three after a warm-up),
+    reports with instance-table checksums compared."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import torch
+    x, y, fc, b, p = gen(name)
+    th = min(p["thetas"]) if "thetas" in p else p["theta"]
+    ctx = colo.Context(x, y, fc, b, p["r"])
+    sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
+    grep, _ = ctx.mine(th, max_k=p["max_k"], sched=sched, model=model, hash_tables=True)
+    best = None
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.rebin(p["r"])
+        ctx.mine(th, max_k=p["max_k"], sched=sched, model=model, hash_tables=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    ctx.close()
+    log(f"cpu full {name}: gpu {best:.3f} s; oracle on {threads} threads ...")
+    oracle.set_threads(threads)
+    os.environ["ORC_PROGRESS"] = "1"  # the oracle notes its progress on stderr
+    t0 = time.perf_counter()
+    o = oracle.Oracle.from_arrays(x, y, fc, b)
+    orep = o.mine(p["r"], th, max_k=p["max_k"], hash_tables=True)
+    cpu_s = time.perf_counter() - t0
+    out = {"workload": describe(p), "theta": th, "gpu_s": best, f"cpu_s_{threads}t": cpu_s, "cores": threads,
+           "identical": orep == grep, "kind": "port", **cpu_info()}
+    log(f"cpu full {name}: oracle {cpu_s:.1f} s, identical {out['identical']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +209,8 @@ def main():
     ap.add_argument("--no-model", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "colo_tune.txt"))
+    ap.add_argument("--cpu-full", default=None, metavar="WORKLOAD",
+                    help="only time complete mining of WORKLOAD by the oracle on the host cores beside the GPU")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (traffic-counter calibration)")
     args = ap.parse_args()
@@ -185,6 +222,11 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
+        return 0
+    if args.cpu_full:
+        import colo
+        model = None if args.no_model or not os.path.exists(args.model) else colo.CostModel.load(args.model)
+        print(json.dumps(cpu_full(colo, model, cpu_threads(), args.cpu_full)), flush=True)
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is a design "

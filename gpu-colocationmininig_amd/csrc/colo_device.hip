@@ -573,8 +573,9 @@ __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, co
 }
 
 // threshold_Kernel (K:545-573): ub_t = fl32(double(cnt)/double(fl32(N))),
-// pass iff min_t ub_t >= theta.
-__global__ void k_threshold(const uint32_t *counts, const uint32_t *cands, uint32_t C, uint32_t k,
+// pass iff min_t ub_t >= theta.  Without a pass array the flag replaces
+// counts[i*k], as the reference leaves it in countMap (K:570).
+__global__ void k_threshold(uint32_t *counts, const uint32_t *cands, uint32_t C, uint32_t k,
                             const uint32_t *fcount, float theta, uint8_t *pass) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
@@ -584,7 +585,10 @@ __global__ void k_threshold(const uint32_t *counts, const uint32_t *cands, uint3
         float ub = (float)q;
         if (t == 0 || PI > ub) PI = ub;
     }
-    pass[i] = PI >= theta ? 1 : 0;
+    if (pass)
+        pass[i] = PI >= theta ? 1 : 0;
+    else
+        counts[(uint64_t)i * k] = PI >= theta ? 1u : 0u;
 }
 
 int colo::filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
@@ -654,7 +658,7 @@ int colo::filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_
 }
 
 int colo::threshold_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
-                          const uint32_t *d_counts, uint8_t *d_pass, hipStream_t s) {
+                          uint32_t *d_counts, uint8_t *d_pass, hipStream_t s) {
     colo_ctx *ctx_for_err = c;
     if (C == 0) return COLO_OK;
     hipLaunchKernelGGL(k_threshold, dim3(grid_for(C)), dim3(kBlock), 0, s, d_counts, d_cands, C, k,

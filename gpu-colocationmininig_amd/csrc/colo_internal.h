@@ -274,9 +274,10 @@ int scan_slots_ws(const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *ws, h
 int filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
                  uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
 // The filter's pass flags (threshold_Kernel, K:545-573) from counts already
-// known (a threshold sweep keeps the theta-independent counts).
+// known (a threshold sweep keeps the theta-independent counts); with
+// d_pass NULL each flag replaces d_counts[c*k] (the reference's layout).
 int threshold_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,
-                    const uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
+                    uint32_t *d_counts, uint8_t *d_pass, hipStream_t s);
 // Batched passes over many candidates (one launch each).
 // d_seg: 9 item boundaries, one segment per XCD (see xcd_segments);
 // d_queue: 8 counters.

@@ -16,6 +16,9 @@
  *    scalebyConstant does nothing.
  *  - The filters leave the pass flag of candidate c in countMap[c*degree]
  *    as the reference does (K:570), with the same rule at degree 2.
+ *  - calcInstanceTableGeneral takes the members' features from the first
+ *    row of table1 (it returns ST_CUDAERR for an empty table, as the
+ *    reference does when there is nothing to launch).
  * Statuses: 0 on success, ST_CUDAERR (5) otherwise; nothing exits.
  */
 #ifndef COLO_DEVICEPROCESS_COMPAT_H
@@ -104,21 +107,13 @@ inline void CountCellFeatures(gridBox g, size_t fNum, size_t n, Integer *d_insta
 template <typename... A>
 inline StatusType initializeDeviceMemConst(A...) { return colo_compat_ctx() ? ST_NOERR : ST_CUDAERR; }
 
-// filter2 / filter2_2 (DeviceProcess.h:18-21): countMap[c*degree] = pass.
+// filter2 / filter2_2 (DeviceProcess.h:18-21): countMap[c*degree] = pass,
+// written on the device.
 inline void filter2_any(Integer *d_coloc, Integer C, Integer degree, Real theta, Integer *countMap) {
     colo_ctx *c = colo_compat_ctx();
     if (!c || !C) return;
-    uint8_t *d_pass = nullptr;
-    if (hipMalloc(&d_pass, C) != hipSuccess) return;
-    if (colo_filter(c, d_coloc, C, degree, theta, countMap, d_pass, nullptr) == COLO_OK) {
-        std::vector<uint8_t> pass(C);
-        std::vector<Integer> cm((size_t)C * degree);
-        (void)hipMemcpy(pass.data(), d_pass, C, hipMemcpyDeviceToHost);
-        (void)hipMemcpy(cm.data(), countMap, cm.size() * 4, hipMemcpyDeviceToHost);
-        for (Integer i = 0; i < C; i++) cm[(size_t)i * degree] = pass[i];
-        (void)hipMemcpy(countMap, cm.data(), cm.size() * 4, hipMemcpyHostToDevice);
-    }
-    (void)hipFree(d_pass);
+    (void)colo_filter(c, d_coloc, C, degree, theta, countMap, nullptr, nullptr);
+    (void)hipStreamSynchronize(nullptr);
 }
 inline void filter2(int *, SInteger *, gridBox, Integer, Integer *, Real PIthreshold, Integer *d_coloc,
                     Integer candiColocCounter, Integer degree, Integer *countMap) {
@@ -159,18 +154,26 @@ inline StatusType generalInstanceTableSlotCounter(size_t R, Integer *slots, Inte
                                nullptr));
 }
 
-// calcInstanceTableGeneral (DeviceProcess.h:9): coloc is not passed by the
-// reference; the last feature and the prefix come from the caller's state,
-// so this shim takes the candidate from colo_compat_candidate().
-inline std::vector<Integer> &colo_compat_candidate() {
-    static std::vector<Integer> v;
-    return v;
+// calcInstanceTableGeneral (DeviceProcess.h:9): the reference passes only
+// the last feature; the features of the other members are those of the
+// instances in table1's first row (instance ids are grouped by feature).
+inline Integer feature_of_instance(Integer id) {
+    const std::vector<Integer> &fs = colo_compat_fstart();
+    size_t lo = 0, hi = fs.size();
+    while (hi - lo > 1) {
+        const size_t mid = (lo + hi) / 2;
+        if (fs[mid] <= id) lo = mid; else hi = mid;
+    }
+    return (Integer)lo;
 }
 inline StatusType calcInstanceTableGeneral(size_t R, Integer *slots, Index64 *, Integer *d_intermediate,
-                                           Integer degree, Integer *table1, size_t, Integer) {
+                                           Integer degree, Integer *table1, size_t, Integer lastFeatureID) {
     colo_ctx *c = colo_compat_ctx();
-    const std::vector<Integer> &cand = colo_compat_candidate();
-    if (cand.size() != degree) return ST_CUDAERR;
+    if (!c || R == 0 || degree < 3) return ST_CUDAERR;  // nothing to launch (cu:94-105)
+    std::vector<Integer> cand(degree);
+    if (hipMemcpy(cand.data(), table1, (degree - 1) * 4, hipMemcpyDeviceToHost) != hipSuccess) return ST_CUDAERR;
+    for (Integer m = 0; m + 1 < degree; m++) cand[m] = feature_of_instance(cand[m]);
+    cand[degree - 1] = lastFeatureID;
     uint64_t *offs = nullptr;
     if (hipMalloc(&offs, (R + 1) * 8) != hipSuccess) return ST_CUDAERR;
     int rc = colo_scan_slots(c, slots, R, offs, nullptr, nullptr);

@@ -103,7 +103,9 @@ int colo_binning_export(colo_ctx *ctx, uint32_t *d_cell_of, uint32_t *d_sorted_i
 /* Cell-aggregate upper-bound filter for C candidates of size k (d_cands:
  * C*k feature ids, device).  d_counts (C*k u32) receives the per-feature
  * counts summed over the cells of qualifying 2x2 quadrants; d_pass (C
- * bytes) receives min_t fl32(cnt_t / N_t) >= theta.
+ * bytes) receives min_t fl32(cnt_t / N_t) >= theta.  With d_pass NULL the
+ * pass flag of candidate c replaces d_counts[c*k], where the reference's
+ * threshold_Kernel leaves it (DeviceProcess_Kernel.h:570).
  * Replaces: filter2 (DeviceProcess.h:18-19, cu:219-235) and filter2_2
  * (DeviceProcess.h:20-21, cu:237-249; applied with the same rule). */
 int colo_filter(colo_ctx *ctx, const uint32_t *d_cands, uint32_t C, uint32_t k, float theta,

@@ -898,6 +898,8 @@ static int64_t find_pattern(const prev_t *L, uint64_t P, uint32_t w, const uint3
  * the report records it on a "q10" line and the tables are built anyway.
  * pi_mode 0 = compat numerators (cu:108-132), 1 = exact.  max_k = 0 means no
  * cap.  Returns a malloc'd report (free with orc_free_str). */
+static double wall_s(void) { return omp_get_wtime(); }
+
 char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, int filter_on,
                int hash_tables, const char *out_prefix, int *err) {
     sbuf b = {0};
@@ -919,6 +921,10 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
     uint32_t k = 2;
     uint64_t total_prev = 0;
     uint32_t levels = 0;
+    /* ORC_PROGRESS set: a line on stderr at least every 10 s of candidates
+     * (long runs under a supervisor that kills silent processes) */
+    const int progress = getenv("ORC_PROGRESS") != NULL;
+    double last_note = wall_s();
     while (nC > 0 && k <= F && (max_k == 0 || k <= max_k)) {
         levels++;
         uint32_t *fcnt = (uint32_t *)calloc(nC * k, sizeof(uint32_t));
@@ -932,6 +938,11 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
         sbuf lb = {0}; lb.cap = 4096; lb.s = (char *)malloc(lb.cap); lb.s[0] = 0;
         for (uint64_t i = 0; i < nC; i++) {
             const uint32_t *cand = C + i * k;
+            if (progress && wall_s() - last_note >= 10.0) {
+                last_note = wall_s();
+                fprintf(stderr, "[oracle] level %u: candidate %llu of %llu\n", k, (unsigned long long)i,
+                        (unsigned long long)nC);
+            }
             sb_printf(&lb, "c");
             for (uint32_t m = 0; m < k; m++) sb_printf(&lb, " %u", cand[m]);
             sb_printf(&lb, " f");
@@ -982,6 +993,9 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
         }
         sb_printf(&b, "level %u cand %llu pass %llu prev %llu\n", k, (unsigned long long)nC,
                   (unsigned long long)npass, (unsigned long long)P);
+        if (progress)
+            fprintf(stderr, "[oracle] level %u done: %llu candidates, %llu prevalent\n", k, (unsigned long long)nC,
+                    (unsigned long long)P);
         sb_printf(&b, "%s", lb.s);
         free(lb.s);
         total_prev += P;

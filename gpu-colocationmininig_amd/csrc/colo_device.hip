@@ -693,14 +693,15 @@ __device__ inline int row_spans(const DevGrid &g, const uint32_t *dirf, uint32_t
     }
     uint32_t r0 = ay ? ay - 1 : 0;
     uint32_t r1 = ay + 1 < g.gy ? ay + 1 : g.gy - 1;
-    int m = 0;
-    for (uint32_t ry = r0; ry <= r1; ry++) {
-        uint64_t base = (uint64_t)ry * g.gx;
-        rs.lo[m] = dirf[base + cl];
-        rs.hi[m] = dirf[base + ch + 1];
-        m++;
+    // static indices keep rs in registers (rows past r1 repeat row r1)
+#pragma unroll
+    for (uint32_t i = 0; i < 3; i++) {
+        const uint32_t ry = r0 + i <= r1 ? r0 + i : r1;
+        const uint64_t base = (uint64_t)ry * g.gx;
+        rs.lo[i] = dirf[base + cl];
+        rs.hi[i] = dirf[base + ch + 1];
     }
-    return m;
+    return (int)(r1 - r0 + 1);
 }
 
 __device__ inline float sqdist(float x1, float y1, float x2, float y2) {

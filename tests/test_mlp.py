@@ -234,6 +234,35 @@ def test_learned_overflow_fallback(colo_mod, datagen):
     assert st["overflows"] >= 1
 
 
+@pytest.mark.gpu
+def test_learned_half_prediction_releases_reservation(colo_mod, datagen):
+    """The committed model with its rows output lowered by log 2, 4, 8, ...
+    (until one table arena is under-sized) predicts a fraction of the
+    tables: a reservation that is large but short.  The exact allocation
+    must replace the reservation, not add to it: results equal the
+    persistent schedule's, and the pool's peak stays within a quarter of
+    the persistent schedule's (holding both would add the reservation to
+    the level's tables)."""
+    print("[case] learned schedule with under-sized predictions", flush=True)
+    ctx, p = _ctx(colo_mod, datagen, "c2", 120000, 997)
+    base = colo_mod.CostModel.load(os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin"))
+    rep_p, st_p = ctx.mine(p["theta"], max_k=4, sched=colo_mod.COLO_SCHED_PERSISTENT)
+    for shift in (2.0, 4.0, 8.0, 16.0, 64.0):
+        m = colo_mod.CostModel.load(os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin"))
+        par = base.params.copy()
+        par[-2] -= np.float32(np.log(shift))  # b4[0]: the log1p(rows) output
+        m.params = par
+        rep_l, st_l = ctx.mine(p["theta"], max_k=4, sched=colo_mod.COLO_SCHED_LEARNED, model=m)
+        assert rep_l == rep_p, shift
+        print(f"[short] rows / {shift:g}: overflows {st_l['overflows']} of {st_l['arenas']} arenas, reserved "
+              f"{st_l['arena_bytes'] / 1e6:.1f} MB; peak learned {st_l['pool_peak_bytes'] / 1e6:.1f} MB, persistent "
+              f"{st_p['pool_peak_bytes'] / 1e6:.1f} MB", flush=True)
+        if st_l["overflows"]:
+            break
+    assert st_l["overflows"] >= 1
+    assert st_l["pool_peak_bytes"] <= 1.25 * st_p["pool_peak_bytes"]
+
+
 # Largest time of the learned schedule, as a fraction of the persistent
 # schedule's, that the gain test accepts on its held-out dataset.
 GAIN_BOUND = 0.95

@@ -720,6 +720,16 @@ constexpr int kWriteUnroll = 8;  // the same for the level-k write, which runs f
 typedef __attribute__((address_space(1))) const uint32_t gcu32;
 __device__ inline const gcu32 *as_global(const uint32_t *p) { return (const gcu32 *)p; }
 
+// List bounds [b0, b1) of anchor rank t from a u64 row-offsets array:
+// entries t and t + 1 in one 16-byte load (8-byte aligned).
+typedef uint64_t u64x2a8 __attribute__((ext_vector_type(2), aligned(8)));
+typedef __attribute__((address_space(1))) const u64x2a8 gcu64x2;
+__device__ inline void nb_bounds(const uint64_t *off, uint64_t base, uint64_t t, uint64_t &b0, uint64_t &b1) {
+    const u64x2a8 v = *(const gcu64x2 *)(off + t);
+    b0 = v.x - base;
+    b1 = v.y - base;
+}
+
 __device__ inline void set_bit(uint32_t *bits, uint64_t i) {
     uint32_t m = 1u << (i & 31);
     uint32_t *w = bits + (i >> 5);
@@ -1119,7 +1129,8 @@ __device__ inline void joink_count_row_nbr(const DevData &D, const uint32_t *row
         my[m] = D.y[row[m]];
     }
     const uint32_t t = row[0] - D.fstart[feat[0]];
-    const uint64_t b0 = nb_off[t] - nb_base, b1 = nb_off[t + 1] - nb_base;
+    uint64_t b0, b1;
+    nb_bounds(nb_off, nb_base, t, b0, b1);
     const uint32_t sl = D.fstart[fl];
     uint32_t cnt = 0;
     BitAcc acc(bits + seg_off[km1]);
@@ -1149,7 +1160,8 @@ __device__ inline void joink_write_row_nbr(const DevData &D, const uint32_t *row
         my[m] = D.y[row[m]];
     }
     const uint32_t t = row[0] - D.fstart[feat[0]];
-    const uint64_t b0 = nb_off[t] - nb_base, b1 = nb_off[t + 1] - nb_base;
+    uint64_t b0, b1;
+    nb_bounds(nb_off, nb_base, t, b0, b1);
     const uint32_t k = km1 + 1;
     uint32_t *out = table + off * k;
     uint32_t w = 0;
@@ -1436,8 +1448,10 @@ __device__ inline void lds_setup(const DevData &D, const CandDesc *descs, uint32
             lo = s_mlo[c] - fs;
             hi = s_mhi[c] - fs + 1;
         } else {
-            const uint64_t b0 = cd.nb_off[t_first] - cd.nb_base, b1 = cd.nb_off[t_first + 1] - cd.nb_base;
-            const uint64_t e0 = cd.nb_off[t_last] - cd.nb_base, e1 = cd.nb_off[t_last + 1] - cd.nb_base;
+            uint64_t b0, b1;
+            nb_bounds(cd.nb_off, cd.nb_base, t_first, b0, b1);
+            uint64_t e0, e1;
+            nb_bounds(cd.nb_off, cd.nb_base, t_last, e0, e1);
             if (b1 > b0 && e1 > e0) {
                 lo = cd.nb_table[b0] - fs;
                 hi = cd.nb_table[e1 - 1] - fs + 1;
@@ -1544,16 +1558,14 @@ __device__ inline void joink_count_row_multi(const DevData &D, const CandDesc *d
     const uint32_t t = rk[0];
     uint64_t nb0 = 0, nb1 = 0;
     if (act) {
-        nb0 = c0.nb_off[t] - c0.nb_base;
-        nb1 = c0.nb_off[t + 1] - c0.nb_base;
+        nb_bounds(c0.nb_off, c0.nb_base, t, nb0, nb1);
     }
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
         const uint64_t b0 = nb0, b1 = nb1;
         if (act && j + 1 < ncand) {
             const CandDesc &cn = descs[j + 1];
-            nb0 = cn.nb_off[t] - cn.nb_base;
-            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+            nb_bounds(cn.nb_off, cn.nb_base, t, nb0, nb1);
         }
         // descriptor fields in registers: the loop's bit stores could alias
         // the descriptor, which would otherwise be re-read every entry
@@ -1650,14 +1662,14 @@ __device__ inline void joink_count_rows2(const DevData &D, const CandDesc *descs
         }
     }
     const uint32_t t = rk0[0];
-    uint64_t nb0 = c0.nb_off[t] - c0.nb_base, nb1 = c0.nb_off[t + 1] - c0.nb_base;
+    uint64_t nb0, nb1;
+    nb_bounds(c0.nb_off, c0.nb_base, t, nb0, nb1);
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
         const uint64_t b0 = nb0, b1 = nb1;
         if (j + 1 < ncand) {
             const CandDesc &cn = descs[j + 1];
-            nb0 = cn.nb_off[t] - cn.nb_base;
-            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+            nb_bounds(cn.nb_off, cn.nb_base, t, nb0, nb1);
         }
         const gcu32 *nbt = as_global(cd.nb_table);
         const uint32_t sl = D.fstart[cd.f[km1]];
@@ -1751,12 +1763,10 @@ __device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *desc
     const uint32_t t0 = rk0[0], t1 = rk1[0];
     uint64_t na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
     if (act0) {
-        na0 = c0.nb_off[t0] - c0.nb_base;
-        na1 = c0.nb_off[t0 + 1] - c0.nb_base;
+        nb_bounds(c0.nb_off, c0.nb_base, t0, na0, na1);
     }
     if (own1) {
-        nb0 = c0.nb_off[t1] - c0.nb_base;
-        nb1 = c0.nb_off[t1 + 1] - c0.nb_base;
+        nb_bounds(c0.nb_off, c0.nb_base, t1, nb0, nb1);
     }
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
@@ -1764,12 +1774,10 @@ __device__ inline void joink_count_rows2w(const DevData &D, const CandDesc *desc
         if (j + 1 < ncand) {
             const CandDesc &cn = descs[j + 1];
             if (act0) {
-                na0 = cn.nb_off[t0] - cn.nb_base;
-                na1 = cn.nb_off[t0 + 1] - cn.nb_base;
+                nb_bounds(cn.nb_off, cn.nb_base, t0, na0, na1);
             }
             if (own1) {
-                nb0 = cn.nb_off[t1] - cn.nb_base;
-                nb1 = cn.nb_off[t1 + 1] - cn.nb_base;
+                nb_bounds(cn.nb_off, cn.nb_base, t1, nb0, nb1);
             }
         }
         const gcu32 *nbt = as_global(cd.nb_table);
@@ -1870,14 +1878,14 @@ __device__ inline void joink_count_rowsR(const DevData &D, const CandDesc *descs
         }
     }
     const uint32_t t = rk[0][0];
-    uint64_t nb0 = c0.nb_off[t] - c0.nb_base, nb1 = c0.nb_off[t + 1] - c0.nb_base;
+    uint64_t nb0, nb1;
+    nb_bounds(c0.nb_off, c0.nb_base, t, nb0, nb1);
     for (uint32_t j = 0; j < ncand; j++) {
         const CandDesc &cd = descs[j];
         const uint64_t b0 = nb0, b1 = nb1;
         if (j + 1 < ncand) {
             const CandDesc &cn = descs[j + 1];
-            nb0 = cn.nb_off[t] - cn.nb_base;
-            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+            nb_bounds(cn.nb_off, cn.nb_base, t, nb0, nb1);
         }
         const gcu32 *nbt = as_global(cd.nb_table);
         const uint32_t sl = D.fstart[cd.f[km1]];
@@ -1995,7 +2003,8 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
         const uint32_t t = __shfl(anc, gs, 64) - fa0;
         for (uint32_t j = 0; j < ncand; j++) {
             const CandDesc &cd = descs[j];
-            const uint64_t b0 = cd.nb_off[t] - cd.nb_base, b1 = cd.nb_off[t + 1] - cd.nb_base;
+            uint64_t b0, b1;
+            nb_bounds(cd.nb_off, cd.nb_base, t, b0, b1);
             const uint32_t m = (uint32_t)(b1 - b0);
             const uint32_t sl = D.fstart[cd.f[km1]];
             uint32_t *bt = WRITE ? nullptr : bits + cd.bits_off;
@@ -2295,15 +2304,13 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
     // wave-uniform, so descriptor fields remain scalar loads)
     uint64_t nb0 = 0, nb1 = 0;
     if (todo & 1) {
-        nb0 = c0.nb_off[t] - c0.nb_base;
-        nb1 = c0.nb_off[t + 1] - c0.nb_base;
+        nb_bounds(c0.nb_off, c0.nb_base, t, nb0, nb1);
     }
     for (uint32_t j = 0; j < ncand; j++) {
         const uint64_t b0 = nb0, b1 = nb1;
         if (j + 1 < ncand && ((todo >> (j + 1)) & 1)) {
             const CandDesc &cn = descs[j + 1];
-            nb0 = cn.nb_off[t] - cn.nb_base;
-            nb1 = cn.nb_off[t + 1] - cn.nb_base;
+            nb_bounds(cn.nb_off, cn.nb_base, t, nb0, nb1);
         }
         if (!((todo >> j) & 1)) continue;
         const CandDesc &cd = descs[j];

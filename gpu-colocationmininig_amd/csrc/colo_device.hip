@@ -450,9 +450,10 @@ __global__ void k_runs(const uint32_t *run_flag, const uint64_t *run_idx, const 
     run_cell[run_idx[p]] = scell[p];
 }
 
-// Presence mask of every cell (bit f set when feature f occurs), from the runs.
+// Presence mask of every cell (bit f of word f / 64 set when feature f
+// occurs; W words per cell), from the runs.
 __global__ void k_cellmask(const uint32_t *run_cell, const uint64_t *run_off, uint32_t F, const uint64_t *dH,
-                           unsigned long long *cellmask) {
+                           uint32_t W, unsigned long long *cellmask) {
     uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= *dH) return;
     uint32_t lo = 0, hi = F;
@@ -460,17 +461,21 @@ __global__ void k_cellmask(const uint32_t *run_cell, const uint64_t *run_off, ui
         uint32_t mid = (lo + hi) >> 1;
         if (run_off[mid] <= h) lo = mid; else hi = mid;
     }
-    atomicOr(&cellmask[run_cell[h]], 1ull << lo);
+    atomicOr(&cellmask[(uint64_t)run_cell[h] * W + (lo >> 6)], 1ull << (lo & 63));
 }
 
-// Quadrant masks: OR of the 4 cells of quadrant (qx, qy), qx < gx-1, qy < gy-1.
-__global__ void k_qmask(const uint64_t *cellmask, uint32_t gx, uint32_t gy, uint64_t *qmask) {
-    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t nq = (uint64_t)(gx - 1) * (gy - 1);
-    if (q >= nq) return;
+// Quadrant masks: OR of the 4 cells of quadrant (qx, qy), qx < gx-1, qy < gy-1
+// (one thread per quadrant and mask word).
+__global__ void k_qmask(const uint64_t *cellmask, uint32_t gx, uint32_t gy, uint32_t W, uint64_t *qmask) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nq = (uint64_t)(gx - 1) * (gy - 1);
+    if (i >= nq * W) return;
+    const uint64_t q = i / W;
+    const uint32_t w = (uint32_t)(i - q * W);
     uint32_t qy = (uint32_t)(q / (gx - 1)), qx = (uint32_t)(q - (uint64_t)qy * (gx - 1));
     uint64_t c0 = (uint64_t)qy * gx + qx;
-    qmask[q] = cellmask[c0] | cellmask[c0 + 1] | cellmask[c0 + gx] | cellmask[c0 + gx + 1];
+    qmask[i] = cellmask[c0 * W + w] | cellmask[(c0 + 1) * W + w] | cellmask[(c0 + gx) * W + w] |
+               cellmask[(c0 + gx + 1) * W + w];
 }
 
 // Binning results for the host, in one block (u64 words): [0] bounds flag,
@@ -517,11 +522,13 @@ struct FilterGroup {
     uint64_t run_begin, run_end;
 };
 
+// W: 64-bit words of a feature mask (features <= 64 W).
+template <int W>
 __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, const uint64_t *chunk_prefix,
                                                    uint64_t n_groups, const uint32_t *memb, const uint64_t *cmask,
                                                    uint32_t k, const uint32_t *run_cell, const uint32_t *dir,
                                                    const uint64_t *qmask, DevGrid g, uint32_t *counts) {
-    __shared__ uint64_t s_q[4][kBlock];
+    __shared__ uint64_t s_q[W][4][kBlock];
     __shared__ uint32_t s_n[kBlock];
     uint64_t b = blockIdx.x;
     uint64_t lo = 0, hi = n_groups;
@@ -535,12 +542,18 @@ __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, co
     const uint32_t *dirf = dir + (uint64_t)G.f * (g.cells + 1);
     const bool mine = threadIdx.x < G.n_memb;
     const uint32_t me = mine ? memb[G.memb_off + threadIdx.x] : 0;
-    const uint64_t M = mine ? cmask[me / k] : 0;
+    uint64_t M[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) M[w] = mine ? cmask[(uint64_t)(me / k) * W + w] : 0;
     uint64_t sum = 0;
     const bool quads = g.gx >= 2 && g.gy >= 2;
     for (uint64_t base = r0; base < r1; base += kBlock) {
         const uint64_t h = base + threadIdx.x;
-        uint64_t q[4] = {0, 0, 0, 0};
+        uint64_t q[W][4];
+#pragma unroll
+        for (int w = 0; w < W; w++)
+#pragma unroll
+            for (int dq = 0; dq < 4; dq++) q[w][dq] = 0;
         uint32_t n = 0;
         if (h < r1 && quads) {
             const uint32_t c = run_cell[h];
@@ -551,19 +564,28 @@ __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, co
                 if (cy < dqy || cx < dqx) continue;
                 const uint32_t qy = cy - dqy, qx = cx - dqx;
                 if (qy + 1 >= g.gy || qx + 1 >= g.gx) continue;
-                q[dq] = qmask[(uint64_t)qy * (g.gx - 1) + qx];
+#pragma unroll
+                for (int w = 0; w < W; w++) q[w][dq] = qmask[((uint64_t)qy * (g.gx - 1) + qx) * W + w];
             }
             n = dirf[c + 1] - dirf[c];
         }
 #pragma unroll
-        for (int dq = 0; dq < 4; dq++) s_q[dq][threadIdx.x] = q[dq];
+        for (int w = 0; w < W; w++)
+#pragma unroll
+            for (int dq = 0; dq < 4; dq++) s_q[w][dq][threadIdx.x] = q[w][dq];
         s_n[threadIdx.x] = n;
         __syncthreads();
         if (mine) {
             const uint32_t cnt = (uint32_t)(r1 - base < kBlock ? r1 - base : kBlock);
             for (uint32_t j = 0; j < cnt; j++) {
-                const bool ok = ((s_q[0][j] & M) == M) | ((s_q[1][j] & M) == M) | ((s_q[2][j] & M) == M) |
-                                ((s_q[3][j] & M) == M);
+                bool ok = false;
+#pragma unroll
+                for (int dq = 0; dq < 4; dq++) {
+                    bool sup = true;
+#pragma unroll
+                    for (int w = 0; w < W; w++) sup &= (s_q[w][dq][j] & M[w]) == M[w];
+                    ok |= sup;
+                }
                 sum += ok ? s_n[j] : 0u;
             }
         }
@@ -595,18 +617,21 @@ int colo::filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_
                        uint32_t *d_counts, uint8_t *d_pass, hipStream_t s) {
     colo_ctx *ctx_for_err = c;
     if (C == 0) return COLO_OK;
-    if (c->F > 64) return set_error(c, COLO_EUNSUPPORTED, "filter supports at most 64 features");
+    const uint32_t W = mask_words(c->F);
+    if (W > kMaxMaskWords)
+        return set_error(c, COLO_EUNSUPPORTED, "the cell-aggregate filter supports at most 256 features (mine with the "
+                                               "filter off for more)");
     std::vector<uint32_t> hc((uint64_t)C * k);
     COLO_HIP(hipMemcpyAsync(hc.data(), d_cands, hc.size() * 4, hipMemcpyDeviceToHost, s));
     COLO_HIP(hipStreamSynchronize(s));
     // members (P, t) grouped by feature; candidate masks
-    std::vector<uint64_t> cmask(C, 0);
+    std::vector<uint64_t> cmask((uint64_t)C * W, 0);
     std::vector<std::vector<uint32_t>> by_f(c->F);
     for (uint32_t i = 0; i < C; i++)
         for (uint32_t t = 0; t < k; t++) {
             uint32_t f = hc[(uint64_t)i * k + t];
             if (f >= c->F) return set_error(c, COLO_EARG, "candidate feature out of range");
-            cmask[i] |= 1ull << f;
+            cmask[(uint64_t)i * W + (f >> 6)] |= 1ull << (f & 63);
             by_f[f].push_back(i * k + t);
         }
     std::vector<uint32_t> memb;
@@ -647,7 +672,8 @@ int colo::filter_batch(colo_ctx *c, const uint32_t *d_cands, uint32_t C, uint32_
         COLO_HIP(hipMemcpyAsync(d_cmask, cmask.data(), cmask.size() * 8, hipMemcpyHostToDevice, s));
         COLO_HIP(hipMemcpyAsync(d_memb, memb.data(), memb.size() * 4, hipMemcpyHostToDevice, s));
         DevGrid g = c->dev().g;
-        hipLaunchKernelGGL(k_filter, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_groups, d_prefix,
+        hipLaunchKernelGGL(W == 1 ? k_filter<1> : W == 2 ? k_filter<2> : W == 3 ? k_filter<3> : k_filter<4>,
+                           dim3((unsigned)chunks), dim3(kBlock), 0, s, d_groups, d_prefix,
                            (uint64_t)groups.size(), d_memb, d_cmask, k, c->d_run_cell, c->d_dir, c->d_qmask, g,
                            d_counts);
         COLO_HIP(hipGetLastError());
@@ -2668,15 +2694,16 @@ static int bin_launch(colo_ctx *c, hipStream_t s) {
     if (c->F && n)
         hipLaunchKernelGGL(k_bin_tail, dim3((c->F + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->d_flag, run_idx,
                            c->d_fstart, c->d_fcount, c->F, c->d_spos, small);
-    // cell and quadrant masks (F <= 64)
-    if (c->F <= 64) {
+    // cell and quadrant masks (W 64-bit words each)
+    const uint32_t W = mask_words(c->F);
+    if (W <= kMaxMaskWords) {
         uint64_t nq = (c->gx >= 2 && c->gy >= 2) ? (uint64_t)(c->gx - 1) * (c->gy - 1) : 0;
-        if ((rc = fill_u32((uint32_t *)c->d_cellmask, 2ull * c->cells, 0, s))) return rc;
+        if ((rc = fill_u32((uint32_t *)c->d_cellmask, 2ull * c->cells * W, 0, s))) return rc;
         if (n)
             hipLaunchKernelGGL(k_cellmask, dim3(grid_for(n, 1u << 30)), dim3(kBlock), 0, s, c->d_run_cell, small + 2,
-                               c->F, small + 1, (unsigned long long *)c->d_cellmask);
-        if (nq) hipLaunchKernelGGL(k_qmask, dim3(grid_for(nq, 1u << 30)), dim3(kBlock), 0, s, c->d_cellmask, c->gx,
-                                   c->gy, c->d_qmask);
+                               c->F, small + 1, W, (unsigned long long *)c->d_cellmask);
+        if (nq) hipLaunchKernelGGL(k_qmask, dim3(grid_for(nq * W, 1u << 30)), dim3(kBlock), 0, s, c->d_cellmask, c->gx,
+                                   c->gy, W, c->d_qmask);
     }
     COLO_HIP(hipMemcpyAsync(c->h_bin, small, 8 * (3 + 3ull * c->F), hipMemcpyDeviceToHost, s));
     COLO_HIP(hipGetLastError());
@@ -2764,10 +2791,11 @@ static int set_grid(colo_ctx *c, float r) {
     c->dir_len = (uint64_t)c->F * (c->cells + 1ull);
     int rc = alloc_dev(c, (void **)&c->d_dir, 4 * c->dir_len);
     if (!rc) rc = alloc_dev(c, (void **)&c->d_run_cell, 4 * ((uint64_t)c->n + 1));  // runs <= n
-    if (!rc && c->F <= 64) {
+    if (!rc && mask_words(c->F) <= kMaxMaskWords) {
+        const uint64_t W = mask_words(c->F);
         uint64_t nq = (c->gx >= 2 && c->gy >= 2) ? (uint64_t)(c->gx - 1) * (c->gy - 1) : 0;
-        rc = alloc_dev(c, (void **)&c->d_cellmask, 8ull * c->cells);
-        if (!rc) rc = alloc_dev(c, (void **)&c->d_qmask, 8 * (nq ? nq : 1));
+        rc = alloc_dev(c, (void **)&c->d_cellmask, 8ull * c->cells * W);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_qmask, 8 * (nq ? nq : 1) * W);
     }
     return rc;
 }

@@ -17,6 +17,8 @@ namespace colo {
 constexpr int kMaxK = 16;          // largest pattern size the kernels accept
 constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
 constexpr int kMaxRun = 64;        // candidates sharing a parent table in one work item
+constexpr uint32_t kMaxMaskWords = 4;  // feature masks of the filter: 64-bit words, so at most 256 features
+constexpr uint32_t mask_words(uint32_t F) { return F ? (F + 63) / 64 : 1; }
 
 // Grid geometry as the kernels read it (gridBox, Codes/DataTypes.h:77-83).
 struct DevGrid {

@@ -905,15 +905,14 @@ struct Miner {
     // Slots arena budget of a level's count pass in u32 words: 1/16 of the
     // device's memory, or the tuned limit (KiB) when one is set.
     bool force_slots_ = false;
-    // Per-row slots kept by a count pass: a third of the pool's headroom,
-    // at least 1/16 of device memory.  (At 50M instances the level-3 slots
-    // would take 70 GB; kept, they spare the recount of the tables built
-    // from them: c5 sweep 763 -> 742 ms with all kept.)
+    // Per-row slots kept by a count pass: 1/16 of device memory.  A larger
+    // budget spares recounts (c5 sweep 763 -> 742 ms with all 70 GB of
+    // level-3 slots kept) but leaves the pool too little room to keep its
+    // blocks between runs: a budget sized from the pool's headroom made
+    // repeated runs free and reallocate arenas (c5 persistent 0.76 -> 6.1 s).
     uint64_t slots_budget_words() {
         if (c->tune.slots_budget_kb) return (uint64_t)c->tune.slots_budget_kb << 8;
-        const size_t cap = c->pool.cap_bytes(), used = c->pool.in_use;
-        const size_t head = cap > used ? (cap - used) / 3 : 0;
-        return (uint64_t)(std::max(c->pool.total_bytes() / 16, head) / 4);
+        return (uint64_t)(c->pool.total_bytes() / 16 / 4);
     }
     bool ensure_buf(DevBuf &b, size_t bytes) {
         if (b.ensure(bytes)) return true;

@@ -45,7 +45,7 @@ extern "C" {
 #define COLO_ENOMEM 4      /* device or host allocation failed            */
 #define COLO_EHIP 5        /* HIP runtime error                           */
 #define COLO_EARG 6        /* invalid argument                            */
-#define COLO_EUNSUPPORTED 7/* outside the supported envelope (e.g. F>64)  */
+#define COLO_EUNSUPPORTED 7/* outside the supported envelope (e.g. F>256) */
 
 typedef struct colo_ctx colo_ctx;
 
@@ -289,8 +289,7 @@ typedef struct colo_tune {
     uint32_t count_var_k[16];
     /* item rows of level k (index k, 2..15); COLO_TUNE_UNSET = item_rows */
     uint32_t item_rows_k[16];
-    /* largest slots arena of a level's count pass in KiB; 0 = a third of
-     * the memory pool's headroom at the level, at least 1/16 of the
+    /* largest slots arena of a level's count pass in KiB; 0 = 1/16 of the
      * device's memory.  Candidates above it are counted without per-row
      * slots and recounted when their table is built (results unchanged). */
     uint32_t slots_budget_kb;

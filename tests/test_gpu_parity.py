@@ -493,3 +493,32 @@ def test_filter_threshold_on_a_bound(fx, request):
         oc2, op2 = orc.filter(cands, 2, float(th))
         np.testing.assert_array_equal(gc, oc2)
         np.testing.assert_array_equal(gp, op2)
+
+
+@pytest.mark.parametrize("F", [70, 130, 200])
+def test_many_features(datagen, colo_mod, oracle_mod, F):
+    """More than 64 features: the filter's feature masks span 2-4 64-bit
+    words.  Filter counts and pass flags of pairs and triples (features on
+    both sides of every 64-feature boundary among them), and the complete
+    report of a mining run whose planted patterns use features above 64,
+    equal the oracle's."""
+    _say(f"many features F={F}")
+    x, y, fc, b, p = _workload(datagen, "c1", N=40 * F, seed=F, F=F, planted=[3, 4], plant_rate=0.8,
+                               plant_band=(F - 8, F), r=60.0, theta=0.3, max_k=4)
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    rng = np.random.default_rng(F)
+    for k in (2, 3):
+        cands = np.sort(np.array([rng.choice(F, k, replace=False) for _ in range(300)]), axis=1).astype(np.uint32)
+        cands[:4, -1] = [63, 64, F - 1, 127 if F > 127 else F - 2]  # word boundaries
+        cands = np.unique(np.sort(cands, axis=1), axis=0)
+        cands = cands[np.all(np.diff(cands, axis=1) > 0, axis=1)]
+        gc, gp = ctx.filter(cands, k, p["theta"])
+        oc, op = orc.filter(cands, k, p["theta"])
+        np.testing.assert_array_equal(gc, oc)
+        np.testing.assert_array_equal(gp, op)
+    rep, _ = ctx.mine(p["theta"], max_k=p["max_k"], sched=colo_mod.COLO_SCHED_PERSISTENT)
+    assert rep == orc.mine(p["r"], p["theta"], max_k=p["max_k"])
+    lv3 = [ln for ln in rep.split("\n") if ln.startswith("level 3 ")]
+    assert lv3 and not lv3[0].endswith(" prev 0"), lv3  # a prevalent triple of features above 64

@@ -195,6 +195,37 @@ def cpu_full(colo, model, threads, name):
     return out
 
 
+def e2e_child(args):
+    """One-shot mining in a fresh process, from host arrays to pattern files:
+    context creation (upload and binning), launch shapes from the
+    autotuner's cache, one mining run at the lowest threshold."""
+    import tempfile
+    import colo
+    import torch
+    x, y, fc, bounds, p = gen(args.workload, N=args.N)
+    th = min(p["thetas"]) if "thetas" in p else p["theta"]
+    model = None if args.no_model or not os.path.exists(args.model) else colo.CostModel.load(args.model)
+    sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
+    torch.cuda.init()
+    with tempfile.TemporaryDirectory() as td:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx = colo.Context(x, y, fc, bounds, p["r"])
+        ctx.autotune(th, max_k=p["max_k"], sched=sched, model=model, cache=args.tune_cache)
+        _, st = ctx.mine(th, max_k=p["max_k"], sched=sched, hash_tables=False, out_prefix=os.path.join(td, ""),
+                         model=model)
+        torch.cuda.synchronize()
+        e2e = time.perf_counter() - t0
+        files = sorted(f for f in os.listdir(td) if f.startswith("ColocationDegree_"))
+    print(json.dumps({"s": e2e, "theta": th, "mining_ms": st["total_ms"], "alloc_ms": st["alloc_ms"],
+                      "alloc_gb": st["alloc_bytes"] / 1e9,
+                      "level_ms": {str(k): round(v, 2) for k, v in enumerate(st["level_ms"]) if v > 0},
+                      "pattern_files": files, "upload_and_bin_included": True, "csv_parse_included": False,
+                      "launch_shapes": "autotuner cache (" + os.path.basename(args.tune_cache) + ")"}), flush=True)
+    ctx.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +240,7 @@ def main():
     ap.add_argument("--no-model", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "colo_tune.txt"))
+    ap.add_argument("--e2e-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-full", default=None, metavar="WORKLOAD",
                     help="only time complete mining of WORKLOAD by the oracle on the host cores beside the GPU")
     ap.add_argument("--copy-probe", action="store_true",
@@ -223,6 +255,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
         return 0
+    if args.e2e_child:
+        return e2e_child(args)
     if args.cpu_full:
         import colo
         model = None if args.no_model or not os.path.exists(args.model) else colo.CostModel.load(args.model)
@@ -317,6 +351,7 @@ def main():
     out["level_ms"] = {f"{t:g}": {str(k): round(v, 2) for k, v in enumerate(s["level_ms"]) if v > 0}
                        for t, s in zip(thetas, first)}
     out["memory"] = {"pool_peak_gb": max(s["pool_peak_bytes"] for s in stats) / 1e9,
+                     "alloc_ms_per_step": sum(s["alloc_ms"] for s in stats) / args.steps,
                      "slots_deferred": sum(s["slots_deferred"] for s in first),
                      "recounts": sum(s["recounts"] for s in first)}
 
@@ -388,6 +423,33 @@ def main():
         log(f"schedules identical (with table checksums): {identical}")
         if not identical:
             log("ERROR: schedules produced different reports")
+    if not args.no_plain:
+        # level-2 join of every feature pair with the filter off (the
+        # filter-independent neighbour-pair rate; sizes up to 3 so that the
+        # level-2 tables the next level extends are written, as in a step)
+        _, st0 = ctx.mine(min(thetas), max_k=3, filter_on=False, sched=main_sched, hash_tables=False,
+                          model=model if main_sched == colo.COLO_SCHED_LEARNED else None)
+        out["level2_filter_off"] = {"pairs": st0["l2_pairs"], "join_ms": st0["l2_join_ms"],
+                                    "pairs_per_s": st0["l2_pairs"] / (st0["l2_join_ms"] / 1000.0)
+                                    if st0["l2_join_ms"] else None, "candidates": st0["candidates"]}
+        log(f"level 2 with the filter off: {st0['l2_pairs']} pairs in {st0['l2_join_ms']:.1f} ms")
+        ctx.close()
+        # end to end from host-resident arrays, as a one-shot run: a new
+        # process creates the context (upload and binning), takes the launch
+        # shapes from the autotuner's cache and mines once at the lowest
+        # threshold, writing the ColocationDegree_<k>.txt files
+        import subprocess
+        torch.cuda.empty_cache()
+        cmd = [sys.executable, os.path.abspath(__file__), "--e2e-child", "--workload", args.workload,
+               "--tune-cache", args.tune_cache, "--model", args.model] + (["--no-model"] if model is None else []) + \
+              (["--N", str(args.N)] if args.N else [])
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if r.returncode == 0 and r.stdout.strip():
+            out["end_to_end"] = json.loads(r.stdout.strip().splitlines()[-1])
+            log(f"end to end at theta {out['end_to_end']['theta']:g}: {out['end_to_end']['s']:.3f} s "
+                f"({len(out['end_to_end']['pattern_files'])} pattern files)")
+        else:
+            log("end-to-end run failed: " + r.stderr[-500:])
     if not args.no_cpu:
         threads = cpu_threads()
         log(f"cpu baseline on {threads} threads")

@@ -487,16 +487,30 @@ struct Miner {
             bits_tot += words;
         }
         // The slots arena holds one u32 per parent row and candidate, but
-        // only the tables built afterwards read slots.  Above the budget the
-        // largest candidates keep none (their rows and bits are still
-        // counted); a table built for one of them is recounted with slots
-        // (recount_missing_slots).
+        // only the tables built afterwards read slots.  Above the budget some
+        // candidates keep none (their rows and bits are still counted); a
+        // table built for one of them is recounted with slots
+        // (recount_missing_slots).  Slots go first to the candidates most
+        // likely to be prevalent -- the highest filter bound -- and among
+        // equal bounds to the smallest.
         const uint64_t budget = force_slots_ ? ~0ull : slots_budget_words();
         if (slots_tot > budget) {
+            std::vector<float> ub(m, 0.f);
+            if (level_fcnt_ && level_fcnt_->size() >= C.size())
+                for (uint64_t u = 0; u < m; u++) {
+                    float b = 1.f;
+                    for (uint32_t j = 0; j < k; j++) {
+                        const float v = (float)((double)(*level_fcnt_)[idx[u] * k + j] /
+                                                (double)(float)c->fcount[C[idx[u] * k + j]]);
+                        b = j == 0 || v < b ? v : b;
+                    }
+                    ub[u] = b;
+                }
             std::vector<uint64_t> ord(m);
             for (uint64_t u = 0; u < m; u++) ord[u] = u;
-            std::stable_sort(ord.begin(), ord.end(),
-                             [&](uint64_t a, uint64_t b) { return desc[a].n_anchor < desc[b].n_anchor; });
+            std::stable_sort(ord.begin(), ord.end(), [&](uint64_t a, uint64_t b) {
+                return ub[a] != ub[b] ? ub[a] > ub[b] : desc[a].n_anchor < desc[b].n_anchor;
+            });
             std::vector<uint8_t> keep(m, 0);
             uint64_t kept = 0;
             for (uint64_t u : ord)
@@ -910,6 +924,7 @@ struct Miner {
     // level-3 slots kept) but leaves the pool too little room to keep its
     // blocks between runs: a budget sized from the pool's headroom made
     // repeated runs free and reallocate arenas (c5 persistent 0.76 -> 6.1 s).
+    const std::vector<uint32_t> *level_fcnt_ = nullptr;  // the level's filter counts (C x k), while it runs
     uint64_t slots_budget_words() {
         if (c->tune.slots_budget_kb) return (uint64_t)c->tune.slots_budget_kb << 8;
         return (uint64_t)(c->pool.total_bytes() / 16 / 4);
@@ -1151,9 +1166,11 @@ struct Miner {
                 if (rc0) return set_error(c, rc0, "cost model prediction");
             }
             st.model_ms += ms_since(tm);
+            level_fcnt_ = o.filter_on ? &fcnt : nullptr;
             int rc = o.sched == COLO_SCHED_PLAIN
                          ? run_level_plain(k, C, todo, Lprev, res, L)
                          : run_level_persistent(k, C, todo, Lprev, res, L, arena, learned ? &pred : nullptr);
+            level_fcnt_ = nullptr;
             if (rc) return rc;
             if (memo) {
                 for (uint64_t i = 0; i < nC; i++)

@@ -44,7 +44,7 @@ class Grid(C.Structure):
 class MineOpts(C.Structure):
     _fields_ = [("theta", C.c_float), ("max_k", C.c_uint32), ("pi_mode", C.c_int32), ("filter_on", C.c_int32),
                 ("sched", C.c_int32), ("hash_tables", C.c_int32), ("out_prefix", C.c_char_p),
-                ("model", C.c_void_p), ("log_path", C.c_char_p)]
+                ("model", C.c_void_p), ("log_path", C.c_char_p), ("subset_bound", C.c_int32)]
 
 
 TUNE_UNSET = 0xFFFFFFFF
@@ -73,7 +73,8 @@ class MineStats(C.Structure):
                [(n, C.c_uint64) for n in ("overflows", "arenas")] + \
                [(n, C.c_double) for n in ("model_ms", "alloc_ms", "alloc_bytes")] + [("level_ms", C.c_double * 16)] + \
                [("level_count_kernel_ms", C.c_double * 16), ("pool_peak_bytes", C.c_double),
-                ("slots_deferred", C.c_uint64), ("recounts", C.c_uint64)]
+                ("slots_deferred", C.c_uint64), ("recounts", C.c_uint64), ("pruned", C.c_uint64),
+                ("bound_ms", C.c_double)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -472,10 +473,8 @@ class Context:
 
     @_on_stream
     def mine(self, theta, max_k=0, pi_mode=0, filter_on=True, sched=COLO_SCHED_PERSISTENT, hash_tables=True,
-             out_prefix=None, model=None, log_path=None):
-        o = MineOpts(C.c_float(theta), max_k, pi_mode, int(filter_on), sched, int(hash_tables),
-                     out_prefix.encode() if out_prefix else None, model.h if model is not None else None,
-                     log_path.encode() if log_path else None)
+             out_prefix=None, model=None, log_path=None, subset_bound=False):
+        o = _opts(theta, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, log_path, subset_bound)
         rep = C.c_void_p()
         st = MineStats()
         rc = lib().colo_mine(self.h, C.byref(o), C.byref(rep), C.byref(st))
@@ -488,10 +487,10 @@ class Context:
 
     @_on_stream
     def mine_sweep(self, thetas, max_k=0, pi_mode=0, filter_on=True, sched=COLO_SCHED_PERSISTENT, hash_tables=True,
-                   out_prefix=None, model=None):
+                   out_prefix=None, model=None, subset_bound=False):
         """Mines every theta in one call, reusing theta-independent results
         (colo_mine_sweep); returns [(report, stats)] in the given order."""
-        o = _opts(0.0, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, None)
+        o = _opts(0.0, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, None, subset_bound)
         n = len(thetas)
         th = np.ascontiguousarray(thetas, np.float32)
         reps = (C.c_void_p * n)()
@@ -523,10 +522,10 @@ OUTPUT_NAMES = ["log1p rows", "log1p refinement tests"]
 RECORD_FLOATS = MLP_IN + MLP_OUT + 1  # features, log1p(rows), log1p(tests), k
 
 
-def _opts(theta, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, log_path):
+def _opts(theta, max_k, pi_mode, filter_on, sched, hash_tables, out_prefix, model, log_path, subset_bound=False):
     return MineOpts(C.c_float(theta), max_k, pi_mode, int(filter_on), sched, int(hash_tables),
                     out_prefix.encode() if out_prefix else None, model.h if model is not None else None,
-                    log_path.encode() if log_path else None)
+                    log_path.encode() if log_path else None, int(subset_bound))
 
 
 def read_log(path, with_k=False):

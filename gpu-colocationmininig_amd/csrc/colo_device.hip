@@ -2527,6 +2527,141 @@ __global__ __launch_bounds__(kBlock) void k_pi_abs(const uint32_t *bits, const S
     }
 }
 
+// ------------------------------------------------------------ subset bound
+// Workgroups take fixed chunks of kBoundWords words of one task (a bound
+// task or a copy range), found by binary search over the tasks' first
+// chunk numbers: a hot feature's windows (hundreds of thousands of words)
+// spread over many workgroups instead of one.
+
+constexpr uint64_t kBoundWords = 4096;
+
+__device__ inline uint64_t chunk_owner(const uint64_t *chunk_prefix, uint64_t n, uint64_t b) {
+    uint64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (chunk_prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_subset_bound(const colo::BoundTask *tasks, const uint64_t *chunk_prefix,
+                                                         uint64_t n, unsigned long long *out) {
+    const uint64_t b = blockIdx.x;
+    const uint64_t t = chunk_owner(chunk_prefix, n, b);
+    const colo::BoundTask *tp = tasks + t;
+    const uint32_t nw = tp->nw, words = tp->words, last = tp->last_bit;
+    const uint64_t w0 = (b - chunk_prefix[t]) * kBoundWords;
+    const uint64_t w1 = w0 + kBoundWords < words ? w0 + kBoundWords : words;
+    uint64_t sum = 0;
+    for (uint64_t i = w0 + threadIdx.x; i < w1; i += kBlock) {
+        uint32_t a = tp->w[0][i];
+        for (uint32_t j = 1; j < nw; j++) a &= tp->w[j][i];
+        sum += __popc(a);
+        if (i == (last >> 5)) out[2 * t + 1] = (a >> (last & 31)) & 1u;
+    }
+    uint64_t ex;
+    const uint64_t tot = block_excl_scan(sum, ex);
+    if (threadIdx.x == 0 && tot) atomicAdd(&out[2 * t], (unsigned long long)tot);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_ranges(const uint32_t *src, uint32_t *dst,
+                                                          const colo::CopyRange *ranges, const uint64_t *chunk_prefix,
+                                                          uint64_t n) {
+    const uint64_t b = blockIdx.x;
+    const uint64_t t = chunk_owner(chunk_prefix, n, b);
+    const colo::CopyRange r = ranges[t];
+    const uint64_t w0 = (b - chunk_prefix[t]) * kBoundWords;
+    const uint64_t w1 = w0 + kBoundWords < r.words ? w0 + kBoundWords : r.words;
+    for (uint64_t i = w0 + threadIdx.x; i < w1; i += kBlock) dst[r.dst + i] = src[r.src + i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_rank_to_sorted(const uint32_t *src, uint32_t N, const uint32_t *spos_f,
+                                                           uint32_t base, uint32_t *dst) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    if ((src[i >> 5] >> (i & 31)) & 1u) {
+        const uint32_t r = spos_f[i] - base;
+        atomicOr(&dst[r >> 5], 1u << (r & 31));
+    }
+}
+
+// Chunk numbers of n tasks of the given word counts (prefix[i] = first
+// chunk of task i); returns the total.
+template <typename T, typename W>
+static uint64_t chunk_prefix_of(const T *v, uint64_t n, W words_of, std::vector<uint64_t> &prefix) {
+    prefix.resize(n);
+    uint64_t chunks = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        prefix[i] = chunks;
+        chunks += (words_of(v[i]) + kBoundWords - 1) / kBoundWords;
+    }
+    return chunks;
+}
+
+int colo::subset_bound_batch(colo_ctx *c, const BoundTask *tasks, uint64_t n, uint64_t *out, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (!n) return COLO_OK;
+    for (uint64_t i = 0; i < n; i++)
+        if (tasks[i].nw < 1 || tasks[i].nw > kMaxK - 1 || (tasks[i].words && tasks[i].last_bit >= 32ull * tasks[i].words))
+            return set_error(c, COLO_EARG, "subset bound: malformed task");
+    std::vector<uint64_t> prefix;
+    const uint64_t chunks = chunk_prefix_of(tasks, n, [](const BoundTask &t) { return (uint64_t)t.words; }, prefix);
+    if (chunks >= (1ull << 31)) return set_error(c, COLO_EUNSUPPORTED, "subset bound: too many chunks");
+    const size_t tb = ((size_t)n * sizeof(BoundTask) + 15) & ~(size_t)15;
+    void *tmp;
+    int rc = scratch(c, tb + (size_t)n * 24, &tmp);
+    if (rc) return rc;
+    BoundTask *d_tasks = (BoundTask *)tmp;
+    uint64_t *d_prefix = (uint64_t *)((char *)tmp + tb);
+    uint64_t *d_out = d_prefix + n;
+    if ((rc = fill_u32((uint32_t *)d_out, 4ull * n, 0, s))) return rc;
+    COLO_HIP(hipMemcpyAsync(d_tasks, tasks, (size_t)n * sizeof(BoundTask), hipMemcpyHostToDevice, s));
+    COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+    if (chunks) {
+        hipLaunchKernelGGL(k_subset_bound, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_tasks, d_prefix, n,
+                           (unsigned long long *)d_out);
+        COLO_HIP(hipGetLastError());
+    }
+    COLO_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 16, hipMemcpyDeviceToHost, s));
+    COLO_HIP(hipStreamSynchronize(s));
+    return COLO_OK;
+}
+
+int colo::gather_ranges(colo_ctx *c, const uint32_t *d_src, uint32_t *d_dst, const CopyRange *ranges, uint64_t n,
+                        hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (!n) return COLO_OK;
+    std::vector<uint64_t> prefix;
+    const uint64_t chunks = chunk_prefix_of(ranges, n, [](const CopyRange &r) { return r.words; }, prefix);
+    if (chunks >= (1ull << 31)) return set_error(c, COLO_EUNSUPPORTED, "gather: too many chunks");
+    const size_t rb = ((size_t)n * sizeof(CopyRange) + 15) & ~(size_t)15;
+    void *tmp;
+    int rc = scratch(c, rb + (size_t)n * 8, &tmp);
+    if (rc) return rc;
+    CopyRange *d_ranges = (CopyRange *)tmp;
+    uint64_t *d_prefix = (uint64_t *)((char *)tmp + rb);
+    COLO_HIP(hipMemcpyAsync(d_ranges, ranges, (size_t)n * sizeof(CopyRange), hipMemcpyHostToDevice, s));
+    COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+    if (chunks) {
+        hipLaunchKernelGGL(k_gather_ranges, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_src, d_dst, d_ranges,
+                           d_prefix, n);
+        COLO_HIP(hipGetLastError());
+    }
+    COLO_HIP(hipStreamSynchronize(s));
+    return COLO_OK;
+}
+
+int colo::rank_to_sorted(colo_ctx *c, const uint32_t *d_src, uint32_t f, uint32_t *d_dst, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    if (f >= c->F) return set_error(c, COLO_EARG, "rank_to_sorted: feature out of range");
+    const uint32_t N = c->fcount[f];
+    if (!N) return COLO_OK;
+    hipLaunchKernelGGL(k_rank_to_sorted, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_src, N,
+                       c->d_spos + c->fstart[f], c->fstart[f], d_dst);
+    COLO_HIP(hipGetLastError());
+    return COLO_OK;
+}
+
 void colo::seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, uint64_t *seg_off,
                       uint64_t *words_total) {
     uint64_t o = bits_off;

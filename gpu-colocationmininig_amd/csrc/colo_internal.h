@@ -331,6 +331,29 @@ struct RefineItem {
 // instance, i.e. the last-feature instances its count pass tests.
 int refine_tests_batch(colo_ctx *c, const RefineItem *items, uint32_t n, uint64_t *out, hipStream_t s);
 int mlp_predict_host(colo_mlp *m, const float *X, uint32_t n, float *y);
+
+// Subset bound of the miner: for one (candidate, member) the AND of the
+// member's participation windows in the candidate's (k-1)-subsets (nw
+// windows of words u32 each, bit = sorted rank inside the feature), its
+// popcount and its bit last_bit (the sorted rank of input rank N-1).
+struct BoundTask {
+    const uint32_t *w[kMaxK - 1];
+    uint32_t nw, words, last_bit, pad;
+};
+// n tasks (host in/out): out[2i] = popcount, out[2i+1] = last bit.  Waits
+// for the stream before returning.
+int subset_bound_batch(colo_ctx *c, const BoundTask *tasks, uint64_t n, uint64_t *out, hipStream_t s);
+// Word ranges copied in one launch (ranges host-side): d_dst[r.dst + i] =
+// d_src[r.src + i], i < r.words.  Waits for the stream before returning.
+struct CopyRange {
+    uint64_t src, dst, words;
+};
+int gather_ranges(colo_ctx *c, const uint32_t *d_src, uint32_t *d_dst, const CopyRange *ranges, uint64_t n,
+                  hipStream_t s);
+// Participation bits of feature f from input-rank to sorted-rank order:
+// d_dst (ceil(N/32) words, zeroed by the caller) gets the bit of the sorted
+// rank of every input rank set in d_src.
+int rank_to_sorted(colo_ctx *c, const uint32_t *d_src, uint32_t f, uint32_t *d_dst, hipStream_t s);
 }  // namespace colo
 
 #define COLO_HIP(call)                                                                  \

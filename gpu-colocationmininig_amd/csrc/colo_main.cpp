@@ -2,8 +2,11 @@
 // (Codes/main.cpp:8-48 + colocationFinder::Begin, cpp:1068-1147):
 //   colo_miner <config.txt> [--sched plain|persistent|learned] [--model F]
 //              [--max-k K] [--pi-mode compat|exact] [--no-filter]
+//              [--no-subset-bound]
 // --sched learned uses the cost model F (default: models/cost_mlp.bin
-// beside the executable).
+// beside the executable).  The subset bound (on by default) skips
+// candidates that it proves non-prevalent; the pattern files are the same
+// either way.
 // Reads the reference's config.txt / parameter.txt / CSV, writes
 // <location>ColocationDegree_<k>.txt and prints the timing line.
 #include <cstdio>
@@ -20,6 +23,7 @@ int main(int argc, char **argv) {
     }
     colo_mine_opts o{};
     o.filter_on = 1;
+    o.subset_bound = 1;
     o.sched = COLO_SCHED_PERSISTENT;
     std::string model_path;
     {
@@ -41,6 +45,8 @@ int main(int argc, char **argv) {
             o.pi_mode = !std::strcmp(argv[++i], "exact") ? 1 : 0;
         } else if (!std::strcmp(argv[i], "--no-filter")) {
             o.filter_on = 0;
+        } else if (!std::strcmp(argv[i], "--no-subset-bound")) {
+            o.subset_bound = 0;
         } else {
             std::fprintf(stderr, "unknown option %s\n", argv[i]);
             return 2;

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <string>
 #include <thread>
@@ -123,6 +124,11 @@ struct Prev {
     uint32_t *nptr = nullptr;       // level 2: second column as sorted positions
     uint64_t offs_at = 0;           // index of the first offset inside offs
     uint64_t off_base = 0;          // value of the first offset
+    // exact participation (subset bound): the members' windows of one bit
+    // per instance in sorted-rank order, back to back (seg_layout), from
+    // word part_off of part
+    std::shared_ptr<DevBuf> part;
+    uint64_t part_off = 0;
 };
 
 // Results that do not depend on theta, kept across the runs of a threshold
@@ -138,6 +144,12 @@ struct SweepMemo {
     std::map<std::vector<uint32_t>, Res> res;
     std::map<std::pair<uint32_t, uint32_t>, Prev> l2;
     std::map<std::vector<uint32_t>, std::vector<uint32_t>> fcnt;  // filter counts (theta-independent)
+    // subset bounds and the participation of every prevalent pattern so far
+    // (both theta-independent): a candidate first met at a higher theta
+    // finds its subsets' participation here when they were remembered
+    // instead of counted
+    std::map<std::vector<uint32_t>, float> q;
+    std::map<std::vector<uint32_t>, std::pair<std::shared_ptr<DevBuf>, uint64_t>> part;
     uint64_t hits = 0, misses = 0;
 };
 
@@ -337,6 +349,25 @@ struct Miner {
                 p.nslots = nslots;
                 p.rows = r.rows;
                 p.parent = parent;
+                if (keep_parts_ && !force_slots_) {
+                    // the count pass indexes members 0..k-2 by input rank
+                    // and the last by sorted rank; the kept copy is all
+                    // sorted rank
+                    uint64_t so[kMaxK], words;
+                    seg_layout(c, k, cand, 0, so, &words);
+                    p.part = std::make_shared<DevBuf>(&c->pool);
+                    if (!p.part->ensure(words * 4)) return oom("participation", words * 4);
+                    if ((rc = fill_u32(p.part->as<uint32_t>(), words, 0, s))) return rc;
+                    for (uint32_t j = 0; j < k && !rc; j++) {
+                        if (j + 1 < k)
+                            rc = rank_to_sorted(c, bits.as<uint32_t>() + so[j], cand[j], p.part->as<uint32_t>() + so[j], s);
+                        else
+                            COLO_HIP(hipMemcpyAsync(p.part->as<uint32_t>() + so[j], bits.as<uint32_t>() + so[j],
+                                                    (c->fcount[cand[j]] + 31) / 32 * 4, hipMemcpyDeviceToDevice, s));
+                    }
+                    if (rc) return rc;
+                    st.launches += k;
+                }
                 L.push_back(std::move(p));
             }
         }
@@ -459,6 +490,7 @@ struct Miner {
         if (!m) return COLO_OK;
         std::vector<CandDesc> desc(m);
         std::vector<int64_t> parent(m, -1);
+        std::vector<uint64_t> cwords(m);  // bitmap words of each candidate
         uint64_t slots_tot = 0, bits_tot = 0;
         for (uint64_t u = 0; u < m; u++) {
             const uint32_t *cand = &C[idx[u] * k];
@@ -485,6 +517,7 @@ struct Miner {
             seg_layout(c, k, cand, 0, d.seg_off, &words);
             d.bits_off = bits_tot;
             bits_tot += words;
+            cwords[u] = words;
         }
         // The slots arena holds one u32 per parent row and candidate, but
         // only the tables built afterwards read slots.  Above the budget some
@@ -683,7 +716,107 @@ struct Miner {
                 L.push_back(std::move(p));
             }
         }
+        if (keep_parts_ && !force_slots_) {
+            // the prevalent candidates' bitmaps, copied out of the arena
+            // (which the next pass reuses) for the next level's subset bound
+            std::vector<CopyRange> cr;
+            uint64_t tot = 0;
+            for (uint64_t u = 0; u < m; u++)
+                if (res[idx[u]].prevalent) {
+                    cr.push_back(CopyRange{desc[u].bits_off, tot, cwords[u]});
+                    tot += cwords[u];
+                }
+            if (!cr.empty()) {
+                auto store = std::make_shared<DevBuf>(&c->pool);
+                if (!store->ensure(tot * 4)) return oom("participation store", tot * 4);
+                if ((rc = gather_ranges(c, arena_bits.as<uint32_t>(), store->as<uint32_t>(), cr.data(), cr.size(), s)))
+                    return rc;
+                st.launches += 1;
+                const size_t j0 = L.size() - cr.size();
+                for (size_t t = 0; t < cr.size(); t++) {
+                    L[j0 + t].part = store;
+                    L[j0 + t].part_off = cr[t].dst;
+                }
+            }
+        }
+        return COLO_OK;
+    }
 
+    // Subset bounds of the level's candidates that pass the filter
+    // (k >= 3): q[i] = min over members j of fl32(cnt_j + t_j) / fl32(N_j),
+    // cnt_j = |AND over the (k-1)-subsets S containing member j of its
+    // participation in S|, t_j = 0 with exact numerators and, with the
+    // reference's (cu:108-132), the last-instance bit of member j-1's AND
+    // for j >= 1.  A candidate's participation in a member is a subset of
+    // that in every such S, and a reference numerator exceeds the exact one
+    // only by member j-1's last-instance bit, so both PIs are at most q.
+    // Candidates not computed keep NaN.
+    int subset_bounds(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
+                      const std::vector<Prev> &Lprev, std::vector<float> &q) {
+        const uint64_t nC = C.size() / k;
+        q.assign(nC, std::numeric_limits<float>::quiet_NaN());
+        std::vector<BoundTask> tasks;
+        std::vector<uint64_t> owner;
+        uint32_t sub[kMaxK];
+        for (uint64_t i = 0; i < nC; i++) {
+            if (!pass[i]) continue;
+            const uint32_t *cand = &C[i * k];
+            if (memo) {
+                auto it = memo->q.find(std::vector<uint32_t>(cand, cand + k));
+                if (it != memo->q.end()) {
+                    q[i] = it->second;
+                    continue;
+                }
+            }
+            // window of member j inside S_ip = the candidate without member ip
+            const uint32_t *base[kMaxK];
+            uint64_t so[kMaxK][kMaxK];
+            for (uint32_t ip = 0; ip < k; ip++) {
+                uint32_t w = 0;
+                for (uint32_t t = 0; t < k; t++)
+                    if (t != ip) sub[w++] = cand[t];
+                const int64_t pi = find(Lprev, sub, k - 1);
+                const DevBuf *buf = nullptr;
+                uint64_t off = 0;
+                if (pi >= 0 && Lprev[pi].part) {
+                    buf = Lprev[pi].part.get();
+                    off = Lprev[pi].part_off;
+                } else if (memo) {
+                    auto it = memo->part.find(std::vector<uint32_t>(sub, sub + k - 1));
+                    if (it != memo->part.end()) {
+                        buf = it->second.first.get();
+                        off = it->second.second;
+                    }
+                }
+                if (!buf) return set_error(c, COLO_EARG, "internal: subset participation missing");
+                base[ip] = buf->as<uint32_t>();
+                seg_layout(c, k - 1, sub, off, so[ip], nullptr);
+            }
+            for (uint32_t j = 0; j < k; j++) {
+                BoundTask t{};
+                for (uint32_t ip = 0; ip < k; ip++)
+                    if (ip != j) t.w[t.nw++] = base[ip] + so[ip][j < ip ? j : j - 1];
+                t.words = (c->fcount[cand[j]] + 31) / 32;
+                t.last_bit = c->last_sorted[cand[j]];
+                tasks.push_back(t);
+            }
+            owner.push_back(i);
+        }
+        std::vector<uint64_t> out(tasks.size() * 2);
+        int rc = subset_bound_batch(c, tasks.data(), tasks.size(), out.data(), s);
+        if (rc) return rc;
+        if (!tasks.empty()) st.launches += 1;
+        for (size_t g = 0; g < owner.size(); g++) {
+            const uint32_t *cand = &C[owner[g] * k];
+            float qq = 1.0f;
+            for (uint32_t j = 0; j < k; j++) {
+                const uint64_t v = out[2 * (g * k + j)] + (o.pi_mode == 0 && j >= 1 ? out[2 * (g * k + j - 1) + 1] : 0);
+                volatile float b = (float)v / (float)c->fcount[cand[j]];
+                if (b < qq) qq = b;
+            }
+            q[owner[g]] = qq;
+            if (memo) memo->q.emplace(std::vector<uint32_t>(cand, cand + k), qq);
+        }
         return COLO_OK;
     }
 
@@ -925,6 +1058,7 @@ struct Miner {
     // blocks between runs: a budget sized from the pool's headroom made
     // repeated runs free and reallocate arenas (c5 persistent 0.76 -> 6.1 s).
     const std::vector<uint32_t> *level_fcnt_ = nullptr;  // the level's filter counts (C x k), while it runs
+    bool keep_parts_ = false;  // keep the prevalent patterns' participation (a next level's subset bound)
     uint64_t slots_budget_words() {
         if (c->tune.slots_budget_kb) return (uint64_t)c->tune.slots_budget_kb << 8;
         return (uint64_t)(c->pool.total_bytes() / 16 / 4);
@@ -1128,12 +1262,28 @@ struct Miner {
             std::vector<Prev> L;
             std::shared_ptr<DevBuf> arena;
             auto tl = Clock::now();
-            // candidates to count: those passing the filter whose results are
-            // not known from an earlier run of a sweep
-            std::vector<uint8_t> todo = pass;
+            // subset bound (from level 3): candidates whose bound is below
+            // theta cannot be prevalent and are not counted
+            std::vector<float> sbq;
+            std::vector<uint8_t> live = pass;
+            if (o.subset_bound && k >= 3) {
+                auto tb = Clock::now();
+                int rc0 = subset_bounds(k, C, pass, Lprev, sbq);
+                if (rc0) return rc0;
+                for (uint64_t i = 0; i < nC; i++)
+                    if (pass[i] && sbq[i] < o.theta) {
+                        live[i] = 0;
+                        st.pruned++;
+                    }
+                st.bound_ms += ms_since(tb);
+            }
+            keep_parts_ = o.subset_bound && (o.max_k == 0 || k + 1 <= o.max_k);
+            // candidates to count: those passing both tests whose results
+            // are not known from an earlier run of a sweep
+            std::vector<uint8_t> todo = live;
             if (memo)
                 for (uint64_t i = 0; i < nC; i++) {
-                    if (!pass[i]) continue;
+                    if (!live[i]) continue;
                     auto it = memo->res.find(std::vector<uint32_t>(&C[i * k], &C[i * k] + k));
                     if (it == memo->res.end()) {
                         memo->misses++;
@@ -1228,6 +1378,13 @@ struct Miner {
                     rep.printf("\n");
                     continue;
                 }
+                if (!sbq.empty()) {
+                    rep.printf(" q %08x", fbits(sbq[i]));
+                    if (!live[i]) {
+                        rep.printf("\n");
+                        continue;
+                    }
+                }
                 const CandRes &r = res[i];
                 rep.printf(" r %llu n", (unsigned long long)r.rows);
                 for (uint32_t m = 0; m < k; m++) rep.printf(" %llu", (unsigned long long)r.numc[m]);
@@ -1299,6 +1456,9 @@ struct Miner {
                 rep.printf("\n");
             }
             for (Prev &p : L) p.slots.reset();
+            if (memo)
+                for (const Prev &p : L)
+                    if (p.part) memo->part.emplace(p.pat, std::make_pair(p.part, p.part_off));
             if (memo && k == 2)
                 for (const Prev &p : L)
                     if (p.tptr && p.nptr) memo->l2.emplace(std::make_pair(p.pat[0], p.pat[1]), p);
@@ -1307,6 +1467,7 @@ struct Miner {
                 for (size_t i = 0; i < L.size(); i++) {
                     l2_index_[(size_t)L[i].pat[0] * c->F + L[i].pat[1]] = (int64_t)l2_.size();
                     l2_.push_back(L[i]);  // shares the table and offsets
+                    l2_.back().part.reset();
                 }
             } else {
                 for (Prev &p : L) p.offs.reset();
@@ -1392,8 +1553,8 @@ static std::string tune_key(const colo_ctx *c, const colo_mine_opts &o) {
     for (char &ch : arch)
         if (ch == ' ') ch = '_';
     char b[256];
-    snprintf(b, sizeof b, "v4/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d", arch.c_str(), cus, c->n, c->F, fbits(c->r2),
-             fbits(o.theta), o.max_k, o.sched);
+    snprintf(b, sizeof b, "v4/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d/b%d", arch.c_str(), cus, c->n, c->F,
+             fbits(c->r2), fbits(o.theta), o.max_k, o.sched, o.subset_bound ? 1 : 0);
     return b;
 }
 

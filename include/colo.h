@@ -215,6 +215,14 @@ typedef struct colo_mine_opts {
     const char *log_path;   /* append (features, rows, refinement tests)
                                records of every counted candidate (training
                                data of the cost model), or NULL             */
+    int32_t subset_bound;   /* 1 = from size 3 on, skip the candidates whose
+                               subset bound is below theta: the bound caps
+                               both PIs by the AND of each member's
+                               participation in the candidate's (k-1)-subsets
+                               (no counterpart in the reference; the pattern
+                               files are unchanged, the report shows the
+                               bound after "q" and omits the skipped
+                               candidates' counts)                         */
 } colo_mine_opts;
 
 /* Per-run measurements (milliseconds unless stated).  The *_kernel_ms
@@ -247,6 +255,8 @@ typedef struct colo_mine_stats {
     uint64_t slots_deferred;/* candidates counted without per-row slots
                                (slots arena above its budget)              */
     uint64_t recounts;      /* candidates recounted for their table's slots */
+    uint64_t pruned;        /* candidates skipped by the subset bound        */
+    double bound_ms;        /* subset bounds: wall time                      */
 } colo_mine_stats;
 
 /* HIP graphs: the binning (colo_bin, colo_ctx_create) is captured once per

@@ -855,6 +855,9 @@ typedef struct {
     uint64_t nslots;
     uint64_t rows;
     uint32_t *table;    /* generated table, rows*k, or NULL */
+    uint8_t *part;      /* exact participation, 0/1 per instance of each
+                           member in input rank order, members back to back
+                           (kept with subset_bound) */
 } prev_t;
 
 static void write_patterns(const orc_ctx *c, const char *prefix, uint32_t k, const prev_t *L, uint64_t P) {
@@ -897,11 +900,70 @@ static int64_t find_pattern(const prev_t *L, uint64_t P, uint32_t w, const uint3
  * needInstanceTable test (Q10) would skip tables that the next level needs,
  * the report records it on a "q10" line and the tables are built anyway.
  * pi_mode 0 = compat numerators (cu:108-132), 1 = exact.  max_k = 0 means no
- * cap.  Returns a malloc'd report (free with orc_free_str). */
+ * cap.  subset_bound = 1 skips, from level 3 on, the candidates whose subset
+ * bound (subset_bound_q) is below theta: their report line ends with the
+ * bound after "q", and every candidate that passes both tests carries it
+ * too.  Returns a malloc'd report (free with orc_free_str). */
 static double wall_s(void) { return omp_get_wtime(); }
 
+/* Subset bound of a size-k candidate (k >= 3).  An instance of member f
+ * that takes part in a row of the candidate's table takes part in a row of
+ * every (k-1)-subset S containing f (drop the other column), so
+ *   num_exact(f) <= cnt_f = |intersection over S of part_f(S)|.
+ * A reference numerator (pi_mode 0, cu:108-132 with the back-to-back layout
+ * of K:298-310) is num_j = P_{j-1}[N_{j-1}-1] + pop(P_j) - P_j[N_j-1] for
+ * j >= 1 and pop(P_0) - P_0[N_0-1] for j = 0, so
+ *   num_compat(j) <= cnt_j + last_{j-1},
+ * last_{j-1} = whether instance N-1 of member j-1 lies in its intersection.
+ * Both PIs are therefore at most
+ *   q = min(1, min_j fl32(cnt_j + t_j) / fl32(N_j)),  t_j = 0 (exact) or
+ *   last_{j-1} for j >= 1 (reference numerators),
+ * and a candidate with q < theta is not prevalent.  Every (k-1)-subset of a
+ * candidate is a prevalent pattern of the previous level (the Apriori
+ * subset test of cpp:517-557), whose participation Lprev holds.  Returns q;
+ * no counterpart in the reference. */
+static float subset_bound_q(const orc_ctx *c, const prev_t *Lprev, uint64_t Pprev, uint32_t k, const uint32_t *cand,
+                            int pi_mode) {
+    const uint8_t *win[64];
+    uint32_t sub[64];
+    uint64_t cnt[64];
+    int last[64];
+    for (uint32_t j = 0; j < k; j++) {
+        uint32_t nw = 0;
+        for (uint32_t i = 0; i < k; i++) {
+            if (i == j) continue;
+            uint32_t w = 0;
+            for (uint32_t t = 0; t < k; t++) if (t != i) sub[w++] = cand[t];
+            int64_t s = find_pattern(Lprev, Pprev, k - 1, sub);
+            if (s < 0 || !Lprev[s].part) return 1.0f;   /* not reached: candgen keeps only such candidates */
+            uint32_t pos = j < i ? j : j - 1;
+            uint64_t off = 0;
+            for (uint32_t t = 0; t < pos; t++) off += c->fcount[sub[t]];
+            win[nw++] = Lprev[s].part + off;
+        }
+        uint32_t N = c->fcount[cand[j]];
+        uint64_t n1 = 0;
+        for (uint32_t w = 0; w < N; w++) {
+            uint8_t a = 1;
+            for (uint32_t t = 0; t < nw && a; t++) a &= win[t][w];
+            n1 += a;
+        }
+        cnt[j] = n1;
+        uint8_t a = N > 0;
+        for (uint32_t t = 0; t < nw && a; t++) a &= win[t][N - 1];
+        last[j] = a;
+    }
+    float q = 1.0f;
+    for (uint32_t j = 0; j < k; j++) {
+        uint64_t v = cnt[j] + (pi_mode == 0 && j >= 1 ? (uint64_t)last[j - 1] : 0);
+        volatile float b = (float)v / (float)c->fcount[cand[j]];
+        if (b < q) q = b;
+    }
+    return q;
+}
+
 char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, int filter_on,
-               int hash_tables, const char *out_prefix, int *err) {
+               int hash_tables, int subset_bound, const char *out_prefix, int *err) {
     sbuf b = {0};
     b.cap = 4096; b.s = (char *)malloc(b.cap); b.s[0] = 0;
     *err = ORC_OK;
@@ -933,6 +995,13 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
         else memset(pass, 1, nC);
         uint64_t npass = 0;
         for (uint64_t i = 0; i < nC; i++) npass += pass[i];
+        float *qv = NULL;
+        if (subset_bound && k >= 3) {
+            qv = (float *)malloc(sizeof(float) * nC);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int64_t i = 0; i < (int64_t)nC; i++)
+                qv[i] = pass[i] ? subset_bound_q(c, Lprev, Pprev, k, C + i * k, pi_mode) : 1.0f;
+        }
         prev_t *L = (prev_t *)calloc(nC, sizeof(prev_t));
         uint64_t P = 0;
         sbuf lb = {0}; lb.cap = 4096; lb.s = (char *)malloc(lb.cap); lb.s[0] = 0;
@@ -949,6 +1018,10 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
             for (uint32_t m = 0; m < k; m++) sb_printf(&lb, " %u", fcnt[i * k + m]);
             sb_printf(&lb, " p %u", (unsigned)pass[i]);
             if (!pass[i]) { sb_printf(&lb, "\n"); continue; }
+            if (qv) {
+                sb_printf(&lb, " q %08x", fbits(qv[i]));
+                if (qv[i] < theta) { sb_printf(&lb, "\n"); continue; }
+            }
             uint64_t wsz = 0;
             for (uint32_t m = 0; m < k; m++) wsz += c->fcount[cand[m]];
             uint32_t *bm = (uint32_t *)calloc(wsz + 1, sizeof(uint32_t));
@@ -978,6 +1051,10 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
             sb_printf(&lb, " e");
             for (uint32_t m = 0; m < k; m++) sb_printf(&lb, " %llu", (unsigned long long)exn[m]);
             sb_printf(&lb, " pi %08x %08x v %d\n", fbits(pic), fbits(pie), prevalent);
+            if (prevalent && subset_bound) {
+                L[P].part = (uint8_t *)malloc(wsz + 1);
+                for (uint64_t w = 0; w < wsz; w++) L[P].part[w] = ex[w] ? 1 : 0;
+            }
             if (ex != bm) free(ex);
             free(bm);
             if (prevalent) {
@@ -1035,14 +1112,14 @@ char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, in
         }
         free(need);
         for (uint64_t i = 0; i < P; i++) { free(L[i].slots); L[i].slots = NULL; }
-        for (uint64_t i = 0; i < Pprev; i++) { free(Lprev[i].pat); free(Lprev[i].table); }
+        for (uint64_t i = 0; i < Pprev; i++) { free(Lprev[i].pat); free(Lprev[i].table); free(Lprev[i].part); }
         free(Lprev);
         Lprev = L; Pprev = P;
-        free(fcnt); free(pass); free(C);
+        free(fcnt); free(pass); free(qv); free(C);
         C = N; nC = nN;
         k++;
     }
-    for (uint64_t i = 0; i < Pprev; i++) { free(Lprev[i].pat); free(Lprev[i].table); }
+    for (uint64_t i = 0; i < Pprev; i++) { free(Lprev[i].pat); free(Lprev[i].table); free(Lprev[i].part); }
     free(Lprev);
     free(C);
     sb_printf(&b, "end levels %u patterns %llu\n", levels, (unsigned long long)total_prev);

@@ -78,7 +78,7 @@ def lib():
         L.orc_row_hash.restype = C.c_uint64
         L.orc_row_hash.argtypes = [_u32p, C.c_uint32]
         L.orc_mine.restype = C.c_void_p
-        L.orc_mine.argtypes = [vp, C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_int, C.c_int,
+        L.orc_mine.argtypes = [vp, C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_char_p, C.POINTER(C.c_int)]
         L.orc_free_str.argtypes = [C.c_void_p]
         L.orc_boundary_pairs.argtypes = [vp, _u64p]
@@ -231,9 +231,10 @@ class Oracle:
         keys = ("tests", "neighbours", "fma_dx_first", "fma_dy_first", "double_sum", "any_form")
         return {k: int(v) for k, v in zip(keys, out)}
 
-    def mine(self, r, theta, max_k=0, pi_mode=0, filter_on=True, hash_tables=True, out_prefix=None):
+    def mine(self, r, theta, max_k=0, pi_mode=0, filter_on=True, hash_tables=True, out_prefix=None,
+             subset_bound=False):
         err = C.c_int(0)
-        p = lib().orc_mine(self.h, r, theta, max_k, pi_mode, int(filter_on), int(hash_tables),
+        p = lib().orc_mine(self.h, r, theta, max_k, pi_mode, int(filter_on), int(hash_tables), int(subset_bound),
                            out_prefix.encode() if out_prefix else None, C.byref(err))
         if not p:
             raise ValueError(f"oracle: mining failed (code {err.value})")

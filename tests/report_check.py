@@ -3,8 +3,11 @@ sample of its candidates against the oracle.
 
 A report (written identically by the GPU miner and by the oracle's orc_mine)
 has one line per candidate:
-    c <ids> f <filter counts> p <pass> [r <rows> n <compat num> e <exact num>
-    pi <compat PI bits> <exact PI bits> v <prevalent>]
+    c <ids> f <filter counts> p <pass> [q <subset bound bits>] [r <rows>
+    n <compat num> e <exact num> pi <compat PI bits> <exact PI bits>
+    v <prevalent>]
+(q only when mining with the subset bound, from size 3 on; a candidate
+whose bound is below theta has no counts)
 and one per instance table the reference would build:
     t <ids> rows <rows> [h <row-order independent checksum>]
 """
@@ -25,7 +28,9 @@ def parse_report(text):
             k = fi - 1
             cand = tuple(int(v) for v in t[1:fi])
             d = {"f": [int(v) for v in t[fi + 1:fi + 1 + k]], "p": int(t[t.index("p") + 1])}
-            if d["p"]:
+            if "q" in t:
+                d["q"] = t[t.index("q") + 1]
+            if "r" in t:
                 d["rows"] = int(t[t.index("r") + 1])
                 ni, ei, pi = t.index("n"), t.index("e"), t.index("pi")
                 d["numc"] = [int(v) for v in t[ni + 1:ni + 1 + k]]
@@ -62,10 +67,10 @@ def choose_sample(text, seed, per_level=None, max_parent_rows=3e7):
     per_level = per_level or {2: 15, 3: 10, 4: 5, 5: 3, 6: 3}
     cands, _, _ = parse_report(text)
     rng = np.random.default_rng(seed)
-    rows = {c: d["rows"] for c, d in cands.items() if d["p"]}
+    rows = {c: d["rows"] for c, d in cands.items() if "rows" in d}
     out = []
     for k in sorted({len(c) for c in cands}):
-        counted = sorted(c for c in cands if len(c) == k and cands[c]["p"])
+        counted = sorted(c for c in cands if len(c) == k and "rows" in cands[c])
         if k == 2:
             top = sorted(counted, key=lambda c: -rows[c])[:5]
             out.extend(top)
@@ -125,7 +130,7 @@ def check_sample(oracle_mod, orc, text, theta, sample, say=print):
             assert int(ps) == d["p"], c
         for c in cs:
             d = cands[c]
-            if not d["p"]:
+            if "rows" not in d:
                 continue
             say(f"oracle check {c}")
             if k == 2:

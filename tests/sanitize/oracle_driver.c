@@ -30,7 +30,7 @@ void orc_filter_literal(const orc_ctx *c, const uint32_t *cands, uint32_t C, uin
                         uint32_t *counts, uint8_t *pass);
 uint64_t orc_candgen(const uint32_t *L, uint64_t P, uint32_t km1, uint32_t *out, uint64_t cap);
 char *orc_mine(orc_ctx *c, float r, float theta, uint32_t max_k, int pi_mode, int filter_on, int hash_tables,
-               const char *out_prefix, int *err);
+               int subset_bound, const char *out_prefix, int *err);
 void orc_free_str(char *s);
 
 #define CHECK(c)                                                          \
@@ -75,12 +75,15 @@ int main(int argc, char **argv) {
     char prefix[512];
     snprintf(prefix, sizeof prefix, "%s/san_", tmp);
     int err = 0;
-    char *rep = orc_mine(c, 12.0f, 0.05f, 0, 0, 1, 1, prefix, &err);
+    char *rep = orc_mine(c, 12.0f, 0.05f, 0, 0, 1, 1, 0, prefix, &err);
     CHECK(rep && !err && strstr(rep, "end levels 4"));
-    char *rep2 = orc_mine(c, 12.0f, 0.05f, 0, 1, 0, 1, NULL, &err);
+    char *rep2 = orc_mine(c, 12.0f, 0.05f, 0, 1, 0, 1, 0, NULL, &err);
     CHECK(rep2 && !err);
+    char *rep3 = orc_mine(c, 12.0f, 0.05f, 0, 0, 1, 1, 1, NULL, &err);
+    CHECK(rep3 && !err && strstr(rep3, "end levels 4"));
     orc_free_str(rep);
     orc_free_str(rep2);
+    orc_free_str(rep3);
     /* filter: both forms agree */
     uint32_t cands[2 * 10], C = 0;
     for (uint32_t a = 0; a < F; a++)

@@ -522,3 +522,71 @@ def test_many_features(datagen, colo_mod, oracle_mod, F):
     assert rep == orc.mine(p["r"], p["theta"], max_k=p["max_k"])
     lv3 = [ln for ln in rep.split("\n") if ln.startswith("level 3 ")]
     assert lv3 and not lv3[0].endswith(" prev 0"), lv3  # a prevalent triple of features above 64
+
+
+# 14 features of 20k instances around 30 shared hotspots: every pair is
+# prevalent at theta 0.15 and most of the 364 triples are not
+DENSE = dict(N=20000, seed=3, F=14, hot_frac=0.5, hotspots=30, sigma=(300, 300), theta=0.15, r=80.0)
+
+
+@pytest.fixture(scope="module")
+def dense(datagen, colo_mod, oracle_mod):
+    _say("fixture dense c1 variant N=20000 F=14")
+    x, y, fc, b, p = _workload(datagen, "c1", **DENSE)
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    return ctx, orc, p
+
+
+@pytest.mark.parametrize("sched", ["plain", "persistent", "learned"])
+@pytest.mark.parametrize("pi_mode", [0, 1])
+def test_subset_bound(dense, colo_mod, sched, pi_mode):
+    """Mining with the subset bound: bounds, skipped candidates, counts and
+    tables equal the oracle's under every schedule and numerator mode, and
+    the prevalent patterns and tables equal those of mining without it."""
+    ctx, orc, p = dense
+    _say(f"subset bound sched={sched} pi_mode={pi_mode}")
+    sc = {"plain": colo_mod.COLO_SCHED_PLAIN, "persistent": colo_mod.COLO_SCHED_PERSISTENT,
+          "learned": colo_mod.COLO_SCHED_LEARNED}[sched]
+    model = None
+    if sc == colo_mod.COLO_SCHED_LEARNED:
+        model = colo_mod.CostModel.load(os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin"))
+    rep, st = ctx.mine(p["theta"], max_k=4, pi_mode=pi_mode, sched=sc, model=model, subset_bound=True)
+    ref = orc.mine(p["r"], p["theta"], max_k=4, pi_mode=pi_mode, subset_bound=True)
+    assert rep == ref
+    assert st["pruned"] > 200
+    full, _ = ctx.mine(p["theta"], max_k=4, pi_mode=pi_mode, sched=sc, model=model)
+    from report_check import parse_report
+    c_on, t_on, l_on = parse_report(rep)
+    c_off, t_off, l_off = parse_report(full)
+    assert t_on == t_off and l_on == l_off
+    assert {c for c, d in c_on.items() if d.get("v")} == {c for c, d in c_off.items() if d.get("v")}
+
+
+@pytest.mark.parametrize("sched", ["plain", "persistent"])
+def test_subset_bound_sweep(dense, colo_mod, sched):
+    """A threshold sweep with the subset bound equals one mining run per
+    threshold (bounds and participation are kept across the sweep)."""
+    ctx, orc, p = dense
+    _say(f"subset bound sweep sched={sched}")
+    sc = colo_mod.COLO_SCHED_PLAIN if sched == "plain" else colo_mod.COLO_SCHED_PERSISTENT
+    thetas = [0.3, 0.1, 0.15]
+    out = ctx.mine_sweep(thetas, max_k=4, sched=sc, subset_bound=True)
+    for th, (rep, st) in zip(thetas, out):
+        assert rep == orc.mine(p["r"], th, max_k=4, subset_bound=True), th
+
+
+def test_subset_bound_c3_small(datagen, colo_mod, oracle_mod):
+    """c3's generator at 200k instances, sizes up to 5: the learned schedule
+    with the subset bound against the oracle with it."""
+    _say("subset bound c3 N=200000")
+    x, y, fc, b, p = _workload(datagen, "c3", N=200000, seed=7)
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    model = colo_mod.CostModel.load(os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin"))
+    rep, st = ctx.mine(p["theta"], max_k=p["max_k"], sched=colo_mod.COLO_SCHED_LEARNED, model=model,
+                       subset_bound=True)
+    assert rep == orc.mine(p["r"], p["theta"], max_k=p["max_k"], subset_bound=True)
+    assert st["pruned"] > 500

@@ -18,7 +18,11 @@ region.  c1/c2/c3 are available with --workload.
 
 Prints one JSON line.  value = neighbour pairs per second of the level-2
 join (pairs counted by the level-2 count pass / wall time of the level-2
-count + PI + scan + write); mining_s = wall time of a step.  The timed
+count + PI + scan + write); mining_s = wall time of a step.  Every schedule
+mines with the subset bound (from size 3 on, candidates whose
+participation-intersection bound is below theta are not counted; the
+prevalent patterns and their tables are unchanged), and the timed schedule
+also runs without it, with the two results compared.  The timed
 schedule is the learned one (persistent launches whose work queue is ordered
 and partitioned, and whose table arenas are pre-sized, by the trained cost
 model in gpu-colocationmininig_amd/models/cost_mlp.bin) when that model is
@@ -124,10 +128,29 @@ def cpu_level2_sample(x, y, fc, bounds, p, budget_s, threads):
             "sample_s": t_tot, "sample_pairs": used, "all_pairs": len(pairs)}
 
 
-def cpu_vs_gpu_mining(colo, model, threads):
+def essentials(report):
+    """What the subset bound must leave unchanged: the level lines, the
+    instance tables (rows and checksums) and the prevalent candidates' lines
+    (bound field removed)."""
+    keep = []
+    for line in report.splitlines():
+        if line.startswith("c ") and line.endswith(" v 1"):
+            t = line.split()
+            if "q" in t:
+                i = t.index("q")
+                del t[i:i + 2]
+            keep.append(" ".join(t))
+        elif not line.startswith("c "):
+            keep.append(line)
+    return keep
+
+
+def cpu_vs_gpu_mining(colo, model, threads, subset_bound):
     """Complete mining of c2 (oracle on `threads` threads and on 1 thread) and
     c1 (1 thread), each beside the GPU learned schedule on the same data,
-    reports (with table checksums) compared."""
+    reports (with table checksums) compared: in full without the subset
+    bound, else their prevalent patterns and tables (the oracle runs the
+    reference's candidate set either way)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import torch
@@ -136,22 +159,24 @@ def cpu_vs_gpu_mining(colo, model, threads):
         x, y, fc, b, p = gen(name)
         ctx = colo.Context(x, y, fc, b, p["r"])
         sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
-        ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=model, hash_tables=False)  # warm
+        kw = dict(max_k=p["max_k"], sched=sched, model=model, subset_bound=subset_bound)
+        ctx.mine(p["theta"], hash_tables=False, **kw)  # warm
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ctx.rebin(p["r"])
-        rep, _ = ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=model, hash_tables=False)
+        rep, _ = ctx.mine(p["theta"], hash_tables=False, **kw)
         torch.cuda.synchronize()
         g_s = time.perf_counter() - t0
-        grep, _ = ctx.mine(p["theta"], max_k=p["max_k"], sched=sched, model=model, hash_tables=True)
-        entry = {"workload": describe(p), "gpu_s": g_s}
+        grep, _ = ctx.mine(p["theta"], hash_tables=True, **kw)
+        entry = {"workload": describe(p), "gpu_s": g_s, "gpu_subset_bound": subset_bound}
         for t in tlist:
             oracle.set_threads(t)
             t0 = time.perf_counter()
             o = oracle.Oracle.from_arrays(x, y, fc, b)
             orep = o.mine(p["r"], p["theta"], max_k=p["max_k"], hash_tables=True)
             entry[f"cpu_s_{t}t"] = time.perf_counter() - t0
-            entry["identical"] = entry.get("identical", True) and orep == grep
+            same = essentials(orep) == essentials(grep) if subset_bound else orep == grep
+            entry["identical"] = entry.get("identical", True) and same
             del o
         out[name] = entry
         log(f"cpu vs gpu {name}: " + ", ".join(f"{k} {v:.3f}" for k, v in entry.items() if k.endswith("_s") or "_s_" in k)
@@ -160,10 +185,12 @@ def cpu_vs_gpu_mining(colo, model, threads):
     return out
 
 
-def cpu_full(colo, model, threads, name):
+def cpu_full(colo, model, threads, name, subset_bound):
     """Complete mining of one workload by the oracle on `threads` host
-    threads and by the GPU (learned schedule, best of three after a warm-up),
-    reports with instance-table checksums compared."""
+    threads (the reference's candidate set) and by the GPU (learned
+    schedule, best of three after a warm-up), reports with instance-table
+    checksums compared (prevalent patterns and tables with the subset
+    bound)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import torch
@@ -171,13 +198,13 @@ def cpu_full(colo, model, threads, name):
     th = min(p["thetas"]) if "thetas" in p else p["theta"]
     ctx = colo.Context(x, y, fc, b, p["r"])
     sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
-    grep, _ = ctx.mine(th, max_k=p["max_k"], sched=sched, model=model, hash_tables=True)
+    grep, _ = ctx.mine(th, max_k=p["max_k"], sched=sched, model=model, hash_tables=True, subset_bound=subset_bound)
     best = None
     for _ in range(3):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ctx.rebin(p["r"])
-        ctx.mine(th, max_k=p["max_k"], sched=sched, model=model, hash_tables=False)
+        ctx.mine(th, max_k=p["max_k"], sched=sched, model=model, hash_tables=False, subset_bound=subset_bound)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
@@ -189,8 +216,9 @@ def cpu_full(colo, model, threads, name):
     o = oracle.Oracle.from_arrays(x, y, fc, b)
     orep = o.mine(p["r"], th, max_k=p["max_k"], hash_tables=True)
     cpu_s = time.perf_counter() - t0
-    out = {"workload": describe(p), "theta": th, "gpu_s": best, f"cpu_s_{threads}t": cpu_s, "cores": threads,
-           "identical": orep == grep, "kind": "port", **cpu_info()}
+    same = essentials(orep) == essentials(grep) if subset_bound else orep == grep
+    out = {"workload": describe(p), "theta": th, "gpu_s": best, "gpu_subset_bound": subset_bound,
+           f"cpu_s_{threads}t": cpu_s, "cores": threads, "identical": same, "kind": "port", **cpu_info()}
     log(f"cpu full {name}: oracle {cpu_s:.1f} s, identical {out['identical']}")
     return out
 
@@ -211,13 +239,14 @@ def e2e_child(args):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ctx = colo.Context(x, y, fc, bounds, p["r"])
-        ctx.autotune(th, max_k=p["max_k"], sched=sched, model=model, cache=args.tune_cache)
+        sb = args.subset_bound == "on"
+        ctx.autotune(th, max_k=p["max_k"], sched=sched, model=model, cache=args.tune_cache, subset_bound=sb)
         _, st = ctx.mine(th, max_k=p["max_k"], sched=sched, hash_tables=False, out_prefix=os.path.join(td, ""),
-                         model=model)
+                         model=model, subset_bound=sb)
         torch.cuda.synchronize()
         e2e = time.perf_counter() - t0
         files = sorted(f for f in os.listdir(td) if f.startswith("ColocationDegree_"))
-    print(json.dumps({"s": e2e, "theta": th, "mining_ms": st["total_ms"], "alloc_ms": st["alloc_ms"],
+    print(json.dumps({"s": e2e, "theta": th, "subset_bound": sb, "mining_ms": st["total_ms"], "alloc_ms": st["alloc_ms"],
                       "alloc_gb": st["alloc_bytes"] / 1e9,
                       "level_ms": {str(k): round(v, 2) for k, v in enumerate(st["level_ms"]) if v > 0},
                       "pattern_files": files, "upload_and_bin_included": True, "csv_parse_included": False,
@@ -238,6 +267,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--model", default=os.path.join(ROOT, "gpu-colocationmininig_amd", "models", "cost_mlp.bin"))
     ap.add_argument("--no-model", action="store_true")
+    ap.add_argument("--subset-bound", choices=["on", "off"], default="on",
+                    help="mine with the subset bound (default on; the timed schedule also runs without it)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "colo_tune.txt"))
     ap.add_argument("--e2e-child", action="store_true", help=argparse.SUPPRESS)
@@ -260,7 +291,7 @@ def main():
     if args.cpu_full:
         import colo
         model = None if args.no_model or not os.path.exists(args.model) else colo.CostModel.load(args.model)
-        print(json.dumps(cpu_full(colo, model, cpu_threads(), args.cpu_full)), flush=True)
+        print(json.dumps(cpu_full(colo, model, cpu_threads(), args.cpu_full, args.subset_bound == "on")), flush=True)
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is a design "
@@ -285,14 +316,17 @@ def main():
     sched_name = {colo.COLO_SCHED_LEARNED: "learned", colo.COLO_SCHED_PERSISTENT: "persistent",
                   colo.COLO_SCHED_PLAIN: "plain"}
     thetas = p["thetas"]
+    sb = args.subset_bound == "on"
 
-    def tune_for(sched, m):
+    def tune_for(sched, m, bound=sb):
         if args.no_tune:
             return ctx.tune
         t0 = time.perf_counter()
         os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
-        t = ctx.autotune(min(thetas), max_k=p["max_k"], sched=sched, model=m, cache=args.tune_cache)
-        log(f"autotuned {sched_name[sched]} launch shapes in {time.perf_counter() - t0:.1f} s: {t}")
+        t = ctx.autotune(min(thetas), max_k=p["max_k"], sched=sched, model=m, cache=args.tune_cache,
+                         subset_bound=bound)
+        log(f"autotuned {sched_name[sched]} launch shapes{'' if bound else ' (no subset bound)'} in "
+            f"{time.perf_counter() - t0:.1f} s: {t}")
         return t
 
     default_tune = ctx.tune
@@ -300,13 +334,13 @@ def main():
 
     bin_ms = []
 
-    def step(sched, hash_tables=False):
+    def step(sched, hash_tables=False, bound=sb):
         t0 = time.perf_counter()
         ctx.rebin(p["r"])
         torch.cuda.synchronize()
         bin_ms.append((time.perf_counter() - t0) * 1000.0)
         out = ctx.mine_sweep(thetas, max_k=p["max_k"], sched=sched, hash_tables=hash_tables,
-                             model=model if sched == colo.COLO_SCHED_LEARNED else None)
+                             model=model if sched == colo.COLO_SCHED_LEARNED else None, subset_bound=bound)
         return [o[0] for o in out], [o[1] for o in out]
 
     for i in range(args.warmup):
@@ -334,7 +368,7 @@ def main():
            "data": f"synthetic: seeded generator ({p['name']}), generated in memory",
            "config": {"workload": describe(p), "instances": p["N"], "features": p["F"], "r": p["r"],
                       "thetas": thetas, "max_k": p["max_k"], "schedule": sched_name[main_sched],
-                      "launch_shapes": tune}}
+                      "subset_bound": sb, "launch_shapes": tune}}
     out["mining_s"] = {sched_name[main_sched]: ms_per_step / 1000.0}
     if model is not None:
         n_m = sum(s["model_n"] for s in stats)
@@ -350,6 +384,8 @@ def main():
     out["levels"] = {f"{t:g}": s["levels"] for t, s in zip(thetas, first)}
     out["level_ms"] = {f"{t:g}": {str(k): round(v, 2) for k, v in enumerate(s["level_ms"]) if v > 0}
                        for t, s in zip(thetas, first)}
+    out["subset_bound"] = {"on": sb, "pruned": {f"{t:g}": s["pruned"] for t, s in zip(thetas, first)},
+                           "bound_ms_per_step": sum(s["bound_ms"] for s in stats) / args.steps}
     out["memory"] = {"pool_peak_gb": max(s["pool_peak_bytes"] for s in stats) / 1e9,
                      "alloc_ms_per_step": sum(s["alloc_ms"] for s in stats) / args.steps,
                      "slots_deferred": sum(s["slots_deferred"] for s in first),
@@ -387,6 +423,29 @@ def main():
 
     if not args.no_plain:
         main_s = ms_per_step / 1000.0
+        if sb:
+            # the timed schedule without the subset bound (every candidate
+            # that passes the filter is counted, as in the reference), with
+            # launch shapes tuned for that; same prevalent patterns and tables
+            ntune = tune_for(main_sched, model, bound=False)
+            step(main_sched, bound=False)  # warm
+            reps = min(args.steps, 3)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                step(main_sched, bound=False)
+            torch.cuda.synchronize()
+            nb = (time.perf_counter() - t0) / reps
+            out["mining_s"][f"{sched_name[main_sched]}_no_subset_bound"] = nb
+            out["subset_bound"]["speedup"] = nb / main_s
+            out["subset_bound"]["no_bound_launch_shapes"] = ntune
+            log(f"{sched_name[main_sched]} without the subset bound: {nb:.3f} s per step")
+            with_b, _ = step(main_sched, hash_tables=True)
+            without_b, _ = step(main_sched, hash_tables=True, bound=False)
+            same = all(essentials(a) == essentials(b) for a, b in zip(with_b, without_b))
+            out["subset_bound"]["patterns_and_tables_identical"] = same
+            log(f"with and without the subset bound, prevalent patterns and table checksums identical: {same}")
+            ctx.tune = tune
         # persistent with the launch shapes tuned for itself
         if main_sched != colo.COLO_SCHED_PERSISTENT:
             ctx.tune = default_tune
@@ -428,7 +487,7 @@ def main():
         # filter-independent neighbour-pair rate; sizes up to 3 so that the
         # level-2 tables the next level extends are written, as in a step)
         _, st0 = ctx.mine(min(thetas), max_k=3, filter_on=False, sched=main_sched, hash_tables=False,
-                          model=model if main_sched == colo.COLO_SCHED_LEARNED else None)
+                          model=model if main_sched == colo.COLO_SCHED_LEARNED else None, subset_bound=sb)
         out["level2_filter_off"] = {"pairs": st0["l2_pairs"], "join_ms": st0["l2_join_ms"],
                                     "pairs_per_s": st0["l2_pairs"] / (st0["l2_join_ms"] / 1000.0)
                                     if st0["l2_join_ms"] else None, "candidates": st0["candidates"]}
@@ -441,7 +500,8 @@ def main():
         import subprocess
         torch.cuda.empty_cache()
         cmd = [sys.executable, os.path.abspath(__file__), "--e2e-child", "--workload", args.workload,
-               "--tune-cache", args.tune_cache, "--model", args.model] + (["--no-model"] if model is None else []) + \
+               "--tune-cache", args.tune_cache, "--model", args.model, "--subset-bound", args.subset_bound] + \
+              (["--no-model"] if model is None else []) + \
               (["--N", str(args.N)] if args.N else [])
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
         if r.returncode == 0 and r.stdout.strip():
@@ -461,7 +521,7 @@ def main():
         cb["cpu_mining_s_lower_bound"] = cb["sample_s"]
         cb["gpu_step_s"] = ms_per_step / 1000.0
         out["cpu_baseline"] = cb
-        out["cpu_vs_gpu_mining"] = cpu_vs_gpu_mining(colo, model, threads)
+        out["cpu_vs_gpu_mining"] = cpu_vs_gpu_mining(colo, model, threads, sb)
     log("done")
     print(json.dumps(out), flush=True)
     return 0

@@ -52,12 +52,13 @@ TUNE_UNSET = 0xFFFFFFFF
 
 class Tune(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("count_var", "write_var", "blocks_per_cu", "item_rows")] + \
-               [("count_var_k", C.c_uint32 * 16), ("item_rows_k", C.c_uint32 * 16), ("slots_budget_kb", C.c_uint32)]
+               [("count_var_k", C.c_uint32 * 16), ("item_rows_k", C.c_uint32 * 16), ("slots_budget_kb", C.c_uint32),
+                ("write_rows_k", C.c_uint32 * 16)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
-        d["count_var_k"] = list(self.count_var_k)
-        d["item_rows_k"] = list(self.item_rows_k)
+        for n in ("count_var_k", "item_rows_k", "write_rows_k"):
+            d[n] = list(getattr(self, n))
         return d
 
 
@@ -305,13 +306,14 @@ class Context:
         for i in range(16):
             t.count_var_k[i] = cur["count_var_k"][i]
             t.item_rows_k[i] = cur["item_rows_k"][i]
+            t.write_rows_k[i] = cur["write_rows_k"][i]
         self._check(lib().colo_set_tune(self.h, C.byref(t)))
 
     @_on_stream
     def autotune(self, theta, max_k=0, sched=COLO_SCHED_PERSISTENT, model=None, cache=None, pi_mode=0,
-                 filter_on=True):
+                 filter_on=True, subset_bound=False):
         """Chooses the launch shapes on this context's data (colo_autotune)."""
-        o = _opts(theta, max_k, pi_mode, filter_on, sched, False, None, model, None)
+        o = _opts(theta, max_k, pi_mode, filter_on, sched, False, None, model, None, subset_bound)
         t = Tune()
         self._check(lib().colo_autotune(self.h, C.byref(o), cache.encode() if cache else None, C.byref(t)))
         return t.as_dict()

@@ -3063,6 +3063,7 @@ int colo_get_tune(const colo_ctx *c, colo_tune *o) {
     o->blocks_per_cu = c->tune.blocks_per_cu;
     o->item_rows = c->tune.item_rows;
     o->slots_budget_kb = c->tune.slots_budget_kb;
+    for (int k = 0; k < 16; k++) o->write_rows_k[k] = c->tune.write_rows_k[k];
     return COLO_OK;
 }
 
@@ -3070,7 +3071,8 @@ int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     if (!c || !t) return COLO_EARG;
     for (int k = 0; k < 16; k++)
         if ((t->count_var_k[k] != COLO_TUNE_UNSET && t->count_var_k[k] > 13) ||
-            (t->item_rows_k[k] != COLO_TUNE_UNSET && (t->item_rows_k[k] < 64 || t->item_rows_k[k] > 65536)))
+            (t->item_rows_k[k] != COLO_TUNE_UNSET && (t->item_rows_k[k] < 64 || t->item_rows_k[k] > 65536)) ||
+            (t->write_rows_k[k] != COLO_TUNE_UNSET && (t->write_rows_k[k] < 64 || t->write_rows_k[k] > 65536)))
             return set_error(c, COLO_EARG, "launch shape out of range");
     if (t->count_var > 13 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
@@ -3084,6 +3086,7 @@ int colo_set_tune(colo_ctx *c, const colo_tune *t) {
     c->tune.blocks_per_cu = t->blocks_per_cu;
     c->tune.item_rows = t->item_rows;
     c->tune.slots_budget_kb = t->slots_budget_kb;
+    for (int k = 0; k < 16; k++) c->tune.write_rows_k[k] = t->write_rows_k[k];
     return COLO_OK;
 }
 

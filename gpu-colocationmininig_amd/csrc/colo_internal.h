@@ -208,6 +208,13 @@ struct TuneParams {
     uint32_t item_rows_of(uint32_t k) const {
         return (k < 16 && item_rows_k[k] != COLO_TUNE_UNSET) ? item_rows_k[k] : item_rows;
     }
+    uint32_t write_rows_k[16] = {COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET,
+                                 COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET,
+                                 COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET,
+                                 COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET, COLO_TUNE_UNSET};
+    uint32_t write_rows_of(uint32_t k) const {
+        return (k < 16 && write_rows_k[k] != COLO_TUNE_UNSET) ? write_rows_k[k] : item_rows_of(k);
+    }
     uint32_t write_var = 0;       // level-k write: same choice
     uint32_t blocks_per_cu = 8;   // persistent grid: workgroups per CU
     uint32_t item_rows = 1024;    // rows (anchors) per work item without the cost model

@@ -903,11 +903,51 @@ struct Miner {
             if (k == 2) st.l2_rows += p.rows;
         }
         if (desc.empty()) return COLO_OK;
-        for (auto &r : parent_runs(desc, k))
-            make_items(r.first, desc[r.first].n_anchor, c->tune.item_rows_of(k), items, r.second);
         if (!ensure_buf(d_desc, desc.size() * sizeof(CandDesc)) || !ensure_buf(d_queue, 64))
             return set_error(c, COLO_ENOMEM, "descriptors");
         COLO_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(CandDesc), hipMemcpyHostToDevice, s));
+        const auto runs = parent_runs(desc, k);
+        if (tuning_ && k < 16) {
+            // the level's item size of the write: each size writes the
+            // whole tables (the same rows every time), the fastest is kept
+            static const uint32_t sizes[] = {1024, 4096, 16384};
+            hipEvent_t ev[2];
+            COLO_HIP(hipEventCreate(&ev[0]));
+            if (hipEventCreate(&ev[1]) != hipSuccess) {
+                (void)hipEventDestroy(ev[0]);
+                return set_error(c, COLO_EHIP, "event");
+            }
+            double best = 1e300;
+            uint32_t bs = c->tune.write_rows_of(k);
+            int rc0 = COLO_OK;
+            for (uint32_t sz : sizes) {
+                items.clear();
+                for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, sz, items, r.second);
+                if ((rc0 = upload_items(items, nullptr))) break;
+                float ms = 0;
+                if (hipEventRecord(ev[0], s) != hipSuccess) { rc0 = set_error(c, COLO_EHIP, "event"); break; }
+                if ((rc0 = write_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(), nullptr,
+                                       d_queue.as<uint32_t>(), s)))
+                    break;
+                if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+                    rc0 = set_error(c, COLO_EHIP, "tuning launch");
+                    break;
+                }
+                if (c->verbose)
+                    fprintf(stderr, "[colo tune] level %u write, %u-row items: %.2f ms\n", k, sz, ms);
+                if (ms < best) {
+                    best = ms;
+                    bs = sz;
+                }
+            }
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+            if (rc0) return rc0;
+            c->tune.write_rows_k[k] = bs;
+            items.clear();
+        }
+        for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, c->tune.write_rows_of(k), items, r.second);
         {
             int rc0 = upload_items(items, nullptr);
             if (rc0) return rc0;
@@ -1553,7 +1593,7 @@ static std::string tune_key(const colo_ctx *c, const colo_mine_opts &o) {
     for (char &ch : arch)
         if (ch == ' ') ch = '_';
     char b[256];
-    snprintf(b, sizeof b, "v4/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d/b%d", arch.c_str(), cus, c->n, c->F,
+    snprintf(b, sizeof b, "v5/%s/%dcu/n%u/F%u/r%08x/t%08x/k%u/s%d/b%d", arch.c_str(), cus, c->n, c->F,
              fbits(c->r2), fbits(o.theta), o.max_k, o.sched, o.subset_bound ? 1 : 0);
     return b;
 }
@@ -1569,7 +1609,7 @@ static bool tune_load(const char *path, const std::string &key, TuneParams *t) {
         char k[256];
         int used = 0;
         if (sscanf(line, "%255s%n", k, &used) != 1 || key != k) continue;
-        constexpr int nf = 4 + 16 + 16;
+        constexpr int nf = 4 + 16 + 16 + 16;
         unsigned long v[nf];
         const char *p = line + used;
         int n = 0;
@@ -1588,6 +1628,7 @@ static bool tune_load(const char *path, const std::string &key, TuneParams *t) {
             t->count_var_k[i] = (uint32_t)v[4 + i];
             t->item_rows_k[i] = (uint32_t)v[20 + i];
         }
+        for (int i = 0; i < 16; i++) t->write_rows_k[i] = (uint32_t)v[36 + i];
         found = true;
     }
     fclose(fp);
@@ -1624,7 +1665,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             c->tune = best;
             return rc;
         };
-        for (int k = 0; k < 16; k++) best.count_var_k[k] = best.item_rows_k[k] = COLO_TUNE_UNSET;
+        for (int k = 0; k < 16; k++) best.count_var_k[k] = best.item_rows_k[k] = best.write_rows_k[k] = COLO_TUNE_UNSET;
         double cur[3], m[3];
         int rc = measure(best, cur);  // warm-up (pool, cached features)
         if (rc) return fail(rc);
@@ -1674,6 +1715,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
                         best.item_rows);
                 for (int k = 0; k < 16; k++) fprintf(fp, " %u", best.count_var_k[k]);
                 for (int k = 0; k < 16; k++) fprintf(fp, " %u", best.item_rows_k[k]);
+                for (int k = 0; k < 16; k++) fprintf(fp, " %u", best.write_rows_k[k]);
                 fprintf(fp, "\n");
                 fclose(fp);
             }

@@ -303,6 +303,9 @@ typedef struct colo_tune {
      * device's memory.  Candidates above it are counted without per-row
      * slots and recounted when their table is built (results unchanged). */
     uint32_t slots_budget_kb;
+    /* parent rows per work item of level k's batched table write (index
+     * 2..15); COLO_TUNE_UNSET = the level's item rows */
+    uint32_t write_rows_k[16];
 } colo_tune;
 #define COLO_TUNE_UNSET 0xffffffffu
 
@@ -312,10 +315,12 @@ int colo_set_tune(colo_ctx *ctx, const colo_tune *t);
  * shapes with the least HIP-event kernel time.  Per level, the count
  * variant and item size are chosen inside one mining run: each pair is
  * timed on a sample of that level's work items (one block of 16 items in
- * 8).  The write variant and the workgroups per CU are then chosen one at a
- * time by full runs (coordinate descent).  The context then uses the
- * result.  With cache_path, results are stored under a key of device and
- * workload (instances, features, r, theta, max_k, schedule) and a cached
+ * 8), and the write item size on that level's whole table write.  The
+ * write variant and the workgroups per CU are then chosen one at a time by
+ * full runs (coordinate descent).  The
+ * context then uses the result.  With cache_path, results are stored under
+ * a key of device and workload (instances, features, r, theta, max_k,
+ * schedule, subset bound) and a cached
  * entry is taken without measuring.  Every shape gives the same results;
  * only time differs. */
 int colo_autotune(colo_ctx *ctx, const colo_mine_opts *opts, const char *cache_path, colo_tune *out);

@@ -419,7 +419,8 @@ def main():
                        "algorithmic_bytes_per_launch": by / n if n else None}
     out["kernels_ms_per_step"] = {kn: v[0] / args.steps for kn, v in kern.items()}
     out["kernels_ms_per_step"].update({"filter": tot("filter_kernel_ms") / args.steps,
-                                       "pi": tot("pi_kernel_ms") / args.steps})
+                                       "pi": tot("pi_kernel_ms") / args.steps,
+                                       "offsets_scan": tot("scan_kernel_ms") / args.steps})
 
     if not args.no_plain:
         main_s = ms_per_step / 1000.0

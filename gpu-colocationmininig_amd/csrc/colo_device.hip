@@ -169,6 +169,7 @@ struct ScanSegDev {
     const uint32_t *in;
     uint64_t *out;
     uint64_t n, tile0, tile1;
+    uint32_t chunked, pad;  // chunked: out[c] = sum of in[0 .. 64c)
 };
 struct ScanTile {
     uint32_t seg, pad;
@@ -184,6 +185,16 @@ __global__ __launch_bounds__(kBlock) void k_segscan_reduce(const ScanSegDev *seg
 #pragma unroll
     for (int i = 0; i < kScanItems; i++)
         if (base + i < sg.n && base + i < t.start + kScanTile) v += sg.in[base + i];
+    if (sg.chunked) {
+        // sums of 64 rows (8 threads of 8 items), kept in out until
+        // k_segscan_apply_chunks turns them into prefixes
+        uint64_t g = v;
+        g += __shfl_xor(g, 1, 64);
+        g += __shfl_xor(g, 2, 64);
+        g += __shfl_xor(g, 4, 64);
+        const uint64_t ch = (t.start >> 6) + (threadIdx.x >> 3);
+        if ((threadIdx.x & 7) == 0 && ch < (sg.n + 63) >> 6) sg.out[ch] = g;
+    }
     uint64_t ex;
     const uint64_t tot = block_excl_scan(v, ex);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
@@ -200,13 +211,41 @@ __global__ __launch_bounds__(kBlock) void k_segscan_carry(const ScanSegDev *segs
         if (t < sg.tile1) partial[t] = run + ex;
         run += tot;
     }
-    if (threadIdx.x == 0) sg.out[sg.n] = run;
+    if (threadIdx.x == 0) sg.out[sg.chunked ? (sg.n + 63) >> 6 : sg.n] = run;
+}
+
+// Chunked segments: the 32 chunk sums of each tile (kScanTile = 32 x 64
+// rows) become the tile's carry plus their exclusive prefix.  One thread
+// per chunk, eight tiles per workgroup.
+__global__ __launch_bounds__(kBlock) void k_segscan_apply_chunks(const ScanSegDev *segs, const ScanTile *tiles,
+                                                                 uint64_t nt, const uint64_t *carry) {
+    static_assert(kScanTile == 32 * 64, "a tile holds 32 chunks of 64 rows");
+    const uint64_t ti = (uint64_t)blockIdx.x * (kBlock / 32) + (threadIdx.x >> 5);
+    const uint32_t q = threadIdx.x & 31;
+    uint64_t v = 0, ch = 0;
+    bool own = false;
+    ScanSegDev sg{};
+    if (ti < nt) {
+        const ScanTile t = tiles[ti];
+        sg = segs[t.seg];
+        ch = (t.start >> 6) + q;
+        own = sg.chunked && ch < (sg.n + 63) >> 6;
+        if (own) v = sg.out[ch];
+    }
+    uint64_t x = v;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d, 32);
+        if (q >= (uint32_t)d) x += y;
+    }
+    if (own) sg.out[ch] = carry[ti] + x - v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_segscan_apply(const ScanSegDev *segs, const ScanTile *tiles,
                                                           const uint64_t *carry) {
     const ScanTile t = tiles[blockIdx.x];
     const ScanSegDev sg = segs[t.seg];
+    if (sg.chunked) return;  // the whole workgroup: one tile, one segment
     const uint64_t base = t.start + (uint64_t)threadIdx.x * kScanItems;
     uint64_t v[kScanItems];
     uint64_t sum = 0;
@@ -2301,9 +2340,11 @@ __global__ __launch_bounds__(kBlock, var_waves(VAR)) void k_countk_batch(
 }
 
 // Level-k write over the rows of one parent table for ncand tables.
+// off1 (one candidate only): the row's table offset, already known (write
+// variant 3); otherwise it is read from the candidate's row offsets.
 template <int KM1>
 __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *descs, uint32_t ncand, uint64_t i,
-                                             const uint32_t *slots) {
+                                             const uint32_t *slots, uint64_t off1 = ~0ull) {
     const CandDesc &c0 = descs[0];
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1, k = km1 + 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
@@ -2342,7 +2383,7 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
         const CandDesc &cd = descs[j];
         const uint32_t nrows = slot_of(cd);
         const gcu32 *nbt = as_global(cd.nb_table);
-        uint32_t *out = cd.table + (cd.offsets[i] - cd.off_base) * k;
+        uint32_t *out = cd.table + (off1 != ~0ull ? off1 : cd.offsets[i] - cd.off_base) * k;
         uint32_t w = 0;
         auto test = [&](uint32_t q, float qx, float qy) {
             bool ok = true;
@@ -2412,6 +2453,21 @@ __global__ __launch_bounds__(kBlock) void k_writek_batch(DevData D, const CandDe
             for (uint32_t j = 0; j < wi.ncand; j++)
                 for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                     joink_write_row_multi<KM1>(D, desc + wi.cand + j, 1, i, slots);
+        } else if (VAR == 3) {
+            // offsets kept per 64 rows (items start at multiples of 64, so
+            // a wave's rows are one chunk): a row's offset is its chunk's
+            // plus the wave's exclusive sum of the slots before it
+            const uint32_t lane = threadIdx.x & 63;
+            for (uint32_t j = 0; j < wi.ncand; j++) {
+                const CandDesc &cd = desc[wi.cand + j];
+                const uint32_t *sl = cd.wslots ? cd.wslots : slots + cd.slots_off;
+                for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
+                    const uint64_t i = c0 + lane;
+                    const uint64_t nr = i < wi.end ? sl[i] : 0;
+                    const uint64_t off = cd.offsets[c0 >> 6] - cd.off_base + wave_incl_scan(nr) - nr;
+                    if (nr) joink_write_row_multi<KM1>(D, &cd, 1, i, slots, off);
+                }
+            }
         } else {
             for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
                 const uint32_t n = (uint32_t)(wi.end - c0 < 64 ? wi.end - c0 : 64);
@@ -2458,6 +2514,7 @@ template <int KM1> struct PickCount13 { static auto get() { return &k_countk_bat
 template <int KM1> struct PickWrite0 { static auto get() { return &k_writek_batch<KM1, 0>; } };
 template <int KM1> struct PickWrite1 { static auto get() { return &k_writek_batch<KM1, 1>; } };
 template <int KM1> struct PickWrite2 { static auto get() { return &k_writek_batch<KM1, 2>; } };
+template <int KM1> struct PickWrite3 { static auto get() { return &k_writek_batch<KM1, 3>; } };
 
 int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                       uint64_t n_items, uint32_t *d_slots, uint32_t *d_bits, uint64_t *d_rows, uint32_t *d_queue,
@@ -2499,7 +2556,8 @@ int colo::write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
         hipLaunchKernelGGL(k_write2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
                            d_items, d_seg, d_slots, d_queue);
     else
-        hipLaunchKernelGGL(c->tune.write_var == 2 ? pick_k<PickWrite2>(k)
+        hipLaunchKernelGGL(c->tune.write_var == 3   ? pick_k<PickWrite3>(k)
+                           : c->tune.write_var == 2 ? pick_k<PickWrite2>(k)
                            : c->tune.write_var == 1 ? pick_k<PickWrite1>(k) : pick_k<PickWrite0>(k),
                            dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
                            d_slots, d_queue);
@@ -3074,7 +3132,7 @@ int colo_set_tune(colo_ctx *c, const colo_tune *t) {
             (t->item_rows_k[k] != COLO_TUNE_UNSET && (t->item_rows_k[k] < 64 || t->item_rows_k[k] > 65536)) ||
             (t->write_rows_k[k] != COLO_TUNE_UNSET && (t->write_rows_k[k] < 64 || t->write_rows_k[k] > 65536)))
             return set_error(c, COLO_EARG, "launch shape out of range");
-    if (t->count_var > 13 || t->write_var > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
+    if (t->count_var > 13 || t->write_var > 3 || t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->item_rows < 64 ||
         t->item_rows > 65536)
         return set_error(c, COLO_EARG, "launch shape out of range");
     c->tune.count_var = t->count_var;
@@ -3354,7 +3412,7 @@ int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s
     std::vector<ScanSegDev> sd(nseg);
     std::vector<ScanTile> tiles;
     for (uint64_t i = 0; i < nseg; i++) {
-        sd[i] = ScanSegDev{segs[i].in, segs[i].out, segs[i].n, tiles.size(), 0};
+        sd[i] = ScanSegDev{segs[i].in, segs[i].out, segs[i].n, tiles.size(), 0, segs[i].chunked ? 1u : 0u, 0};
         for (uint64_t b = 0; b < segs[i].n; b += kScanTile) tiles.push_back(ScanTile{(uint32_t)i, 0, b});
         sd[i].tile1 = tiles.size();
     }
@@ -3375,7 +3433,14 @@ int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s
         hipLaunchKernelGGL(k_segscan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, d_part);
     }
     hipLaunchKernelGGL(k_segscan_carry, dim3((unsigned)nseg), dim3(kBlock), 0, s, d_sd, d_part);
-    if (nt) hipLaunchKernelGGL(k_segscan_apply, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, d_part);
+    bool any_rows = false, any_chunks = false;
+    for (uint64_t i = 0; i < nseg; i++) (segs[i].chunked ? any_chunks : any_rows) = true;
+    // per-row segments: tile scans; chunked ones: prefixes of the chunk sums
+    if (nt && any_rows)
+        hipLaunchKernelGGL(k_segscan_apply, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, d_part);
+    if (nt && any_chunks)
+        hipLaunchKernelGGL(k_segscan_apply_chunks, dim3((unsigned)((nt + kBlock / 32 - 1) / (kBlock / 32))),
+                           dim3(kBlock), 0, s, d_sd, d_tiles, nt, d_part);
     COLO_HIP(hipGetLastError());
     // the descriptors live in scratch: finish before the next scratch use
     COLO_HIP(hipStreamSynchronize(s));

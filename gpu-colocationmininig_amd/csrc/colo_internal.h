@@ -304,11 +304,14 @@ int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uin
 int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, const uint32_t *d_map, uint64_t *h,
                hipStream_t s);
 // Exclusive scans of many slot arrays in one pass: out has n + 1 entries,
-// the total last.  Waits for the stream before returning.
+// the total last.  With chunked, out holds one entry per 64 rows instead,
+// out[c] = sum of in[0 .. 64c), ceil(n/64) + 1 entries, the total last.
+// Waits for the stream before returning.
 struct ScanSeg {
     const uint32_t *in;
     uint64_t *out;
     uint64_t n;
+    bool chunked = false;
 };
 int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s);
 

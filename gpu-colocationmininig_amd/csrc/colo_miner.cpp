@@ -174,7 +174,7 @@ struct Miner {
     // HIP-event timing of launches on the miner's stream: mark() records an
     // event, span() pairs two marks with a category, harvest() (after a
     // stream synchronisation) adds the elapsed times to the statistics.
-    enum Cat { kCount, kWrite, kFilter, kPi, kL2Count, kL2Write };
+    enum Cat { kCount, kWrite, kFilter, kPi, kL2Count, kL2Write, kScan };
     std::vector<hipEvent_t> pool_;
     size_t used_ = 0;
     struct Span { size_t a, b; int cat; uint32_t k; };
@@ -203,6 +203,7 @@ struct Miner {
                 case kPi: st.pi_kernel_ms += ms; break;
                 case kL2Count: st.l2_count_kernel_ms += ms; st.count_kernel_ms += ms; break;
                 case kL2Write: st.l2_write_kernel_ms += ms; st.write_kernel_ms += ms; break;
+                case kScan: st.scan_kernel_ms += ms; break;
             }
         }
         spans_.clear();
@@ -429,6 +430,8 @@ struct Miner {
     std::shared_ptr<DevBuf> arena_offs;
     DevBuf arena_slots, arena_bits, d_desc, d_items, d_segs, d_segout, d_queue, d_idx, d_rowsel;
 
+    // Write items of chunk offsets start at multiples of 64 rows.
+    static uint64_t write_chunk(uint64_t rows, bool chunked) { return chunked ? (rows + 63) / 64 * 64 : rows; }
     static void make_items(uint64_t cand, uint64_t n, uint64_t chunk, std::vector<WorkItem> &items, uint32_t ncand = 1) {
         for (uint64_t b = 0; b < n; b += chunk) items.push_back(WorkItem{(uint32_t)cand, ncand, b, std::min(n, b + chunk)});
     }
@@ -834,10 +837,15 @@ struct Miner {
         for (size_t i = 0; i < L.size(); i++)
             if (build[i]) words += L[i].rows * table_words(k);
         // row offsets of the written tables: one exclusive scan per table
-        // into one offsets arena (n_anchor + 1 entries each, the total last)
+        // into one offsets arena (n_anchor + 1 entries each, the total
+        // last).  Write variant 3 (levels >= 3) keeps one offset per 64
+        // rows instead and rebuilds the rest from the slots; level 2 keeps
+        // every row's, which serve the later levels as list bounds.
+        const bool chunked = k > 2 && c->tune.write_var == 3;
+        auto n_offs = [&](uint64_t n) { return chunked ? (n + 63) / 64 + 1 : n + 1; };
         uint64_t owords = 0;
         for (size_t i = 0; i < L.size(); i++)
-            if (build[i]) owords += L[i].nslots + 1;
+            if (build[i]) owords += n_offs(L[i].nslots);
         arena_offs = std::make_shared<DevBuf>(&c->pool);
         if (!ensure_buf(*arena_offs, std::max<uint64_t>(owords, 1) * 8))
             return oom("offsets arena", std::max<uint64_t>(owords, 1) * 8);
@@ -849,11 +857,13 @@ struct Miner {
                 if (!build[i]) continue;
                 offs_at[i] = oat;
                 segs.push_back(ScanSeg{L[i].slots->as<uint32_t>() + L[i].slots_off, arena_offs->as<uint64_t>() + oat,
-                                       L[i].nslots});
-                oat += L[i].nslots + 1;
+                                       L[i].nslots, chunked});
+                oat += n_offs(L[i].nslots);
             }
+            size_t e0 = mark();
             int rc0 = scan_segments(c, segs.data(), segs.size(), s);
             if (rc0) return rc0;
+            span(e0, mark(), kScan, k);
         }
         std::shared_ptr<DevBuf> arena;
         if (arena_reserved_ && arena_reserved_->bytes >= words * 4) {
@@ -922,7 +932,7 @@ struct Miner {
             int rc0 = COLO_OK;
             for (uint32_t sz : sizes) {
                 items.clear();
-                for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, sz, items, r.second);
+                for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, write_chunk(sz, chunked), items, r.second);
                 if ((rc0 = upload_items(items, nullptr))) break;
                 float ms = 0;
                 if (hipEventRecord(ev[0], s) != hipSuccess) { rc0 = set_error(c, COLO_EHIP, "event"); break; }
@@ -947,7 +957,8 @@ struct Miner {
             c->tune.write_rows_k[k] = bs;
             items.clear();
         }
-        for (auto &r : runs) make_items(r.first, desc[r.first].n_anchor, c->tune.write_rows_of(k), items, r.second);
+        for (auto &r : runs)
+            make_items(r.first, desc[r.first].n_anchor, write_chunk(c->tune.write_rows_of(k), chunked), items, r.second);
         {
             int rc0 = upload_items(items, nullptr);
             if (rc0) return rc0;
@@ -1654,8 +1665,9 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             (void)hipStreamSynchronize(c->stream);
             if (rc) return rc;
             m3[0] = M.st.count_kernel_ms - M.st.l2_count_kernel_ms;
-            m3[1] = M.st.write_kernel_ms - M.st.l2_write_kernel_ms;
-            m3[2] = M.st.count_kernel_ms + M.st.write_kernel_ms;
+            // a write variant also decides the offsets scan it needs
+            m3[1] = M.st.write_kernel_ms - M.st.l2_write_kernel_ms + M.st.scan_kernel_ms;
+            m3[2] = M.st.count_kernel_ms + M.st.write_kernel_ms + M.st.scan_kernel_ms;
             if (c->verbose)
                 fprintf(stderr, "[colo tune] run: write var %u, %u workgroups per CU: count %.1f ms, write %.1f ms\n",
                         t.write_var, t.blocks_per_cu, M.st.count_kernel_ms, M.st.write_kernel_ms);
@@ -1693,7 +1705,7 @@ int colo_autotune(colo_ctx *c, const colo_mine_opts *opts, const char *cache_pat
             std::vector<uint32_t> values;
             int metric;
         };
-        const std::vector<Axis> axes = {{&TuneParams::write_var, {0, 1, 2}, 1},
+        const std::vector<Axis> axes = {{&TuneParams::write_var, {0, 1, 2, 3}, 1},
                                         {&TuneParams::blocks_per_cu, {4, 8, 16}, 2}};
         for (const Axis &ax : axes) {
             const uint32_t start = best.*(ax.field);

@@ -257,6 +257,7 @@ typedef struct colo_mine_stats {
     uint64_t recounts;      /* candidates recounted for their table's slots */
     uint64_t pruned;        /* candidates skipped by the subset bound        */
     double bound_ms;        /* subset bounds: wall time                      */
+    double scan_kernel_ms;  /* table row-offset scans (batched schedules)    */
 } colo_mine_stats;
 
 /* HIP graphs: the binning (colo_bin, colo_ctx_create) is captured once per
@@ -276,8 +277,10 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         row; 1 = wave-cooperative over 64-row chunks (a
  *                         row group's neighbour list spread over the
  *                         lanes); 2 = one parent row per thread, the work
- *                         item's candidates one after another; 3 (count
- *                         only) = two adjacent rows per thread, their list
+ *                         item's candidates one after another; 3 (write)
+ *                         = 2 with one row offset kept per 64 rows (the
+ *                         wave rebuilds its rows' offsets from the slots);
+ *                         3 (count) = two adjacent rows per thread, their list
  *                         read once when they share the anchor; 6 (count
  *                         only) = four adjacent rows per thread, one list
  *                         pass when all four share the anchor; 8 (count

@@ -155,7 +155,7 @@ def test_deep_levels_every_variant(datagen, colo_mod, oracle_mod):
     old = ctx.tune
     try:
         for var in range(14):
-            ctx.tune = dict(count_var=var, write_var=var % 3, item_rows=256 if var % 2 else 1024)
+            ctx.tune = dict(count_var=var, write_var=var % 4, item_rows=256 if var % 2 else 1024)
             for sched in (colo_mod.COLO_SCHED_PERSISTENT, colo_mod.COLO_SCHED_PLAIN if var == 0 else None):
                 if sched is None:
                     continue

@@ -285,6 +285,8 @@ def test_threshold_sweep(c2small, colo_mod, sched):
                                   dict(count_var=1, write_var=0, blocks_per_cu=16, item_rows=4096),
                                   dict(count_var=2, write_var=2), dict(count_var=3, write_var=1),
                                   dict(write_var=2, write_rows_k=[0xFFFFFFFF] * 2 + [16384, 64, 4096] + [0xFFFFFFFF] * 11),
+                                  dict(write_var=3), dict(count_var=11, write_var=3, item_rows=100),
+                                  dict(write_var=3, write_rows_k=[0xFFFFFFFF] * 2 + [1000, 64, 70000 // 2] + [0xFFFFFFFF] * 11),
                                   dict(count_var=0, count_var_k=[0xFFFFFFFF] * 3 + [3, 1, 2] + [0xFFFFFFFF] * 10),
                                   dict(count_var=4, count_var_k=[0xFFFFFFFF] * 3 + [5, 4, 5] + [0xFFFFFFFF] * 10),
                                   dict(item_rows=1024, item_rows_k=[0xFFFFFFFF] * 2 + [64, 256, 16384] + [0xFFFFFFFF] * 11),

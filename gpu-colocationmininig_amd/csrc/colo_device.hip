@@ -2720,6 +2720,28 @@ int colo::rank_to_sorted(colo_ctx *c, const uint32_t *d_src, uint32_t f, uint32_
     return COLO_OK;
 }
 
+// The miner's PI segments in chunks of kBoundWords words: popcounts summed
+// with one atomic per workgroup; the chunk holding a boundary bit reports it.
+__global__ __launch_bounds__(kBlock) void k_pi_chunks(const uint32_t *bits, const SegDesc *segs,
+                                                      const uint64_t *chunk_prefix, uint64_t n, SegOut *out) {
+    const uint64_t b = blockIdx.x;
+    const uint64_t t = chunk_owner(chunk_prefix, n, b);
+    const SegDesc sd = segs[t];
+    const uint64_t w0 = (b - chunk_prefix[t]) * kBoundWords;
+    const uint64_t w1 = w0 + kBoundWords < sd.words ? w0 + kBoundWords : sd.words;
+    const uint32_t *w = bits + sd.off;
+    uint64_t sum = 0;
+    for (uint64_t i = w0 + threadIdx.x; i < w1; i += kBlock) sum += __popc(w[i]);
+    uint64_t ex;
+    const uint64_t tot = block_excl_scan(sum, ex);
+    if (threadIdx.x == 0) {
+        if (tot) atomicAdd((unsigned long long *)&out[t].pop, (unsigned long long)tot);
+        const uint64_t fw = sd.first_bit >> 5, lw = sd.last_bit >> 5;
+        if (fw >= w0 && fw < w1) out[t].first = (w[fw] >> (sd.first_bit & 31)) & 1u;
+        if (lw >= w0 && lw < w1) out[t].last = (w[lw] >> (sd.last_bit & 31)) & 1u;
+    }
+}
+
 void colo::seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off, uint64_t *seg_off,
                       uint64_t *words_total) {
     uint64_t o = bits_off;
@@ -3400,10 +3422,19 @@ int colo_participation(colo_ctx *c, const uint32_t *d_bits, uint32_t k, const ui
 
 // batched PI over absolute segment offsets (the miner's arena)
 namespace colo {
-int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
-                 hipStream_t s) {
+int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *segs, const SegDesc *d_segs, uint64_t n_segs,
+                 std::vector<uint64_t> &prefix, uint64_t *d_prefix, SegOut *d_out, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
     if (!n_segs) return COLO_OK;
-    hipLaunchKernelGGL(k_pi_abs, dim3((unsigned)n_segs), dim3(kBlock), 0, s, d_bits, d_segs, d_out);
+    const uint64_t chunks = chunk_prefix_of(segs, n_segs, [](const SegDesc &d) { return d.words; }, prefix);
+    if (chunks >= (1ull << 31)) return set_error(c, COLO_EUNSUPPORTED, "pi: too many chunks");
+    static_assert(sizeof(SegOut) % 4 == 0, "SegOut is whole words");
+    int rc = fill_u32((uint32_t *)d_out, n_segs * (sizeof(SegOut) / 4), 0, s);
+    if (rc) return rc;
+    COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), n_segs * 8, hipMemcpyHostToDevice, s));
+    if (chunks)
+        hipLaunchKernelGGL(k_pi_chunks, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_bits, d_segs, d_prefix,
+                           n_segs, d_out);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "pi launch");
 }
 int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s) {

@@ -297,8 +297,12 @@ int count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem 
                 hipStream_t s);
 int write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                 uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
-int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, uint64_t n_segs, SegOut *d_out,
-                 hipStream_t s);
+// PI segments of a level in one launch of fixed chunks of words (a hot
+// feature's segment spreads over many workgroups).  segs: host copy of
+// d_segs; prefix (host, kept by the caller until the stream has run) and
+// d_prefix (n_segs entries) receive each segment's first chunk.
+int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *segs, const SegDesc *d_segs, uint64_t n_segs,
+                 std::vector<uint64_t> &prefix, uint64_t *d_prefix, SegOut *d_out, hipStream_t s);
 // Row-order independent checksum of a rows x k table; d_map (optional) maps
 // stored sorted positions to instance ids.
 int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, const uint32_t *d_map, uint64_t *h,

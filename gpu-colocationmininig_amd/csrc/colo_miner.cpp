@@ -164,7 +164,8 @@ struct Miner {
     SweepMemo *memo = nullptr;
 
     Miner(colo_ctx *c_, const colo_mine_opts &o_) : c(c_), o(o_), s(c_->stream), ctx_for_err(c_) {
-        for (DevBuf *b : {&arena_slots, &arena_bits, &d_desc, &d_items, &d_segs, &d_segout, &d_queue, &d_idx, &d_rowsel})
+        for (DevBuf *b : {&arena_slots, &arena_bits, &d_desc, &d_items, &d_segs, &d_segout, &d_queue, &d_idx, &d_rowsel,
+                          &d_pipre})
             b->pool = &c->pool;
     }
     ~Miner() {
@@ -428,7 +429,7 @@ struct Miner {
 
     // PERSISTENT: one count launch, one PI launch, one scan for the level.
     std::shared_ptr<DevBuf> arena_offs;
-    DevBuf arena_slots, arena_bits, d_desc, d_items, d_segs, d_segout, d_queue, d_idx, d_rowsel;
+    DevBuf arena_slots, arena_bits, d_desc, d_items, d_segs, d_segout, d_queue, d_idx, d_rowsel, d_pipre;
 
     // Write items of chunk offsets start at multiples of 64 rows.
     static uint64_t write_chunk(uint64_t rows, bool chunked) { return chunked ? (rows + 63) / 64 * 64 : rows; }
@@ -690,8 +691,11 @@ struct Miner {
         if (!ensure_buf(d_segs, segs.size() * sizeof(SegDesc)) || !ensure_buf(d_segout, segs.size() * sizeof(SegOut)))
             return set_error(c, COLO_ENOMEM, "segments");
         COLO_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
+        if (!ensure_buf(d_pipre, segs.size() * 8)) return set_error(c, COLO_ENOMEM, "segments");
+        std::vector<uint64_t> pi_prefix;  // read by the upload before the sync below
         size_t e1 = mark();
-        rc = pi_batch_abs(c, arena_bits.as<uint32_t>(), d_segs.as<SegDesc>(), segs.size(), d_segout.as<SegOut>(), s);
+        rc = pi_batch_abs(c, arena_bits.as<uint32_t>(), segs.data(), d_segs.as<SegDesc>(), segs.size(), pi_prefix,
+                          d_pipre.as<uint64_t>(), d_segout.as<SegOut>(), s);
         if (rc) return rc;
         span(e1, mark(), kPi);
         std::vector<SegOut> so(segs.size());

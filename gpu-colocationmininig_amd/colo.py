@@ -75,7 +75,7 @@ class MineStats(C.Structure):
                [(n, C.c_double) for n in ("model_ms", "alloc_ms", "alloc_bytes")] + [("level_ms", C.c_double * 16)] + \
                [("level_count_kernel_ms", C.c_double * 16), ("pool_peak_bytes", C.c_double),
                 ("slots_deferred", C.c_uint64), ("recounts", C.c_uint64), ("pruned", C.c_uint64),
-                ("bound_ms", C.c_double), ("scan_kernel_ms", C.c_double)]
+                ("bound_ms", C.c_double), ("scan_kernel_ms", C.c_double), ("graphs", C.c_uint64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -277,7 +277,8 @@ class Context:
         self._check(lib().colo_set_verbose(self.h, int(level)))
 
     def set_graphs(self, on):
-        """Binning replayed from a captured HIP graph (on) or launched eagerly."""
+        """Binning replayed from a captured HIP graph and each level's passes
+        launched as captured graphs (on), or everything launched eagerly."""
         self._check(lib().colo_set_graphs(self.h, int(bool(on))))
 
     def graph_info(self):

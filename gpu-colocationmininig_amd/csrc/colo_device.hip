@@ -3122,6 +3122,7 @@ int colo_set_verbose(colo_ctx *c, int level) {
 int colo_set_graphs(colo_ctx *c, int on) {
     if (!c) return COLO_EARG;
     c->use_graphs = on != 0;
+    if (on) c->level_graphs_off = false;  // try level captures again
     return COLO_OK;
 }
 
@@ -3422,23 +3423,25 @@ int colo_participation(colo_ctx *c, const uint32_t *d_bits, uint32_t k, const ui
 
 // batched PI over absolute segment offsets (the miner's arena)
 namespace colo {
-int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *segs, const SegDesc *d_segs, uint64_t n_segs,
-                 std::vector<uint64_t> &prefix, uint64_t *d_prefix, SegOut *d_out, hipStream_t s) {
-    colo_ctx *ctx_for_err = c;
+uint64_t pi_chunk_prefix(const SegDesc *segs, uint64_t n_segs, std::vector<uint64_t> &prefix) {
+    return chunk_prefix_of(segs, n_segs, [](const SegDesc &d) { return d.words; }, prefix);
+}
+int pi_batch_launch(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, const uint64_t *d_prefix,
+                    uint64_t n_segs, uint64_t chunks, SegOut *d_out, hipStream_t s) {
     if (!n_segs) return COLO_OK;
-    const uint64_t chunks = chunk_prefix_of(segs, n_segs, [](const SegDesc &d) { return d.words; }, prefix);
     if (chunks >= (1ull << 31)) return set_error(c, COLO_EUNSUPPORTED, "pi: too many chunks");
     static_assert(sizeof(SegOut) % 4 == 0, "SegOut is whole words");
     int rc = fill_u32((uint32_t *)d_out, n_segs * (sizeof(SegOut) / 4), 0, s);
     if (rc) return rc;
-    COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), n_segs * 8, hipMemcpyHostToDevice, s));
     if (chunks)
         hipLaunchKernelGGL(k_pi_chunks, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_bits, d_segs, d_prefix,
                            n_segs, d_out);
     return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "pi launch");
 }
-int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s) {
+int scan_prepare(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, ScanPlan &plan, hipStream_t s) {
     colo_ctx *ctx_for_err = c;
+    plan = ScanPlan{};
+    plan.nseg = nseg;
     if (!nseg) return COLO_OK;
     std::vector<ScanSegDev> sd(nseg);
     std::vector<ScanTile> tiles;
@@ -3446,33 +3449,49 @@ int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s
         sd[i] = ScanSegDev{segs[i].in, segs[i].out, segs[i].n, tiles.size(), 0, segs[i].chunked ? 1u : 0u, 0};
         for (uint64_t b = 0; b < segs[i].n; b += kScanTile) tiles.push_back(ScanTile{(uint32_t)i, 0, b});
         sd[i].tile1 = tiles.size();
+        (segs[i].chunked ? plan.any_chunks : plan.any_rows) = true;
     }
     const uint64_t nt = tiles.size();
-    size_t bytes = nseg * sizeof(ScanSegDev) + nt * sizeof(ScanTile) + 8 * nt + 64;
+    plan.nt = nt;
+    const size_t sd_b = nseg * sizeof(ScanSegDev), t_b = nt * sizeof(ScanTile);
     void *tmp;
-    int rc = scratch(c, bytes, &tmp);
+    int rc = scratch(c, sd_b + t_b + 8 * nt + 64, &tmp);
     if (rc) return rc;
     char *q = (char *)tmp;
-    ScanSegDev *d_sd = (ScanSegDev *)q;
-    q += nseg * sizeof(ScanSegDev);
-    ScanTile *d_tiles = (ScanTile *)q;
-    q += nt * sizeof(ScanTile);
-    uint64_t *d_part = (uint64_t *)(((uintptr_t)q + 7) & ~(uintptr_t)7);
-    COLO_HIP(hipMemcpyAsync(d_sd, sd.data(), nseg * sizeof(ScanSegDev), hipMemcpyHostToDevice, s));
-    if (nt) {
-        COLO_HIP(hipMemcpyAsync(d_tiles, tiles.data(), nt * sizeof(ScanTile), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_segscan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, d_part);
-    }
-    hipLaunchKernelGGL(k_segscan_carry, dim3((unsigned)nseg), dim3(kBlock), 0, s, d_sd, d_part);
-    bool any_rows = false, any_chunks = false;
-    for (uint64_t i = 0; i < nseg; i++) (segs[i].chunked ? any_chunks : any_rows) = true;
+    plan.d_sd = q;
+    q += sd_b;
+    plan.d_tiles = q;
+    q += t_b;
+    plan.d_part = (uint64_t *)(((uintptr_t)q + 7) & ~(uintptr_t)7);
+    // one host block holding both arrays, kept with the plan until the
+    // stream has consumed the upload
+    plan.host.resize(sd_b + t_b);
+    memcpy(plan.host.data(), sd.data(), sd_b);
+    if (t_b) memcpy(plan.host.data() + sd_b, tiles.data(), t_b);
+    COLO_HIP(hipMemcpyAsync(plan.d_sd, plan.host.data(), sd_b + t_b, hipMemcpyHostToDevice, s));
+    return COLO_OK;
+}
+int scan_launch(colo_ctx *c, const ScanPlan &plan, hipStream_t s) {
+    if (!plan.nseg) return COLO_OK;
+    const ScanSegDev *d_sd = (const ScanSegDev *)plan.d_sd;
+    const ScanTile *d_tiles = (const ScanTile *)plan.d_tiles;
+    const uint64_t nt = plan.nt;
+    if (nt) hipLaunchKernelGGL(k_segscan_reduce, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, plan.d_part);
+    hipLaunchKernelGGL(k_segscan_carry, dim3((unsigned)plan.nseg), dim3(kBlock), 0, s, d_sd, plan.d_part);
     // per-row segments: tile scans; chunked ones: prefixes of the chunk sums
-    if (nt && any_rows)
-        hipLaunchKernelGGL(k_segscan_apply, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, d_part);
-    if (nt && any_chunks)
+    if (nt && plan.any_rows)
+        hipLaunchKernelGGL(k_segscan_apply, dim3((unsigned)nt), dim3(kBlock), 0, s, d_sd, d_tiles, plan.d_part);
+    if (nt && plan.any_chunks)
         hipLaunchKernelGGL(k_segscan_apply_chunks, dim3((unsigned)((nt + kBlock / 32 - 1) / (kBlock / 32))),
-                           dim3(kBlock), 0, s, d_sd, d_tiles, nt, d_part);
-    COLO_HIP(hipGetLastError());
+                           dim3(kBlock), 0, s, d_sd, d_tiles, nt, plan.d_part);
+    return hipGetLastError() == hipSuccess ? COLO_OK : set_error(c, COLO_EHIP, "scan launch");
+}
+int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s) {
+    colo_ctx *ctx_for_err = c;
+    ScanPlan plan;
+    int rc = scan_prepare(c, segs, nseg, plan, s);
+    if (!rc) rc = scan_launch(c, plan, s);
+    if (rc) return rc;
     // the descriptors live in scratch: finish before the next scratch use
     COLO_HIP(hipStreamSynchronize(s));
     return COLO_OK;

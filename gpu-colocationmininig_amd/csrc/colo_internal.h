@@ -258,6 +258,7 @@ struct colo_ctx {
     hipGraph_t bin_graph = nullptr;
     hipGraphExec_t bin_exec = nullptr;
     bool use_graphs = true;
+    bool level_graphs_off = false;  // set when a level's capture failed: eager from then on
     uint32_t graph_captures = 0, graph_replays = 0;
     std::vector<uint64_t> pairwork; // F x F 3x3 join work per (first, last), filled on first use
     colo::DevPool pool;             // miner buffers (tables, arenas, slots, offsets)
@@ -298,11 +299,13 @@ int count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem 
 int write_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const WorkItem *d_items, const uint64_t *d_seg,
                 uint64_t n_items, const uint32_t *d_slots, uint32_t *d_queue, hipStream_t s);
 // PI segments of a level in one launch of fixed chunks of words (a hot
-// feature's segment spreads over many workgroups).  segs: host copy of
-// d_segs; prefix (host, kept by the caller until the stream has run) and
-// d_prefix (n_segs entries) receive each segment's first chunk.
-int pi_batch_abs(colo_ctx *c, const uint32_t *d_bits, const SegDesc *segs, const SegDesc *d_segs, uint64_t n_segs,
-                 std::vector<uint64_t> &prefix, uint64_t *d_prefix, SegOut *d_out, hipStream_t s);
+// feature's segment spreads over many workgroups).  pi_chunk_prefix (host):
+// each segment's first chunk; returns the number of chunks.  pi_batch_launch
+// clears d_out and launches (launches only: capturable in a graph); d_prefix
+// holds the prefix.
+uint64_t pi_chunk_prefix(const SegDesc *segs, uint64_t n_segs, std::vector<uint64_t> &prefix);
+int pi_batch_launch(colo_ctx *c, const uint32_t *d_bits, const SegDesc *d_segs, const uint64_t *d_prefix,
+                    uint64_t n_segs, uint64_t chunks, SegOut *d_out, hipStream_t s);
 // Row-order independent checksum of a rows x k table; d_map (optional) maps
 // stored sorted positions to instance ids.
 int table_hash(colo_ctx *c, const uint32_t *d_t, uint64_t rows, uint32_t k, const uint32_t *d_map, uint64_t *h,
@@ -318,6 +321,19 @@ struct ScanSeg {
     bool chunked = false;
 };
 int scan_segments(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, hipStream_t s);
+// The same in two steps: scan_prepare builds the tiles and uploads the
+// descriptors into the context's scratch (plan keeps the host copies);
+// scan_launch launches the kernels (launches only: capturable in a graph).
+// Nothing else may use the scratch until the launches have run.
+struct ScanPlan {
+    std::vector<unsigned char> host;  // descriptors and tiles as uploaded
+    void *d_sd = nullptr, *d_tiles = nullptr;
+    uint64_t *d_part = nullptr;
+    uint64_t nseg = 0, nt = 0;
+    bool any_rows = false, any_chunks = false;
+};
+int scan_prepare(colo_ctx *c, const ScanSeg *segs, uint64_t nseg, ScanPlan &plan, hipStream_t s);
+int scan_launch(colo_ctx *c, const ScanPlan &plan, hipStream_t s);
 
 // Segment descriptors of one candidate's bitmap (bits_off = base word).
 void seg_layout(const colo_ctx *c, uint32_t k, const uint32_t *cand, uint64_t bits_off,

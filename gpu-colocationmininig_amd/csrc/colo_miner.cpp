@@ -169,6 +169,8 @@ struct Miner {
             b->pool = &c->pool;
     }
     ~Miner() {
+        (void)hipStreamSynchronize(s);
+        for (auto &x : execs_) (void)hipGraphExecDestroy(x);
         for (auto &e : pool_) (void)hipEventDestroy(e);
     }
 
@@ -195,7 +197,10 @@ struct Miner {
     void harvest() {
         for (const Span &sp : spans_) {
             float ms = 0;
-            if (hipEventElapsedTime(&ms, pool_[sp.a], pool_[sp.b]) != hipSuccess) continue;
+            if (hipEventElapsedTime(&ms, pool_[sp.a], pool_[sp.b]) != hipSuccess) {
+                (void)hipGetLastError();  // an untimed span must not fail a later check
+                continue;
+            }
             if ((sp.cat == kCount || sp.cat == kL2Count) && sp.k < 16) st.level_count_kernel_ms[sp.k] += ms;
             switch (sp.cat) {
                 case kCount: st.count_kernel_ms += ms; break;
@@ -248,8 +253,43 @@ struct Miner {
 
     int sync() {
         COLO_HIP(hipStreamSynchronize(s));
+        for (auto &x : execs_) (void)hipGraphExecDestroy(x);
+        execs_.clear();
         harvest();
         return COLO_OK;
+    }
+
+    // Each pass of a level (count, PI popcounts, offsets scan, table write)
+    // is enqueued as a captured HIP graph when the context's graphs are on.
+    // `launches` may only launch kernels (no host waits, allocations, host
+    // copies or timing marks: the uploads come before, the marks around it;
+    // an event recorded inside a capture cannot be timed by the host here).
+    // Tuning runs stay eager; a stream that cannot be captured runs the
+    // launches eagerly from then on.  Executables are destroyed at the next
+    // synchronisation.
+    std::vector<hipGraphExec_t> execs_;
+    int as_graph(const std::function<int()> &launches) {
+        if (!c->use_graphs || c->level_graphs_off || tuning_) return launches();
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            c->level_graphs_off = true;
+            return launches();
+        }
+        const int rc = launches();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        hipGraphExec_t x = nullptr;
+        if (rc == COLO_OK && e == hipSuccess && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
+            (void)hipGraphDestroy(g);
+            execs_.push_back(x);
+            COLO_HIP(hipGraphLaunch(x, s));
+            st.graphs++;
+            return COLO_OK;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        c->level_graphs_off = true;  // capture unavailable here: eager from now on
+        return launches();
     }
 
     void write_patterns(uint32_t k, const std::vector<Prev> &L) {
@@ -636,15 +676,35 @@ struct Miner {
             if ((rc0 = upload_items(items, pred ? &iw : nullptr))) return rc0;
         }
         auto t0 = Clock::now();
+        // PI segments of every candidate, uploaded before the level's graph
+        std::vector<SegDesc> segs(m * k);
+        for (uint64_t u = 0; u < m; u++) seg_descs(c, k, &C[idx[u] * k], desc[u].bits_off, &segs[u * k], true);
+        if (!ensure_buf(d_segs, segs.size() * sizeof(SegDesc)) || !ensure_buf(d_segout, segs.size() * sizeof(SegOut)) ||
+            !ensure_buf(d_pipre, segs.size() * 8))
+            return set_error(c, COLO_ENOMEM, "segments");
+        std::vector<uint64_t> pi_prefix;  // read by the upload before the sync below
+        const uint64_t pi_chunks = pi_chunk_prefix(segs.data(), segs.size(), pi_prefix);
+        COLO_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
+        COLO_HIP(hipMemcpyAsync(d_pipre.p, pi_prefix.data(), segs.size() * 8, hipMemcpyHostToDevice, s));
+        // count pass, then the PI popcounts
         int rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
         if (!rc) rc = fill_u32(d_rowsel.as<uint32_t>(), 2 * m, 0, s);
         if (rc) return rc;
         size_t e0 = mark();
-        rc = count_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
-                         slots_arena->as<uint32_t>(), arena_bits.as<uint32_t>(), d_rowsel.as<uint64_t>(),
-                         d_queue.as<uint32_t>(), s);
+        rc = as_graph([&]() {
+            return count_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(),
+                               slots_arena->as<uint32_t>(), arena_bits.as<uint32_t>(), d_rowsel.as<uint64_t>(),
+                               d_queue.as<uint32_t>(), s);
+        });
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Count : kCount, k);
+        size_t e1 = mark();
+        rc = as_graph([&]() {
+            return pi_batch_launch(c, arena_bits.as<uint32_t>(), d_segs.as<SegDesc>(), d_pipre.as<uint64_t>(),
+                                   segs.size(), pi_chunks, d_segout.as<SegOut>(), s);
+        });
+        if (rc) return rc;
+        span(e1, mark(), kPi);
         if (pred) {
             // pre-size the level's table arena from the predicted rows while
             // the count pass runs (the exact sizes are known only after it).
@@ -685,19 +745,6 @@ struct Miner {
         // later, only for the tables that are written)
         std::vector<uint64_t> rows_of(m, 0);
         COLO_HIP(hipMemcpyAsync(rows_of.data(), d_rowsel.p, m * 8, hipMemcpyDeviceToHost, s));
-        // PI segments of every candidate, one launch
-        std::vector<SegDesc> segs(m * k);
-        for (uint64_t u = 0; u < m; u++) seg_descs(c, k, &C[idx[u] * k], desc[u].bits_off, &segs[u * k], true);
-        if (!ensure_buf(d_segs, segs.size() * sizeof(SegDesc)) || !ensure_buf(d_segout, segs.size() * sizeof(SegOut)))
-            return set_error(c, COLO_ENOMEM, "segments");
-        COLO_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
-        if (!ensure_buf(d_pipre, segs.size() * 8)) return set_error(c, COLO_ENOMEM, "segments");
-        std::vector<uint64_t> pi_prefix;  // read by the upload before the sync below
-        size_t e1 = mark();
-        rc = pi_batch_abs(c, arena_bits.as<uint32_t>(), segs.data(), d_segs.as<SegDesc>(), segs.size(), pi_prefix,
-                          d_pipre.as<uint64_t>(), d_segout.as<SegOut>(), s);
-        if (rc) return rc;
-        span(e1, mark(), kPi);
         std::vector<SegOut> so(segs.size());
         COLO_HIP(hipMemcpyAsync(so.data(), d_segout.p, so.size() * sizeof(SegOut), hipMemcpyDeviceToHost, s));
         if ((rc = sync())) return rc;
@@ -854,6 +901,7 @@ struct Miner {
         if (!ensure_buf(*arena_offs, std::max<uint64_t>(owords, 1) * 8))
             return oom("offsets arena", std::max<uint64_t>(owords, 1) * 8);
         std::vector<uint64_t> offs_at(L.size(), 0);
+        ScanPlan plan;  // the scan runs with the write, in the level's second graph
         {
             std::vector<ScanSeg> segs;
             uint64_t oat = 0;
@@ -864,11 +912,17 @@ struct Miner {
                                        L[i].nslots, chunked});
                 oat += n_offs(L[i].nslots);
             }
-            size_t e0 = mark();
-            int rc0 = scan_segments(c, segs.data(), segs.size(), s);
+            int rc0 = scan_prepare(c, segs.data(), segs.size(), plan, s);
             if (rc0) return rc0;
-            span(e0, mark(), kScan, k);
         }
+        // the offsets scan (before the tuning's timed writes, which need it)
+        auto scan = [&]() -> int {
+            size_t e0 = mark();
+            int r = as_graph([&]() { return scan_launch(c, plan, s); });
+            if (r) return r;
+            span(e0, mark(), kScan, k);
+            return COLO_OK;
+        };
         std::shared_ptr<DevBuf> arena;
         if (arena_reserved_ && arena_reserved_->bytes >= words * 4) {
             arena = arena_reserved_;      // pre-sized before the count pass
@@ -924,6 +978,7 @@ struct Miner {
         if (tuning_ && k < 16) {
             // the level's item size of the write: each size writes the
             // whole tables (the same rows every time), the fastest is kept
+            if (int r = scan()) return r;
             static const uint32_t sizes[] = {1024, 4096, 16384};
             hipEvent_t ev[2];
             COLO_HIP(hipEventCreate(&ev[0]));
@@ -967,9 +1022,14 @@ struct Miner {
             int rc0 = upload_items(items, nullptr);
             if (rc0) return rc0;
         }
+        // offsets scan (already run when tuning), then the table write
+        int rc = COLO_OK;
+        if (!(tuning_ && k < 16) && (rc = scan())) return rc;
         size_t e0 = mark();
-        int rc = write_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(), nullptr,
-                             d_queue.as<uint32_t>(), s);
+        rc = as_graph([&]() {
+            return write_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, items.size(), nullptr,
+                               d_queue.as<uint32_t>(), s);
+        });
         if (rc) return rc;
         span(e0, mark(), k == 2 ? kL2Write : kWrite);
         {

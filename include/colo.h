@@ -258,13 +258,17 @@ typedef struct colo_mine_stats {
     uint64_t pruned;        /* candidates skipped by the subset bound        */
     double bound_ms;        /* subset bounds: wall time                      */
     double scan_kernel_ms;  /* table row-offset scans (batched schedules)    */
+    uint64_t graphs;        /* level passes launched as captured HIP graphs  */
 } colo_mine_stats;
 
 /* HIP graphs: the binning (colo_bin, colo_ctx_create) is captured once per
  * grid (distance r) as a graph and replayed by later colo_bin calls with the
- * same r; on = 0 runs it eagerly.  Capture falls back to eager launches where
- * the stream cannot be captured.  colo_graph_info reports the captures and
- * replays so far.  No counterpart in the reference. */
+ * same r.  Each pass of a level of the batched schedules (count, PI
+ * popcounts, offsets scan, table write) is launched as a captured graph
+ * (colo_mine_stats.graphs counts them).  on = 0 runs everything
+ * eagerly.  Capture falls back to eager launches where the stream cannot be
+ * captured.  colo_graph_info reports the binning's captures and replays so
+ * far.  No counterpart in the reference. */
 int colo_set_graphs(colo_ctx *ctx, int on);
 int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
 

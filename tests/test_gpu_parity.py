@@ -593,3 +593,32 @@ def test_subset_bound_c3_small(datagen, colo_mod, oracle_mod):
                        subset_bound=True)
     assert rep == orc.mine(p["r"], p["theta"], max_k=p["max_k"], subset_bound=True)
     assert st["pruned"] > 500
+
+
+@pytest.mark.parametrize("sched", ["persistent", "learned"])
+def test_level_graphs(datagen, colo_mod, oracle_mod, sched):
+    """Each level's count, PI, scan and write passes run as captured HIP graphs:
+    three rounds of mining with graphs, each followed by eager mining on the
+    same context, all equal to the oracle's report (with the subset bound
+    and table checksums); graphs off launches none."""
+    _say(f"level graphs sched={sched}")
+    x, y, fc, b, p = _workload(datagen, "c1", **DENSE)
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(p["r"])
+    expect = orc.mine(p["r"], p["theta"], max_k=4, subset_bound=True)
+    ctx = colo_mod.Context(x, y, fc, b, p["r"])
+    sc = colo_mod.COLO_SCHED_PERSISTENT if sched == "persistent" else colo_mod.COLO_SCHED_LEARNED
+    model = None
+    if sc == colo_mod.COLO_SCHED_LEARNED:
+        model = colo_mod.CostModel.load(os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin"))
+    for i in range(3):
+        ctx.set_graphs(True)
+        rep, st = ctx.mine(p["theta"], max_k=4, sched=sc, model=model, subset_bound=True)
+        assert rep == expect, i
+        assert st["graphs"] >= 4 * st["levels"] - 2, st["graphs"]
+        # the timing marks inside the graphs are recorded
+        assert st["count_kernel_ms"] > 0 and st["write_kernel_ms"] > 0 and st["pi_kernel_ms"] > 0
+        ctx.set_graphs(False)
+        rep, st = ctx.mine(p["theta"], max_k=4, sched=sc, model=model, subset_bound=True)
+        assert rep == expect, i
+        assert st["graphs"] == 0

@@ -1233,18 +1233,27 @@ struct Miner {
         idx.clear();
         for (uint64_t i = 0; i < nC; i++)
             if (pass[i]) idx.push_back(i);
-        // 3x3 join work of every (first, last) feature pair, computed once
-        // per binning in one launch
+        // 3x3 join work of every feature pair, computed once per binning in
+        // one launch.  The work is symmetric (the neighbour-cell relation is),
+        // so each pair is summed once over the cells of its feature with
+        // fewer non-empty cells.
         if (c->pairwork.empty() && c->F) {
             std::vector<uint32_t> pairs;
+            auto runs = [&](uint32_t f) { return c->run_off[f + 1] - c->run_off[f]; };
             for (uint32_t a = 0; a < c->F; a++)
-                for (uint32_t b = 0; b < c->F; b++) {
-                    pairs.push_back(a);
-                    pairs.push_back(b);
+                for (uint32_t b = a + 1; b < c->F; b++) {
+                    const bool swap = runs(b) < runs(a);
+                    pairs.push_back(swap ? b : a);
+                    pairs.push_back(swap ? a : b);
                 }
-            std::vector<uint64_t> pw((size_t)c->F * c->F, 0);
-            int rc = pairwork_batch(c, pairs.data(), (uint32_t)(pairs.size() / 2), pw.data(), s);
+            std::vector<uint64_t> out(pairs.size() / 2, 0);
+            int rc = pairwork_batch(c, pairs.data(), (uint32_t)out.size(), out.data(), s);
             if (rc) return rc;
+            std::vector<uint64_t> pw((size_t)c->F * c->F, 0);
+            for (size_t i = 0; i < out.size(); i++) {
+                const uint32_t a = pairs[2 * i], b = pairs[2 * i + 1];
+                pw[(size_t)a * c->F + b] = pw[(size_t)b * c->F + a] = out[i];
+            }
             c->pairwork = std::move(pw);
         }
         std::vector<uint64_t> pw(idx.size(), 0);

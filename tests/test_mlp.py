@@ -150,7 +150,9 @@ def test_logged_targets_match_oracle(colo_mod, datagen, oracle_mod, tmp_path):
     times N_last in the cell's 3x3 neighbourhood under the reference's
     neighbour-cell rule (orc_nbr_cell); at size k, the sum over the parent
     table's rows of the first instance's degree in the (first, last) pair
-    table, both tables written by the oracle."""
+    table, both tables written by the oracle.  The pair-work feature (the
+    3x3 join work of (first, last), computed once per unordered pair) is
+    checked the same way."""
     from report_check import OracleChains, parse_report
     print("[case] logged cost-model targets vs oracle", flush=True)
     labels, x, y, S, p = datagen.generate("c2", N=60000, seed=995)
@@ -201,6 +203,9 @@ def test_logged_targets_match_oracle(colo_mod, datagen, oracle_mod, tmp_path):
         assert tests >= rows, (c, tests, rows)
         ref = np.float32([np.log1p(rows), np.log1p(tests)])
         assert _ulps_ok(T[u], ref).all(), (c, T[u], ref, rows, tests)
+        # feature 11: the 3x3 join work of (first, last), the level-2 tests
+        work = int((cnt[:, c[0]] * nb[:, c[-1]]).sum())
+        assert _ulps_ok(X[u][11], np.float32(np.log1p(work))), (c, X[u][11], work)
     print(f"[targets] {n2} pairs and {nk} larger candidates match the oracle", flush=True)
     assert n2 > 10 and nk > 5
 

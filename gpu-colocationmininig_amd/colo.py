@@ -75,7 +75,8 @@ class MineStats(C.Structure):
                [(n, C.c_double) for n in ("model_ms", "alloc_ms", "alloc_bytes")] + [("level_ms", C.c_double * 16)] + \
                [("level_count_kernel_ms", C.c_double * 16), ("pool_peak_bytes", C.c_double),
                 ("slots_deferred", C.c_uint64), ("recounts", C.c_uint64), ("pruned", C.c_uint64),
-                ("bound_ms", C.c_double), ("scan_kernel_ms", C.c_double), ("graphs", C.c_uint64)]
+                ("bound_ms", C.c_double), ("scan_kernel_ms", C.c_double), ("graphs", C.c_uint64),
+                ("tables_skipped", C.c_uint64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

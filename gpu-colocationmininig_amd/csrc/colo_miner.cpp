@@ -1316,6 +1316,67 @@ struct Miner {
         return fclose(fp) == 0 ? COLO_OK : set_error(c, COLO_EFILE, "cannot write the cost-model log");
     }
 
+    // The filter's counts and pass flags of a level's candidates (cpp:297,
+    // 760); in a sweep, counts known from an earlier threshold need only the
+    // threshold test.
+    DevBuf d_cands{&c->pool}, d_counts{&c->pool}, d_pass{&c->pool};
+    int level_filter(uint32_t k, const std::vector<uint32_t> &C, std::vector<uint32_t> &fcnt,
+                     std::vector<uint8_t> &pass) {
+        auto tf = Clock::now();
+        const uint64_t nC = C.size() / k;
+        fcnt.assign(nC * k, 0);
+        pass.assign(nC, 1);
+        if (o.filter_on && nC) {
+            if (!d_cands.ensure(C.size() * 4) || !d_counts.ensure(C.size() * 4) || !d_pass.ensure(nC))
+                return set_error(c, COLO_ENOMEM, "filter buffers");
+            COLO_HIP(hipMemcpyAsync(d_cands.p, C.data(), C.size() * 4, hipMemcpyHostToDevice, s));
+            bool cached = memo != nullptr;
+            if (memo)
+                for (uint64_t i = 0; i < nC && cached; i++) {
+                    auto it = memo->fcnt.find(std::vector<uint32_t>(&C[i * k], &C[i * k] + k));
+                    if (it == memo->fcnt.end()) cached = false;
+                    else memcpy(&fcnt[i * k], it->second.data(), k * 4);
+                }
+            size_t e0 = mark();
+            int rc;
+            if (cached) {
+                COLO_HIP(hipMemcpyAsync(d_counts.p, fcnt.data(), fcnt.size() * 4, hipMemcpyHostToDevice, s));
+                rc = threshold_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
+                                     d_pass.as<uint8_t>(), s);
+            } else {
+                rc = filter_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
+                                  d_pass.as<uint8_t>(), s);
+            }
+            if (rc) return rc;
+            span(e0, mark(), kFilter);
+            if (!cached) COLO_HIP(hipMemcpyAsync(fcnt.data(), d_counts.p, fcnt.size() * 4, hipMemcpyDeviceToHost, s));
+            COLO_HIP(hipMemcpyAsync(pass.data(), d_pass.p, nC, hipMemcpyDeviceToHost, s));
+            if ((rc = sync())) return rc;
+            st.launches += cached ? 1 : 2;
+            if (memo && !cached)
+                for (uint64_t i = 0; i < nC; i++)
+                    memo->fcnt[std::vector<uint32_t>(&C[i * k], &C[i * k] + k)] =
+                        std::vector<uint32_t>(&fcnt[i * k], &fcnt[i * k] + k);
+        }
+        st.filter_ms += ms_since(tf);
+        return COLO_OK;
+    }
+    // Subset bounds of a level's candidates that pass the filter (k >= 3).
+    int level_bounds(uint32_t k, const std::vector<uint32_t> &C, const std::vector<uint8_t> &pass,
+                     const std::vector<Prev> &Lprev, std::vector<float> &sbq) {
+        auto tb = Clock::now();
+        int rc = subset_bounds(k, C, pass, Lprev, sbq);
+        st.bound_ms += ms_since(tb);
+        return rc;
+    }
+    // The next level's filter results and bounds, computed before this
+    // level's tables are built (only the tables that surviving candidates
+    // extend are built when no table checksums are asked for).
+    bool pre_valid_ = false;
+    std::vector<uint32_t> pre_fcnt_;
+    std::vector<uint8_t> pre_pass_;
+    std::vector<float> pre_sbq_;
+
     int run(char **report) {
         auto T0 = Clock::now();
         const double sys_ms0 = c->pool.sys_ms, sys_bytes0 = c->pool.sys_bytes;
@@ -1334,52 +1395,27 @@ struct Miner {
         uint32_t k = 2;
         uint64_t total_prev = 0;
         uint32_t levels = 0;
-        DevBuf d_cands(&c->pool), d_counts(&c->pool), d_pass(&c->pool);
         while (!C.empty() && k <= F && (o.max_k == 0 || k <= o.max_k)) {
             levels++;
             cur_k_ = k;
             auto TL = Clock::now();
             const uint64_t nC = C.size() / k;
             st.candidates += nC;
-            // filter (cpp:297, 760)
-            auto tf = Clock::now();
-            std::vector<uint32_t> fcnt(nC * k, 0);
-            std::vector<uint8_t> pass(nC, 1);
-            if (o.filter_on) {
-                if (!d_cands.ensure(C.size() * 4) || !d_counts.ensure(C.size() * 4) || !d_pass.ensure(nC))
-                    return set_error(c, COLO_ENOMEM, "filter buffers");
-                COLO_HIP(hipMemcpyAsync(d_cands.p, C.data(), C.size() * 4, hipMemcpyHostToDevice, s));
-                // in a sweep, counts known from an earlier threshold need
-                // only the threshold test
-                bool cached = memo != nullptr;
-                if (memo)
-                    for (uint64_t i = 0; i < nC && cached; i++) {
-                        auto it = memo->fcnt.find(std::vector<uint32_t>(&C[i * k], &C[i * k] + k));
-                        if (it == memo->fcnt.end()) cached = false;
-                        else memcpy(&fcnt[i * k], it->second.data(), k * 4);
-                    }
-                size_t e0 = mark();
-                int rc;
-                if (cached) {
-                    COLO_HIP(hipMemcpyAsync(d_counts.p, fcnt.data(), fcnt.size() * 4, hipMemcpyHostToDevice, s));
-                    rc = threshold_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
-                                         d_pass.as<uint8_t>(), s);
-                } else {
-                    rc = filter_batch(c, d_cands.as<uint32_t>(), (uint32_t)nC, k, o.theta, d_counts.as<uint32_t>(),
-                                      d_pass.as<uint8_t>(), s);
-                }
-                if (rc) return rc;
-                span(e0, mark(), kFilter);
-                if (!cached) COLO_HIP(hipMemcpyAsync(fcnt.data(), d_counts.p, fcnt.size() * 4, hipMemcpyDeviceToHost, s));
-                COLO_HIP(hipMemcpyAsync(pass.data(), d_pass.p, nC, hipMemcpyDeviceToHost, s));
-                if ((rc = sync())) return rc;
-                st.launches += cached ? 1 : 2;
-                if (memo && !cached)
-                    for (uint64_t i = 0; i < nC; i++)
-                        memo->fcnt[std::vector<uint32_t>(&C[i * k], &C[i * k] + k)] =
-                            std::vector<uint32_t>(&fcnt[i * k], &fcnt[i * k] + k);
+            // filter (cpp:297, 760) and, from level 3, subset bounds: computed
+            // here, or already by the previous level (to choose its tables)
+            std::vector<uint32_t> fcnt;
+            std::vector<uint8_t> pass;
+            std::vector<float> sbq;
+            if (pre_valid_) {
+                fcnt.swap(pre_fcnt_);
+                pass.swap(pre_pass_);
+                sbq.swap(pre_sbq_);
+                pre_valid_ = false;
+            } else {
+                int rc0 = level_filter(k, C, fcnt, pass);
+                if (!rc0 && o.subset_bound && k >= 3) rc0 = level_bounds(k, C, pass, Lprev, sbq);
+                if (rc0) return rc0;
             }
-            st.filter_ms += ms_since(tf);
             uint64_t npass = 0;
             for (uint8_t p : pass) npass += p;
             std::vector<CandRes> res(nC);
@@ -1388,19 +1424,13 @@ struct Miner {
             auto tl = Clock::now();
             // subset bound (from level 3): candidates whose bound is below
             // theta cannot be prevalent and are not counted
-            std::vector<float> sbq;
             std::vector<uint8_t> live = pass;
-            if (o.subset_bound && k >= 3) {
-                auto tb = Clock::now();
-                int rc0 = subset_bounds(k, C, pass, Lprev, sbq);
-                if (rc0) return rc0;
+            if (!sbq.empty())
                 for (uint64_t i = 0; i < nC; i++)
                     if (pass[i] && sbq[i] < o.theta) {
                         live[i] = 0;
                         st.pruned++;
                     }
-                st.bound_ms += ms_since(tb);
-            }
             keep_parts_ = o.subset_bound && (o.max_k == 0 || k + 1 <= o.max_k);
             // candidates to count: those passing both tests whose results
             // are not known from an earlier run of a sweep
@@ -1540,10 +1570,34 @@ struct Miner {
             }
             // tables to build: the reference's (prefixes of the next level's
             // candidates); at level 2 every prevalent pair, whose tables serve
-            // as the anchors' neighbour lists at the later levels
+            // as the anchors' neighbour lists at the later levels.  From level
+            // 3, without table checksums, only the prefixes of the next
+            // level's candidates that pass its filter and subset bound: the
+            // others' tables would never be read (the report lists the
+            // reference's tables and their rows either way).
             std::vector<uint8_t> build = need;
             if (k == 2 && !N.empty())
                 for (auto &b : build) b = 1;
+            if (k >= 3 && !N.empty() && !o.hash_tables) {
+                const auto tp = Clock::now();  // filter and bound time, not host time
+                index_prev(L);
+                int rc0 = level_filter(k + 1, N, pre_fcnt_, pre_pass_);
+                pre_sbq_.clear();
+                if (!rc0 && o.subset_bound) rc0 = level_bounds(k + 1, N, pre_pass_, L, pre_sbq_);
+                index_prev(Lprev);  // this level's parent lookups (recounts)
+                if (rc0) return rc0;
+                pre_valid_ = true;
+                std::unordered_set<std::vector<uint32_t>, PatSet::H> live_pref(16, PatSet::H{k});
+                for (uint64_t i = 0; i < N.size() / (k + 1); i++)
+                    if (pre_pass_[i] && !(!pre_sbq_.empty() && pre_sbq_[i] < o.theta))
+                        live_pref.emplace(N.begin() + i * (k + 1), N.begin() + i * (k + 1) + k);
+                for (size_t i = 0; i < L.size(); i++)
+                    if (build[i] && !live_pref.count(L[i].pat)) {
+                        build[i] = 0;
+                        st.tables_skipped++;
+                    }
+                st.host_ms -= ms_since(tp);
+            }
             st.host_ms += ms_since(th);
             auto tt = Clock::now();
             if (memo) {

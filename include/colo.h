@@ -259,6 +259,10 @@ typedef struct colo_mine_stats {
     double bound_ms;        /* subset bounds: wall time                      */
     double scan_kernel_ms;  /* table row-offset scans (batched schedules)    */
     uint64_t graphs;        /* level passes launched as captured HIP graphs  */
+    uint64_t tables_skipped;/* tables of the reference's set not built: every
+                               candidate extending them fails the next
+                               level's filter or subset bound (only without
+                               table checksums)                            */
 } colo_mine_stats;
 
 /* HIP graphs: the binning (colo_bin, colo_ctx_create) is captured once per

@@ -568,16 +568,19 @@ def test_subset_bound(dense, colo_mod, sched, pi_mode):
 
 
 @pytest.mark.parametrize("sched", ["plain", "persistent"])
-def test_subset_bound_sweep(dense, colo_mod, sched):
+@pytest.mark.parametrize("hashes", [True, False])
+def test_subset_bound_sweep(dense, colo_mod, sched, hashes):
     """A threshold sweep with the subset bound equals one mining run per
-    threshold (bounds and participation are kept across the sweep)."""
+    threshold (bounds and participation are kept across the sweep); without
+    table checksums only the tables that surviving candidates extend are
+    built."""
     ctx, orc, p = dense
-    _say(f"subset bound sweep sched={sched}")
+    _say(f"subset bound sweep sched={sched} hashes={hashes}")
     sc = colo_mod.COLO_SCHED_PLAIN if sched == "plain" else colo_mod.COLO_SCHED_PERSISTENT
     thetas = [0.3, 0.1, 0.15]
-    out = ctx.mine_sweep(thetas, max_k=4, sched=sc, subset_bound=True)
+    out = ctx.mine_sweep(thetas, max_k=4, sched=sc, subset_bound=True, hash_tables=hashes)
     for th, (rep, st) in zip(thetas, out):
-        assert rep == orc.mine(p["r"], th, max_k=4, subset_bound=True), th
+        assert rep == orc.mine(p["r"], th, max_k=4, subset_bound=True, hash_tables=hashes), th
 
 
 def test_subset_bound_c3_small(datagen, colo_mod, oracle_mod):
@@ -622,3 +625,23 @@ def test_level_graphs(datagen, colo_mod, oracle_mod, sched):
         rep, st = ctx.mine(p["theta"], max_k=4, sched=sc, model=model, subset_bound=True)
         assert rep == expect, i
         assert st["graphs"] == 0
+
+
+@pytest.mark.parametrize("sched", ["plain", "persistent", "learned"])
+def test_tables_for_surviving_candidates(dense, colo_mod, sched):
+    """Without table checksums, a level builds only the tables that the next
+    level's candidates passing its filter and subset bound extend: the report
+    (which lists the reference's tables and their rows) equals the oracle's,
+    with and without the subset bound."""
+    ctx, orc, p = dense
+    _say(f"tables for surviving candidates sched={sched}")
+    sc = {"plain": colo_mod.COLO_SCHED_PLAIN, "persistent": colo_mod.COLO_SCHED_PERSISTENT,
+          "learned": colo_mod.COLO_SCHED_LEARNED}[sched]
+    model = None
+    if sc == colo_mod.COLO_SCHED_LEARNED:
+        model = colo_mod.CostModel.load(os.path.join(os.path.dirname(colo_mod.__file__), "models", "cost_mlp.bin"))
+    for sb in (True, False):
+        rep, st = ctx.mine(p["theta"], max_k=4, sched=sc, model=model, subset_bound=sb, hash_tables=False)
+        assert rep == orc.mine(p["r"], p["theta"], max_k=4, subset_bound=sb, hash_tables=False), sb
+        if sb:
+            assert st["tables_skipped"] > 0

@@ -1570,15 +1570,18 @@ struct Miner {
             }
             // tables to build: the reference's (prefixes of the next level's
             // candidates); at level 2 every prevalent pair, whose tables serve
-            // as the anchors' neighbour lists at the later levels.  From level
-            // 3, without table checksums, only the prefixes of the next
-            // level's candidates that pass its filter and subset bound: the
-            // others' tables would never be read (the report lists the
-            // reference's tables and their rows either way).
+            // as the anchors' neighbour lists at the later levels.  Without
+            // table checksums, only the tables that the next level's
+            // candidates passing its filter and subset bound read: their
+            // prefixes and, at level 2, also their (first, last) pairs.  Every
+            // (first, last) pair of a candidate counted at a later level is
+            // one of a counted size-3 candidate (its size-3 subsets are
+            // prevalent).  The others' tables would never be read; the report
+            // lists the reference's tables and their rows either way.
             std::vector<uint8_t> build = need;
             if (k == 2 && !N.empty())
                 for (auto &b : build) b = 1;
-            if (k >= 3 && !N.empty() && !o.hash_tables) {
+            if (!N.empty() && !o.hash_tables) {
                 const auto tp = Clock::now();  // filter and bound time, not host time
                 index_prev(L);
                 int rc0 = level_filter(k + 1, N, pre_fcnt_, pre_pass_);
@@ -1589,8 +1592,10 @@ struct Miner {
                 pre_valid_ = true;
                 std::unordered_set<std::vector<uint32_t>, PatSet::H> live_pref(16, PatSet::H{k});
                 for (uint64_t i = 0; i < N.size() / (k + 1); i++)
-                    if (pre_pass_[i] && !(!pre_sbq_.empty() && pre_sbq_[i] < o.theta))
+                    if (pre_pass_[i] && !(!pre_sbq_.empty() && pre_sbq_[i] < o.theta)) {
                         live_pref.emplace(N.begin() + i * (k + 1), N.begin() + i * (k + 1) + k);
+                        if (k == 2) live_pref.emplace(std::vector<uint32_t>{N[i * 3], N[i * 3 + 2]});
+                    }
                 for (size_t i = 0; i < L.size(); i++)
                     if (build[i] && !live_pref.count(L[i].pat)) {
                         build[i] = 0;

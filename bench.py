@@ -255,6 +255,30 @@ def e2e_child(args):
     return 0
 
 
+def mine_once(args):
+    """One mining run of the workload at its lowest threshold (learned
+    schedule when the model is present, subset bound as --subset-bound), with
+    the launch shapes from the autotuner's cache (a cache miss tunes first)."""
+    import colo
+    import torch
+    x, y, fc, bounds, p = gen(args.workload, N=args.N)
+    th = min(p["thetas"]) if "thetas" in p else p["theta"]
+    model = None if args.no_model or not os.path.exists(args.model) else colo.CostModel.load(args.model)
+    sched = colo.COLO_SCHED_LEARNED if model is not None else colo.COLO_SCHED_PERSISTENT
+    sb = args.subset_bound == "on"
+    ctx = colo.Context(x, y, fc, bounds, p["r"])
+    ctx.autotune(th, max_k=p["max_k"], sched=sched, model=model, cache=args.tune_cache, subset_bound=sb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, st = ctx.mine(th, max_k=p["max_k"], sched=sched, model=model, hash_tables=False, subset_bound=sb)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": describe(p), "theta": th, "ms": (time.perf_counter() - t0) * 1e3,
+                      "level_count_kernel_ms": {str(k): round(v, 2) for k, v in enumerate(st["level_count_kernel_ms"])
+                                                if v > 0}}), flush=True)
+    ctx.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +300,9 @@ def main():
                     help="only time complete mining of WORKLOAD by the oracle on the host cores beside the GPU")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (traffic-counter calibration)")
+    ap.add_argument("--mine-once", action="store_true",
+                    help="only mine the workload once at its lowest threshold with the timed schedule and the "
+                         "autotuner's cached launch shapes (a short program for hardware-counter runs)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -288,6 +315,8 @@ def main():
         return 0
     if args.e2e_child:
         return e2e_child(args)
+    if args.mine_once:
+        return mine_once(args)
     if args.cpu_full:
         import colo
         model = None if args.no_model or not os.path.exists(args.model) else colo.CostModel.load(args.model)

@@ -10,6 +10,15 @@
       `--pmc WRITE_SIZE` runs (KiB per dispatch), each scaled by what the same
       counter reads for a plain device copy of B bytes (bench.py --copy-probe,
       __amd_rocclr_copyBuffer): scale = B / counter.  Prints JSON.
+  counters --trace T.csv --pmc P1.csv [P2.csv ...] --fetch F.csv --write W.csv --cfetch CF.csv --cwrite CW.csv
+           [--match k_count,k_write] [--min-ms 1]
+      Markdown table, one line per dispatch of the matching kernels lasting
+      at least min-ms, of the counter runs of one deterministic program (the
+      same dispatch ids in every run): duration, TA / TD busy and TD stalls
+      (per unit and cycle, 256 units), L2 hit rate, VALU and vector-read
+      instructions, VALU busy (4 cycles per instruction, 1024 SIMDs) and HBM
+      bytes (traffic's copy-calibrated scales).  Cycles per XCD =
+      GRBM_GUI_ACTIVE / 8.
 """
 import argparse
 import collections
@@ -61,6 +70,49 @@ def _counter_per_dispatch(path, counter, kernel):
     return sum(per.values()) / len(per), len(per)
 
 
+def _per_dispatch(path):
+    """{dispatch id: (kernel name, {counter: value summed over dimensions})}."""
+    out = {}
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            d = out.setdefault(r["Dispatch_Id"], (r["Kernel_Name"], collections.defaultdict(float)))
+            d[1][r["Counter_Name"]] += float(r["Counter_Value"])
+    return out
+
+
+def counters(a):
+    dur = {}
+    with open(a.trace) as fh:
+        for r in csv.DictReader(fh):
+            dur[r["Dispatch_Id"]] = (r["Kernel_Name"], (float(r["End_Timestamp"]) - float(r["Start_Timestamp"])) / 1e6)
+    merged = collections.defaultdict(dict)
+    for p in a.pmc + [a.fetch, a.write]:
+        for did, (_, cs) in _per_dispatch(p).items():
+            merged[did].update(cs)
+    cf, _ = _counter_per_dispatch(a.cfetch, "FETCH_SIZE", "__amd_rocclr_copyBuffer")
+    cw, _ = _counter_per_dispatch(a.cwrite, "WRITE_SIZE", "__amd_rocclr_copyBuffer")
+    fs, ws = a.copy_bytes / (cf * 1024.0), a.copy_bytes / (cw * 1024.0)
+    pats = a.match.split(",")
+    rows = ["| kernel | ms | TA busy | TD busy | TD stalled on L1 | L2 hit | VALU instr | VMEM read instr | VALU busy | "
+            "HBM read GB | HBM write GB |", "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for did in sorted(dur, key=lambda x: int(x)):
+        name, ms = dur[did]
+        short = name.split("(")[0].replace("void ", "").strip()
+        if ms < a.min_ms or not any(short.startswith(p) for p in pats) or did not in merged:
+            continue
+        c = merged[did]
+        cyc = c.get("GRBM_GUI_ACTIVE", 0.0) / 8.0
+        if not cyc:
+            continue
+        hit, miss = c.get("TCC_HIT_sum", 0.0), c.get("TCC_MISS_sum", 0.0)
+        rows.append(f"| `{short}` | {ms:.1f} | {c.get('TA_TA_BUSY_sum', 0) / (256 * cyc):.0%} | "
+                    f"{c.get('TD_TD_BUSY_sum', 0) / (256 * cyc):.0%} | {c.get('TD_TC_STALL_sum', 0) / (256 * cyc):.0%} | "
+                    f"{hit / (hit + miss) if hit + miss else 0:.2f} | {c.get('SQ_INSTS_VALU', 0):.3g} | "
+                    f"{c.get('SQ_INSTS_VMEM_RD', 0):.3g} | {c.get('SQ_INSTS_VALU', 0) * 4 / (1024 * cyc):.0%} | "
+                    f"{c.get('FETCH_SIZE', 0) * 1024 * fs / 1e9:.1f} | {c.get('WRITE_SIZE', 0) * 1024 * ws / 1e9:.1f} |")
+    return "\n".join(rows)
+
+
 def traffic(a):
     f, nd = _counter_per_dispatch(a.fetch, "FETCH_SIZE", a.kernel)
     w, _ = _counter_per_dispatch(a.write, "WRITE_SIZE", a.kernel)
@@ -87,9 +139,19 @@ def main():
         t.add_argument(f"--{n}", required=True)
     t.add_argument("--copy-bytes", type=float, default=256 * 1024 * 1024)
     t.add_argument("--workload-key", default=None, help="the workload description bench.py prints in config.workload")
+    c = sub.add_parser("counters")
+    c.add_argument("--trace", required=True)
+    c.add_argument("--pmc", nargs="+", required=True)
+    for n in ("fetch", "write", "cfetch", "cwrite"):
+        c.add_argument(f"--{n}", required=True)
+    c.add_argument("--match", default="k_count,k_write")
+    c.add_argument("--min-ms", type=float, default=1.0)
+    c.add_argument("--copy-bytes", type=float, default=256 * 1024 * 1024)
     a = ap.parse_args()
     if a.cmd == "kernels":
         print(kernels(a.path, a.top, a.templates))
+    elif a.cmd == "counters":
+        print(counters(a))
     else:
         print(json.dumps(traffic(a), indent=1))
 

@@ -87,6 +87,37 @@ def test_c5_sampled_oracle_parity(datagen, colo_mod, oracle_mod):
                   f"model |log err| {st['model_abs_err'] / max(st['model_n'], 1):.3f}, overflows {st['overflows']}, "
                   f"slots deferred {st['slots_deferred']}, recounts {st['recounts']}", flush=True)
     assert reps["learned"] == reps["persistent"] == reps["plain"]
+    # the subset bound (learned schedule, table checksums): every level line,
+    # table line and counted candidate's line as without it, the skipped
+    # candidates not prevalent, and each bound at least the candidate's PI
+    _say("c5 sweep learned with the subset bound")
+    bound = [r for r, _ in ctx.mine_sweep(p["thetas"], max_k=p["max_k"], sched=colo_mod.COLO_SCHED_LEARNED,
+                                          model=model, hash_tables=True, subset_bound=True)]
+    n_skipped = 0
+    for th, rb, rf in zip(p["thetas"], bound, reps["learned"]):
+        cb, tb, lb = parse_report(rb)
+        cf, tf, lf = parse_report(rf)
+        assert tb == tf and lb == lf, th
+        lines_f = {tuple(l.split(" f ")[0].split()[1:]): l for l in rf.splitlines() if l.startswith("c ")}
+        for line in rb.splitlines():
+            if not line.startswith("c "):
+                continue
+            key = tuple(line.split(" f ")[0].split()[1:])
+            c = tuple(int(v) for v in key)
+            t = line.split()
+            has_q = "q" in t
+            if has_q:
+                q = np.uint32(int(t[t.index("q") + 1], 16)).view(np.float32)
+                if "pic" in cf[c]:
+                    assert np.uint32(int(cf[c]["pic"], 16)).view(np.float32) <= q, (th, c)
+                del t[t.index("q"):t.index("q") + 2]
+            if "rows" in cb[c] or not has_q:  # counted, or cut by the filter
+                assert " ".join(t) == lines_f[key], (th, c)
+            else:  # skipped by the bound
+                assert cf[c]["v"] == 0, (th, c)
+                n_skipped += 1
+    print(f"[c5] subset bound: {n_skipped} candidate evaluations skipped over the sweep", flush=True)
+    assert n_skipped > 1000
     ctx.close()
     # the stated depth: sizes 2..6 at the lowest threshold
     cands, tables, levels = parse_report(reps["learned"][0])

@@ -133,8 +133,9 @@ struct Prev {
 
 // Results that do not depend on theta, kept across the runs of a threshold
 // sweep (ascending theta): every candidate's counted rows, numerators and
-// PIs, and the level-2 tables (the prevalent pairs at a higher theta are a
-// subset of those at a lower one).
+// PIs, and the instance tables built (a table's rows do not depend on
+// theta; the prevalent pairs at a higher theta are a subset of those at a
+// lower one).
 struct SweepMemo {
     struct Res {
         uint64_t rows;
@@ -143,6 +144,9 @@ struct SweepMemo {
     };
     std::map<std::vector<uint32_t>, Res> res;
     std::map<std::pair<uint32_t, uint32_t>, Prev> l2;
+    // instance tables of size 3 and more built so far (their rows do not
+    // depend on theta): memory and device pointer
+    std::map<std::vector<uint32_t>, std::pair<std::shared_ptr<DevBuf>, uint32_t *>> tk;
     std::map<std::vector<uint32_t>, std::vector<uint32_t>> fcnt;  // filter counts (theta-independent)
     // subset bounds and the participation of every prevalent pattern so far
     // (both theta-independent): a candidate first met at a higher theta
@@ -1613,15 +1617,21 @@ struct Miner {
             st.host_ms += ms_since(th);
             auto tt = Clock::now();
             if (memo) {
-                // level-2 tables from earlier runs
-                if (k == 2)
-                    for (size_t i = 0; i < L.size(); i++) {
-                        if (!build[i]) continue;
+                // tables from earlier runs
+                for (size_t i = 0; i < L.size(); i++) {
+                    if (!build[i]) continue;
+                    if (k == 2) {
                         auto it = memo->l2.find({L[i].pat[0], L[i].pat[1]});
                         if (it == memo->l2.end()) continue;
                         L[i] = it->second;
-                        build[i] = 0;
+                    } else {
+                        auto it = memo->tk.find(L[i].pat);
+                        if (it == memo->tk.end()) continue;
+                        L[i].table = it->second.first;
+                        L[i].tptr = it->second.second;
                     }
+                    build[i] = 0;
+                }
             }
             // tables to build whose slots were not kept (deferred above the
             // slots budget, or remembered from an earlier threshold)
@@ -1652,6 +1662,9 @@ struct Miner {
             if (memo && k == 2)
                 for (const Prev &p : L)
                     if (p.tptr && p.nptr) memo->l2.emplace(std::make_pair(p.pat[0], p.pat[1]), p);
+            if (memo && k >= 3)
+                for (const Prev &p : L)
+                    if (p.tptr) memo->tk.emplace(p.pat, std::make_pair(p.table, p.tptr));
             if (k == 2) {
                 l2_index_.assign((size_t)c->F * c->F, -1);
                 for (size_t i = 0; i < L.size(); i++) {

@@ -53,6 +53,8 @@ colo::DevData colo_ctx::dev() const {
     d.x = d_x; d.y = d_y; d.cell_of = d_cell_of; d.sidx = d_sidx; d.spos = d_spos;
     d.sxy = d_sxy; d.scell = d_scell; d.dir = d_dir;
     d.fstart = d_fstart; d.fcount = d_fcount;
+    d.mask3 = d_mask3;
+    d.mask_w = mask_words(F);
     return d;
 }
 
@@ -457,28 +459,83 @@ __global__ void k_gather_sorted(const K *skeys, const uint32_t *sidx, uint32_t n
     scell[p] = (uint32_t)(skeys[p] & (((K)1 << cellbits) - 1));
 }
 
-// Per-(feature, cell) counts into the dense directory, one atomic per run of
-// equal keys inside a wavefront; also flags run starts for the filter.
+// Directory: dir[f * (cells + 1) + c] = first sorted position whose
+// (feature, cell) index is at least (f, c), i.e. the exclusive prefix count
+// of the per-(feature, cell) counts.  It is written tile by tile
+// (kDirTile entries): k_dir_tiles flags the run starts (first element of
+// each distinct key) and finds each tile's first element, k_dir_fill
+// places the tile's run starts in LDS and fills the entries between them by
+// a suffix minimum.  Every entry is written once and the keys are read
+// twice; no counts, atomics or scan over the F x (cells + 1) entries.
+constexpr int kDirItems = 16;
+constexpr uint32_t kDirTile = kBlock * kDirItems;
+
 template <typename K>
-__global__ void k_dir_counts(const K *skeys, uint32_t n, uint32_t cellbits, uint32_t cells, uint32_t *dir,
-                             uint32_t *run_flag) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    bool valid = p < n;
-    K k = valid ? skeys[p] : (K)0;
-    K kp = (valid && p > 0) ? skeys[p - 1] : (K)~(K)0;
-    K kn = (valid && p + 1 < n) ? skeys[p + 1] : (K)~(K)0;
-    bool gstart = valid && (p == 0 || kp != k);
-    bool start = valid && (lane == 0 || kp != k);
-    bool end = valid && (lane == 63 || p + 1 == n || kn != k);
-    uint64_t sm = __ballot(start);
-    if (valid) run_flag[p] = gstart ? 1u : 0u;
-    if (end) {
-        uint64_t upto = lane == 63 ? sm : (sm & ((2ull << lane) - 1));
-        int s = 63 - __builtin_clzll(upto);
-        uint32_t f = (uint32_t)(k >> cellbits);
-        uint32_t cell = (uint32_t)(k & (((K)1 << cellbits) - 1));
-        atomicAdd(&dir[(uint64_t)f * (cells + 1) + cell], (uint32_t)(lane - s + 1));
+__device__ inline uint64_t dir_index(K key, uint32_t cellbits, uint32_t cells) {
+    return (uint64_t)(key >> cellbits) * (cells + 1ull) + (uint64_t)(key & (((K)1 << cellbits) - 1));
+}
+
+// run_flag[p] = 1 at run starts; tile_lo[T] (T <= ntiles) = first sorted
+// position whose directory index is >= T * kDirTile (n past the last).
+template <typename K>
+__global__ void k_dir_tiles(const K *skeys, uint32_t n, uint32_t cellbits, uint32_t cells, uint64_t ntiles,
+                            uint32_t *run_flag, uint32_t *tile_lo) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const K k = skeys[p];
+    const bool start = p == 0 || skeys[p - 1] != k;
+    run_flag[p] = start ? 1u : 0u;
+    const uint64_t d = dir_index(k, cellbits, cells);
+    if (start) {
+        // the tiles whose first entry lies in (index of p - 1, index of p]
+        const uint64_t t0 = p ? dir_index(skeys[p - 1], cellbits, cells) / kDirTile + 1 : 0;
+        for (uint64_t t = t0; t <= d / kDirTile; t++) tile_lo[t] = p;
+    }
+    if (p + 1 == n)
+        for (uint64_t t = d / kDirTile + 1; t <= ntiles; t++) tile_lo[t] = n;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_dir_fill(const K *skeys, uint32_t cellbits, uint32_t cells,
+                                                     uint64_t dir_len, uint64_t ntiles, const uint32_t *tile_lo,
+                                                     uint32_t *dir) {
+    __shared__ uint32_t v[kDirTile];
+    __shared__ uint32_t wmin[kBlock / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t e0 = t * kDirTile;
+        const uint32_t lo = tile_lo[t], hi = tile_lo[t + 1];
+        for (uint32_t i = tid; i < kDirTile; i += kBlock) v[i] = hi;
+        __syncthreads();
+        for (uint32_t p = lo + tid; p < hi; p += kBlock) {
+            const K k = skeys[p];
+            if (p == 0 || skeys[p - 1] != k) v[dir_index(k, cellbits, cells) - e0] = p;
+        }
+        __syncthreads();
+        // suffix minimum: thread tid owns entries [kDirItems * tid, + kDirItems)
+        uint32_t r[kDirItems];
+        uint32_t m = ~0u;
+#pragma unroll
+        for (int i = kDirItems - 1; i >= 0; i--) {
+            m = min(m, v[kDirItems * tid + i]);
+            r[i] = m;
+        }
+        uint32_t x = m;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_down(x, d);
+            if (lane + d < 64) x = min(x, y);
+        }
+        uint32_t carry = __shfl_down(x, 1);
+        if (lane == 63) carry = ~0u;
+        if (lane == 0) wmin[wave] = x;
+        __syncthreads();
+        for (uint32_t w = wave + 1; w < kBlock / 64; w++) carry = min(carry, wmin[w]);
+#pragma unroll
+        for (int i = 0; i < kDirItems; i++) v[kDirItems * tid + i] = min(r[i], carry);
+        __syncthreads();
+        for (uint32_t i = tid; i < kDirTile && e0 + i < dir_len; i += kBlock) dir[e0 + i] = v[i];
+        __syncthreads();
     }
 }
 
@@ -515,6 +572,23 @@ __global__ void k_qmask(const uint64_t *cellmask, uint32_t gx, uint32_t gy, uint
     uint64_t c0 = (uint64_t)qy * gx + qx;
     qmask[i] = cellmask[c0 * W + w] | cellmask[(c0 + 1) * W + w] | cellmask[(c0 + gx) * W + w] |
                cellmask[(c0 + gx + 1) * W + w];
+}
+
+// Neighbourhood masks: OR of the cell masks over the clamped 3 x 3 block
+// around each cell, a superset of the cells whose directory spans
+// row_spans reads for an anchor there (one thread per cell and mask word).
+__global__ void k_mask3(const uint64_t *cellmask, uint32_t gx, uint32_t gy, uint32_t W, uint64_t *mask3) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)gx * gy * W) return;
+    const uint64_t c = i / W;
+    const uint32_t w = (uint32_t)(i - c * W);
+    const uint32_t ay = (uint32_t)(c / gx), ax = (uint32_t)(c - (uint64_t)ay * gx);
+    const uint32_t x0 = ax ? ax - 1 : 0, x1 = ax + 1 < gx ? ax + 1 : gx - 1;
+    const uint32_t y0 = ay ? ay - 1 : 0, y1 = ay + 1 < gy ? ay + 1 : gy - 1;
+    uint64_t m = 0;
+    for (uint32_t yy = y0; yy <= y1; yy++)
+        for (uint32_t xx = x0; xx <= x1; xx++) m |= cellmask[((uint64_t)yy * gx + xx) * W + w];
+    mask3[i] = m;
 }
 
 // Binning results for the host, in one block (u64 words): [0] bounds flag,
@@ -584,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, co
     uint64_t M[W];
 #pragma unroll
     for (int w = 0; w < W; w++) M[w] = mine ? cmask[(uint64_t)(me / k) * W + w] : 0;
-    uint64_t sum = 0;
+    uint32_t sum = 0;
     const bool quads = g.gx >= 2 && g.gy >= 2;
     for (uint64_t base = r0; base < r1; base += kBlock) {
         const uint64_t h = base + threadIdx.x;
@@ -630,7 +704,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, co
         }
         __syncthreads();
     }
-    if (mine && sum) atomicAdd(&counts[me], (uint32_t)sum);
+    if (mine && sum) atomicAdd(&counts[me], sum);
 }
 
 // threshold_Kernel (K:545-573): ub_t = fl32(double(cnt)/double(fl32(N))),
@@ -827,10 +901,12 @@ struct BitAcc {
 
 // Level-2 work estimate of a (first, last) feature pair: over the non-empty
 // cells c of `first`, count(first, c) x (instances of `last` in the 3x3
-// cells around c) -- the number of distance tests of the 3x3 join.
+// cells around c) -- the number of distance tests of the 3x3 join.  Cells
+// whose 3 x 3 mask lacks `last` add nothing and read no directory.
 __global__ __launch_bounds__(kBlock) void k_pairwork(const FilterItem *items, const uint64_t *chunk_prefix,
                                                      uint64_t n_items, const uint32_t *pairs, const uint32_t *run_cell,
-                                                     const uint32_t *dir, DevGrid g, unsigned long long *out) {
+                                                     const uint32_t *dir, DevGrid g, const uint64_t *mask3,
+                                                     uint32_t mask_w, unsigned long long *out) {
     uint64_t b = blockIdx.x;
     uint64_t lo = 0, hi = n_items;
     while (hi - lo > 1) {
@@ -846,6 +922,7 @@ __global__ __launch_bounds__(kBlock) void k_pairwork(const FilterItem *items, co
     uint64_t sum = 0;
     for (uint64_t h = r0 + threadIdx.x; h < r1; h += kBlock) {
         uint32_t c = run_cell[h];
+        if (mask3 && !((mask3[(uint64_t)c * mask_w + (fl >> 6)] >> (fl & 63)) & 1)) continue;
         RowSpan rs;
         int m = row_spans(g, dirl, c, rs);
         uint64_t nb = 0;
@@ -942,8 +1019,8 @@ int colo::pairwork_batch(colo_ctx *c, const uint32_t *pairs, uint32_t P, uint64_
         COLO_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(FilterItem), hipMemcpyHostToDevice, s));
         COLO_HIP(hipMemcpyAsync(d_prefix, prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_pairwork, dim3((unsigned)chunks), dim3(kBlock), 0, s, d_items, d_prefix,
-                           (uint64_t)items.size(), d_pairs, c->d_run_cell, c->d_dir, c->dev().g,
-                           (unsigned long long *)d_out);
+                           (uint64_t)items.size(), d_pairs, c->d_run_cell, c->d_dir, c->dev().g, c->d_mask3,
+                           mask_words(c->F), (unsigned long long *)d_out);
         COLO_HIP(hipGetLastError());
     }
     COLO_HIP(hipMemcpyAsync(out, d_out, (size_t)P * 8, hipMemcpyDeviceToHost, s));
@@ -1032,14 +1109,18 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
 // (up to three) row ranges of feature b are walked as one stream, range 0,
 // then 1, then 2 -- the scan order -- kFlat entries per step, so a short
 // range does not cost a dependent round trip of its own.  f(q, hit) is
-// called for every entry in stream order while keep() holds.
+// called for every entry in stream order while keep() holds.  With MASK,
+// an anchor whose cell's 3 x 3 mask lacks b has no entries and reads no
+// directory (the write pass visits only anchors with rows).
 constexpr int kFlat = 8;
-template <typename F, typename K>
+template <bool MASK, typename F, typename K>
 __device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F &&f, K &&keep) {
+    const uint32_t cell = D.scell[p];
+    if (MASK && D.mask3 && !((D.mask3[(uint64_t)cell * D.mask_w + (b >> 6)] >> (b & 63)) & 1)) return;
     const float2 pa = D.sxy[p];
     const uint32_t *dirb = D.dir + (uint64_t)b * (D.g.cells + 1);
     RowSpan rs;
-    const int m = row_spans(D.g, dirb, D.scell[p], rs);
+    const int m = row_spans(D.g, dirb, cell, rs);
     const uint32_t lo0 = m > 0 ? rs.lo[0] : 0, lo1 = m > 1 ? rs.lo[1] : 0, lo2 = m > 2 ? rs.lo[2] : 0;
     const uint32_t p1 = m > 0 ? rs.hi[0] - lo0 : 0;
     const uint32_t p2 = p1 + (m > 1 ? rs.hi[1] - lo1 : 0);
@@ -1066,7 +1147,7 @@ __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32
     const uint32_t sb = D.fstart[b];
     uint32_t cnt = 0;
     BitAcc acc(bits_b);
-    join2_stream(
+    join2_stream<true>(
         D, b, D.fstart[a] + t,
         [&](uint32_t q, bool hit) {
             if (hit) {
@@ -1090,7 +1171,7 @@ __device__ inline void join2_write_flat(const DevData &D, uint32_t a, uint32_t b
     uint32_t *out = table + off * 2;
     uint32_t *nb = nbr ? nbr + off : nullptr;
     uint32_t w = 0;
-    join2_stream(
+    join2_stream<false>(
         D, b, g,
         [&](uint32_t q, bool hit) {
             if (hit) {
@@ -2838,11 +2919,15 @@ static void free_bin_bufs(colo_ctx *c) {
     c->b_stmp = c->b_small = nullptr;
 }
 
+static uint64_t dir_tiles(const colo_ctx *c) { return (c->dir_len + kDirTile - 1) / kDirTile; }
+
+// binning workspace (u64 words): radix and run scans, the directory's tile
+// starts (dir_tiles + 1 u32), the results block
 static uint64_t bin_tmp_words(const colo_ctx *c) {
     const uint64_t n = c->n;
     const uint64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
-    return std::max<uint64_t>(std::max(scan_tmp_words(ntiles * 256), scan_tmp_words(std::max<uint64_t>(c->dir_len, n))),
-                              (uint64_t)c->F + 3);
+    return std::max<uint64_t>(std::max(scan_tmp_words(ntiles * 256), scan_tmp_words(n)),
+                              std::max<uint64_t>(dir_tiles(c) / 2 + 1, (uint64_t)c->F + 3));
 }
 
 // Work buffers of the binning for the current grid.
@@ -2892,13 +2977,19 @@ static int bin_launch(colo_ctx *c, hipStream_t s) {
         hipLaunchKernelGGL((k_gather_sorted<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk,
                            c->d_sidx, n, c->cellbits, c->d_x, c->d_y, c->d_sxy, c->d_spos, c->d_scell);
     }
-    // directory: counts per (f, cell) then exclusive scan in place
-    if ((rc = fill_u32(c->d_dir, c->dir_len, 0, s))) return rc;
+    // directory and run starts: each tile's first element (in stmp), then
+    // the tiles
     uint32_t *run_flag = vals2;  // free once the sorted indices are in d_sidx
-    if (n)
-        hipLaunchKernelGGL((k_dir_counts<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk, n,
-                           c->cellbits, c->cells, c->d_dir, run_flag);
-    scan_rec<uint32_t, uint32_t>(c->d_dir, c->dir_len, c->d_dir, nullptr, stmp, s);
+    if (n) {
+        const uint64_t ntiles = dir_tiles(c);
+        uint32_t *tile_lo = (uint32_t *)stmp;
+        hipLaunchKernelGGL((k_dir_tiles<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk, n,
+                           c->cellbits, c->cells, ntiles, run_flag, tile_lo);
+        hipLaunchKernelGGL((k_dir_fill<K>), dim3((unsigned)std::min<uint64_t>(ntiles, 1u << 20)), dim3(kBlock), 0, s,
+                           sk, c->cellbits, c->cells, c->dir_len, ntiles, (const uint32_t *)tile_lo, c->d_dir);
+    } else if ((rc = fill_u32(c->d_dir, c->dir_len, 0, s))) {
+        return rc;
+    }
     // runs (distinct (feature, cell) pairs) for the filter; H lands in small[1]
     uint64_t *run_idx = sizeof(K) < 8 ? (uint64_t *)c->b_runidx : (uint64_t *)keys2;
     if (n) {
@@ -2919,6 +3010,8 @@ static int bin_launch(colo_ctx *c, hipStream_t s) {
                                c->F, small + 1, W, (unsigned long long *)c->d_cellmask);
         if (nq) hipLaunchKernelGGL(k_qmask, dim3(grid_for(nq * W, 1u << 30)), dim3(kBlock), 0, s, c->d_cellmask, c->gx,
                                    c->gy, W, c->d_qmask);
+        hipLaunchKernelGGL(k_mask3, dim3(grid_for((uint64_t)c->cells * W, 1u << 30)), dim3(kBlock), 0, s,
+                           c->d_cellmask, c->gx, c->gy, W, c->d_mask3);
     }
     COLO_HIP(hipMemcpyAsync(c->h_bin, small, 8 * (3 + 3ull * c->F), hipMemcpyDeviceToHost, s));
     COLO_HIP(hipGetLastError());
@@ -2983,10 +3076,10 @@ static int bin_impl(colo_ctx *c, hipStream_t s) {
 // the r-dependent buffers.
 static int set_grid(colo_ctx *c, float r) {
     free_bin_bufs(c);
-    void *old[] = {c->d_dir, c->d_cellmask, c->d_qmask, c->d_run_cell};
+    void *old[] = {c->d_dir, c->d_cellmask, c->d_qmask, c->d_mask3, c->d_run_cell};
     for (void *p : old)
         if (p) (void)hipFree(p);
-    c->d_dir = nullptr; c->d_cellmask = nullptr; c->d_qmask = nullptr; c->d_run_cell = nullptr;
+    c->d_dir = nullptr; c->d_cellmask = nullptr; c->d_qmask = nullptr; c->d_mask3 = nullptr; c->d_run_cell = nullptr;
     c->r = r;
     volatile float r2 = r * r;
     c->r2 = r2;
@@ -3011,6 +3104,7 @@ static int set_grid(colo_ctx *c, float r) {
         uint64_t nq = (c->gx >= 2 && c->gy >= 2) ? (uint64_t)(c->gx - 1) * (c->gy - 1) : 0;
         rc = alloc_dev(c, (void **)&c->d_cellmask, 8ull * c->cells * W);
         if (!rc) rc = alloc_dev(c, (void **)&c->d_qmask, 8 * (nq ? nq : 1) * W);
+        if (!rc) rc = alloc_dev(c, (void **)&c->d_mask3, 8ull * c->cells * W);
     }
     return rc;
 }
@@ -3101,7 +3195,8 @@ int colo_ctx_destroy(colo_ctx *c) {
     if (!c) return COLO_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->d_x, c->d_y, c->d_sxy, c->d_cell_of, c->d_sidx, c->d_spos, c->d_scell, c->d_dir,
-                    c->d_fstart, c->d_fcount, c->d_cellmask, c->d_qmask, c->d_run_cell, c->d_flag, c->d_scratch};
+                    c->d_fstart, c->d_fcount, c->d_cellmask, c->d_qmask, c->d_mask3, c->d_run_cell, c->d_flag,
+                    c->d_scratch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);

@@ -40,6 +40,10 @@ struct DevData {
     const uint32_t *dir;           // F x (cells+1): first sorted position of (f, c)
     const uint32_t *fstart;        // F
     const uint32_t *fcount;        // F
+    // presence of each feature in the clamped 3 x 3 block of cells around
+    // each cell (mask_w words per cell, bit f of word f / 64), or null
+    const uint64_t *mask3;
+    uint32_t mask_w;
 };
 
 // slots_off of a candidate whose count pass keeps no per-row slots (its
@@ -240,7 +244,7 @@ struct colo_ctx {
     uint64_t dir_len = 0;
     uint32_t *d_fstart = nullptr, *d_fcount = nullptr;
     // filter support
-    uint64_t *d_cellmask = nullptr, *d_qmask = nullptr;
+    uint64_t *d_cellmask = nullptr, *d_qmask = nullptr, *d_mask3 = nullptr;
     uint32_t *d_run_cell = nullptr;
     uint64_t H = 0;
     std::vector<uint64_t> run_off;  // F+1

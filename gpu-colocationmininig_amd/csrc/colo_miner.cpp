@@ -1055,9 +1055,13 @@ struct Miner {
     // step): every (count variant, item size) pair is timed with HIP events
     // on a sample of the level's work items -- blocks of 16 consecutive
     // items, one block in 8, so neighbouring items still share their L2 --
-    // and the fastest pair is kept for the level in c->tune.  The sample
-    // launches write into the level's slots, bits and row totals, which the
-    // real pass clears and rewrites: results do not depend on the tuning.
+    // and the pairs within 10% of the fastest sample (at most four) are
+    // timed again on all of the level's items -- a sample of about one item
+    // per workgroup measures its slowest item more than the variant's
+    // throughput -- and the fastest of those is kept for the level in
+    // c->tune.  The tuning launches write into the level's slots, bits and
+    // row totals, which the real pass clears and rewrites: results do not
+    // depend on the tuning.
     bool tuning_ = false;
     int tune_level(uint32_t k, std::vector<WorkItem> &items, std::vector<double> &iw, bool has_pred,
                    const std::function<void(uint32_t)> &build, uint32_t *d_slots, uint64_t bits_tot, uint64_t m) {
@@ -1072,7 +1076,26 @@ struct Miner {
         }
         double best = 1e300;
         uint32_t bv = c->tune.count_var_of(k), bs = c->tune.item_rows_of(k);
+        struct Sample {
+            float ms;
+            uint32_t var, rows;
+        };
+        std::vector<Sample> samples;
         int rc = COLO_OK;
+        // one count launch over the uploaded items (n of them), in ms
+        auto timed = [&](size_t n, float &ms) {
+            int r = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
+            if (!r) r = fill_u32(d_rowsel.as<uint32_t>(), 2 * m, 0, s);
+            if (r) return r;
+            if (hipEventRecord(ev[0], s) != hipSuccess) return set_error(c, COLO_EHIP, "event");
+            r = count_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, n, d_slots,
+                            arena_bits.as<uint32_t>(), d_rowsel.as<uint64_t>(), d_queue.as<uint32_t>(), s);
+            if (r) return r;
+            if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess)
+                return set_error(c, COLO_EHIP, "tuning launch");
+            return COLO_OK;
+        };
         for (uint32_t si = 0; si < 3 && !rc; si++) {
             build(sizes[si]);
             std::vector<WorkItem> sub;
@@ -1089,29 +1112,40 @@ struct Miner {
                 // varies by several percent)
                 float ms = 1e30f;
                 for (int rep = 0; rep < 2 && !rc; rep++) {
-                    rc = fill_u32(arena_bits.as<uint32_t>(), bits_tot, 0, s);
-                    if (!rc) rc = fill_u32(d_rowsel.as<uint32_t>(), 2 * m, 0, s);
-                    if (rc) break;
                     float one = 0;
-                    if (hipEventRecord(ev[0], s) != hipSuccess) { rc = set_error(c, COLO_EHIP, "event"); break; }
-                    rc = count_batch(c, k, d_desc.as<CandDesc>(), d_items.as<WorkItem>(), d_seg_, sub.size(), d_slots,
-                                     arena_bits.as<uint32_t>(), d_rowsel.as<uint64_t>(), d_queue.as<uint32_t>(), s);
-                    if (rc) break;
-                    if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
-                        hipEventElapsedTime(&one, ev[0], ev[1]) != hipSuccess) {
-                        rc = set_error(c, COLO_EHIP, "tuning launch");
-                        break;
-                    }
-                    ms = std::min(ms, one);
+                    if (!(rc = timed(sub.size(), one))) ms = std::min(ms, one);
                 }
                 if (rc) break;
                 if (c->verbose)
                     fprintf(stderr, "[colo tune] level %u count var %u, %u-row items: %.2f ms on %zu of %zu items\n", k,
                             k > 2 ? vars[vi] : 0u, sizes[si], ms, sub.size(), items.size());
+                samples.push_back(Sample{ms, k > 2 ? vars[vi] : bv, sizes[si]});
                 if (ms < best) {
                     best = ms;
-                    bv = vars[vi];
+                    bv = k > 2 ? vars[vi] : bv;
                     bs = sizes[si];
+                }
+            }
+        }
+        // the close samples again, on every item of the level
+        std::stable_sort(samples.begin(), samples.end(), [](const Sample &a, const Sample &b) { return a.ms < b.ms; });
+        size_t nclose = 0;
+        while (nclose < samples.size() && nclose < 4 && samples[nclose].ms <= 1.1 * best) nclose++;
+        if (!rc && nclose > 1) {
+            float full_best = 1e30f;
+            for (size_t i = 0; i < nclose && !rc; i++) {
+                build(samples[i].rows);
+                if ((rc = upload_items(items, has_pred ? &iw : nullptr))) break;
+                if (k > 2) c->tune.count_var_k[k] = samples[i].var;
+                float ms = 0;
+                if ((rc = timed(items.size(), ms))) break;
+                if (c->verbose)
+                    fprintf(stderr, "[colo tune] level %u count var %u, %u-row items: %.2f ms on all %zu items\n", k,
+                            samples[i].var, samples[i].rows, ms, items.size());
+                if (ms < full_best) {
+                    full_best = ms;
+                    bv = samples[i].var;
+                    bs = samples[i].rows;
                 }
             }
         }

@@ -1141,13 +1141,15 @@ __device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F 
 }
 
 // Batched level-2 count of anchor rank t of feature a (join2_count_anchor
-// with SORTED, entries walked by join2_stream).
+// with SORTED, entries walked by join2_stream; MASK: test the anchor cell's
+// 3 x 3 mask first).
+template <bool MASK = true>
 __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32_t b, uint32_t t, uint32_t *slots,
                                             uint32_t *bits_a, uint32_t *bits_b) {
     const uint32_t sb = D.fstart[b];
     uint32_t cnt = 0;
     BitAcc acc(bits_b);
-    join2_stream<true>(
+    join2_stream<MASK>(
         D, b, D.fstart[a] + t,
         [&](uint32_t q, bool hit) {
             if (hit) {
@@ -2302,11 +2304,48 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
     }
 }
 
-// Level 2: one anchor per thread over the item's anchor range; the item's
-// rows are summed in the workgroup and added to the candidate's total.
+// Runs f(t) for the anchors t of [begin, end) that this wavefront owns (its
+// 64-anchor chunks, one every kBlock anchors) and for which keep(t) holds,
+// with those anchors packed onto the wave's lanes: they are queued in LDS
+// (q: 128 entries for this wave) and run 64 at a time, the last < 64 at the
+// end; skip(t) is called for the others.  Every lane of the wave calls it
+// with the same range.
+template <typename P, typename S, typename F>
+__device__ inline void wave_packed(uint64_t begin, uint64_t end, uint32_t *q, P &&keep, S &&skip, F &&f) {
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = __lanemask_lt();
+    uint32_t qn = 0;  // wave-uniform
+    for (uint64_t i0 = begin + (threadIdx.x & ~63u); i0 < end; i0 += kBlock) {
+        const uint64_t t = i0 + lane;
+        const bool valid = t < end;
+        const bool kp = valid && keep((uint32_t)t);
+        if (valid && !kp) skip((uint32_t)t);
+        const uint64_t bal = __ballot(kp);
+        if (kp) q[qn + (uint32_t)__popcll(bal & lt)] = (uint32_t)t;
+        qn += (uint32_t)__popcll(bal);
+        __builtin_amdgcn_wave_barrier();
+        if (qn >= 64) {
+            const uint32_t mine = q[lane];
+            const uint32_t rest = lane < qn - 64 ? q[64 + lane] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < qn - 64) q[lane] = rest;
+            qn -= 64;
+            __builtin_amdgcn_wave_barrier();
+            f(mine);
+        }
+    }
+    if (lane < qn) f(q[lane]);
+}
+
+// Level 2: the item's anchor range over the block's lanes; the item's rows
+// are summed in the workgroup and added to the candidate's total.  VAR 0:
+// one anchor per thread; VAR 1: the anchors whose cell's 3 x 3 mask holds b
+// packed onto the wave's lanes (wave_packed), the others given zero slots.
+template <int VAR>
 __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDesc *desc, const WorkItem *items,
                                                          const uint64_t *seg, uint32_t *slots, uint32_t *bits,
                                                          unsigned long long *rows, uint32_t *queue) {
+    __shared__ uint32_t s_q[VAR == 1 ? kBlock / 64 : 1][128];
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
@@ -2315,8 +2354,22 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
         uint32_t *bt = bits + cd.bits_off;
         const uint32_t a = cd.f[0], b = cd.f[1];
         uint64_t sum = 0;
-        for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-            sum += join2_count_flat(D, a, b, (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+        if (VAR == 1) {
+            const uint32_t sa = D.fstart[a];
+            wave_packed(
+                wi.begin, wi.end, s_q[VAR == 1 ? threadIdx.x >> 6 : 0],
+                [&](uint32_t t) {
+                    return !D.mask3 ||
+                           ((D.mask3[(uint64_t)D.scell[sa + t] * D.mask_w + (b >> 6)] >> (b & 63)) & 1);
+                },
+                [&](uint32_t t) {
+                    if (sl) sl[t] = 0;
+                },
+                [&](uint32_t t) { sum += join2_count_flat<false>(D, a, b, t, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]); });
+        } else {
+            for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
+                sum += join2_count_flat(D, a, b, (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+        }
         uint64_t ex;
         const uint64_t tot = block_excl_scan(sum, ex);
         if (threadIdx.x == 0 && tot) atomicAdd(&rows[wi.cand], (unsigned long long)tot);
@@ -2604,8 +2657,9 @@ int colo::count_batch(colo_ctx *c, uint32_t k, const CandDesc *d_desc, const Wor
     int rc = fill_u32(d_queue, kXcds, 0, s);
     if (rc) return rc;
     if (k == 2)
-        hipLaunchKernelGGL(k_count2_batch, dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc,
-                           d_items, d_seg, d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
+        hipLaunchKernelGGL(c->tune.count_var_of(2) % 2 ? k_count2_batch<1> : k_count2_batch<0>,
+                           dim3(persistent_grid(c, n_items)), dim3(kBlock), 0, s, c->dev(), d_desc, d_items, d_seg,
+                           d_slots, d_bits, (unsigned long long *)d_rows, d_queue);
     else {
         const uint32_t var = c->tune.count_var_of(k);
         hipLaunchKernelGGL(var == 13   ? pick_k<PickCount13>(k)

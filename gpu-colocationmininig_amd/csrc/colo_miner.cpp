@@ -1066,8 +1066,10 @@ struct Miner {
     int tune_level(uint32_t k, std::vector<WorkItem> &items, std::vector<double> &iw, bool has_pred,
                    const std::function<void(uint32_t)> &build, uint32_t *d_slots, uint64_t bits_tot, uint64_t m) {
         if (k >= 16) return COLO_OK;
-        static const uint32_t vars[] = {0, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13}, sizes[] = {1024, 4096, 16384};
-        const uint32_t nv = k == 2 ? 1 : 12;  // level 2 has one count kernel
+        static const uint32_t vars[] = {0, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13}, vars2[] = {0, 1},
+                              sizes[] = {1024, 4096, 16384};
+        const uint32_t nv = k == 2 ? 2 : 12;  // level 2: one anchor per lane, or packed anchors
+        const uint32_t *vs = k == 2 ? vars2 : vars;
         hipEvent_t ev[2];
         COLO_HIP(hipEventCreate(&ev[0]));
         if (hipEventCreate(&ev[1]) != hipSuccess) {
@@ -1107,7 +1109,7 @@ struct Miner {
                 }
             if ((rc = upload_items(sub, has_pred ? &subw : nullptr))) break;
             for (uint32_t vi = 0; vi < nv && !rc; vi++) {
-                if (k > 2) c->tune.count_var_k[k] = vars[vi];
+                c->tune.count_var_k[k] = vs[vi];
                 // the faster of two timed launches (one launch of a sample
                 // varies by several percent)
                 float ms = 1e30f;
@@ -1118,11 +1120,11 @@ struct Miner {
                 if (rc) break;
                 if (c->verbose)
                     fprintf(stderr, "[colo tune] level %u count var %u, %u-row items: %.2f ms on %zu of %zu items\n", k,
-                            k > 2 ? vars[vi] : 0u, sizes[si], ms, sub.size(), items.size());
-                samples.push_back(Sample{ms, k > 2 ? vars[vi] : bv, sizes[si]});
+                            vs[vi], sizes[si], ms, sub.size(), items.size());
+                samples.push_back(Sample{ms, vs[vi], sizes[si]});
                 if (ms < best) {
                     best = ms;
-                    bv = k > 2 ? vars[vi] : bv;
+                    bv = vs[vi];
                     bs = sizes[si];
                 }
             }
@@ -1136,7 +1138,7 @@ struct Miner {
             for (size_t i = 0; i < nclose && !rc; i++) {
                 build(samples[i].rows);
                 if ((rc = upload_items(items, has_pred ? &iw : nullptr))) break;
-                if (k > 2) c->tune.count_var_k[k] = samples[i].var;
+                c->tune.count_var_k[k] = samples[i].var;
                 float ms = 0;
                 if ((rc = timed(items.size(), ms))) break;
                 if (c->verbose)
@@ -1152,7 +1154,7 @@ struct Miner {
         (void)hipEventDestroy(ev[0]);
         (void)hipEventDestroy(ev[1]);
         if (rc) return rc;
-        if (k > 2) c->tune.count_var_k[k] = bv;
+        c->tune.count_var_k[k] = bv;
         c->tune.item_rows_k[k] = bs;
         return COLO_OK;
     }

@@ -299,14 +299,19 @@ int colo_graph_info(const colo_ctx *ctx, uint32_t *captures, uint32_t *replays);
  *                         capped for 8 waves per SIMD, 7 and 9 = 6 and 8
  *                         capped for 5; 10-13 (count only) = 4, 9, 0 and 8
  *                         with each work item's participation bits staged
- *                         in LDS (k = 3..8); the autotuner tries 0 and 3-13
+ *                         in LDS (k = 3..8); the autotuner tries 0 and 3-13.
+ *                         The level-2 count takes the variant's parity:
+ *                         even = one anchor per thread, odd = the anchors
+ *                         whose cell's 3x3 block holds the second feature
+ *                         packed onto the wavefront's lanes (the others
+ *                         have no rows); the autotuner tries both
  *   blocks_per_cu         persistent workgroups per compute unit (1..32)
  *   item_rows             rows per work-queue item; with the cost model,
  *                         items are cut at equal predicted cost between
  *                         item_rows / 4 and 8 x item_rows (64..65536) */
 typedef struct colo_tune {
     uint32_t count_var, write_var, blocks_per_cu, item_rows;
-    /* count variant of level k (index k, 3..15); COLO_TUNE_UNSET = count_var */
+    /* count variant of level k (index k, 2..15); COLO_TUNE_UNSET = count_var */
     uint32_t count_var_k[16];
     /* item rows of level k (index k, 2..15); COLO_TUNE_UNSET = item_rows */
     uint32_t item_rows_k[16];

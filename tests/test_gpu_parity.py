@@ -70,6 +70,45 @@ def test_binning(fx, request):
     np.testing.assert_array_equal(d, expect.astype(np.uint32))
 
 
+@pytest.mark.parametrize("case", ["one", "corners", "last_cell", "dense_cell"])
+def test_directory_edge_cases(case, colo_mod, oracle_mod):
+    """The directory on sparse layouts: a single instance, instances in the
+    grid's corners (gaps of many 4096-entry tiles between run starts), every
+    instance of a feature in the last cell, one cell holding every instance."""
+    _say(f"directory {case}")
+    S, r = 4000.0, 10.0
+    if case == "one":
+        x, y, fc = np.array([1234.5], np.float32), np.array([777.0], np.float32), np.array([1], np.uint32)
+    elif case == "corners":
+        x = np.array([0.5, S - 0.5, 0.5, S - 0.5, 2000.0], np.float32)
+        y = np.array([0.5, 0.5, S - 0.5, S - 0.5, 2000.0], np.float32)
+        fc = np.array([2, 1, 2], np.uint32)
+    elif case == "last_cell":
+        x = np.array([5.0, S - 0.1, S - 0.2, S - 0.3], np.float32)
+        y = np.array([5.0, S - 0.1, S - 0.2, S - 0.3], np.float32)
+        fc = np.array([1, 3], np.uint32)
+    else:
+        rng = np.random.default_rng(3)
+        x = rng.uniform(1000.0, 1009.0, 600).astype(np.float32)
+        y = rng.uniform(1000.0, 1009.0, 600).astype(np.float32)
+        fc = np.array([300, 200, 100], np.uint32)
+    b = np.array([0, 0, S, S], np.float32)
+    ctx = colo_mod.Context(x, y, fc, b, r)
+    orc = oracle_mod.Oracle.from_arrays(x, y, fc, b)
+    orc.bin(r)
+    cells = ctx.grid()["cells"]
+    cell, sidx, d = ctx.binning()
+    cid, order, fid = _binning_expect(orc, fc)
+    np.testing.assert_array_equal(cell, cid)
+    np.testing.assert_array_equal(sidx, order)
+    key = fid.astype(np.int64) * (cells + 1) + cid
+    expect = np.searchsorted(np.sort(key), np.arange(len(fc) * (cells + 1)), side="left")
+    np.testing.assert_array_equal(d, expect.astype(np.uint32))
+    rep, _ = ctx.mine(0.1, max_k=3)
+    assert rep == orc.mine(r, 0.1, max_k=3)
+    ctx.close()
+
+
 @pytest.mark.parametrize("fx", ["c1", "c2small"])
 def test_filter_level2(fx, request):
     ctx, orc, p = request.getfixturevalue(fx)

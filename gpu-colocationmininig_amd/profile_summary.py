@@ -14,7 +14,8 @@
            [--match k_count,k_write] [--min-ms 1]
       Markdown table, one line per dispatch of the matching kernels lasting
       at least min-ms, of the counter runs of one deterministic program (the
-      same dispatch ids in every run): duration, TA / TD busy and TD stalls
+      n-th dispatch of a kernel is the same work in every run; dispatch ids
+      may differ): duration, TA / TD busy and TD stalls
       (per unit and cycle, 256 units), L2 hit rate, VALU and vector-read
       instructions, VALU busy (4 cycles per instruction, 1024 SIMDs) and HBM
       bytes (traffic's copy-calibrated scales).  Cycles per XCD =
@@ -70,32 +71,49 @@ def _counter_per_dispatch(path, counter, kernel):
     return sum(per.values()) / len(per), len(per)
 
 
-def _per_dispatch(path):
-    """{dispatch id: (kernel name, {counter: value summed over dimensions})}."""
+def _ordinal_keys(pairs):
+    """(kernel name, n) for each (dispatch id, kernel name), n counting that
+    kernel's dispatches in id order."""
+    seen = collections.Counter()
     out = {}
+    for did, name in sorted(pairs, key=lambda x: int(x[0])):
+        out[did] = (name, seen[name])
+        seen[name] += 1
+    return out
+
+
+def _per_dispatch(path):
+    """{(kernel name, n): {counter: value summed over dimensions}} for the
+    n-th dispatch of each kernel."""
+    vals, names = {}, {}
     with open(path) as fh:
         for r in csv.DictReader(fh):
-            d = out.setdefault(r["Dispatch_Id"], (r["Kernel_Name"], collections.defaultdict(float)))
-            d[1][r["Counter_Name"]] += float(r["Counter_Value"])
-    return out
+            names[r["Dispatch_Id"]] = r["Kernel_Name"]
+            vals.setdefault(r["Dispatch_Id"], collections.defaultdict(float))[r["Counter_Name"]] += float(r["Counter_Value"])
+    keys = _ordinal_keys(names.items())
+    return {keys[d]: v for d, v in vals.items()}
 
 
 def counters(a):
     dur = {}
     with open(a.trace) as fh:
-        for r in csv.DictReader(fh):
-            dur[r["Dispatch_Id"]] = (r["Kernel_Name"], (float(r["End_Timestamp"]) - float(r["Start_Timestamp"])) / 1e6)
+        disp = [(r["Dispatch_Id"], r["Kernel_Name"], (float(r["End_Timestamp"]) - float(r["Start_Timestamp"])) / 1e6)
+                for r in csv.DictReader(fh)]
+    keys = _ordinal_keys([(d, n) for d, n, _ in disp])
+    for d, n, ms in disp:
+        dur[keys[d]] = (n, ms)
     merged = collections.defaultdict(dict)
     for p in a.pmc + [a.fetch, a.write]:
-        for did, (_, cs) in _per_dispatch(p).items():
-            merged[did].update(cs)
+        for key, cs in _per_dispatch(p).items():
+            merged[key].update(cs)
     cf, _ = _counter_per_dispatch(a.cfetch, "FETCH_SIZE", "__amd_rocclr_copyBuffer")
     cw, _ = _counter_per_dispatch(a.cwrite, "WRITE_SIZE", "__amd_rocclr_copyBuffer")
     fs, ws = a.copy_bytes / (cf * 1024.0), a.copy_bytes / (cw * 1024.0)
     pats = a.match.split(",")
     rows = ["| kernel | ms | TA busy | TD busy | TD stalled on L1 | L2 hit | VALU instr | VMEM read instr | VALU busy | "
             "HBM read GB | HBM write GB |", "|---|---|---|---|---|---|---|---|---|---|---|"]
-    for did in sorted(dur, key=lambda x: int(x)):
+    order = {keys[d]: int(d) for d, _, _ in disp}
+    for did in sorted(dur, key=lambda x: order[x]):
         name, ms = dur[did]
         short = name.split("(")[0].replace("void ", "").strip()
         if ms < a.min_ms or not any(short.startswith(p) for p in pats) or did not in merged:

@@ -653,8 +653,15 @@ __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, co
     const uint64_t r0 = G.run_begin + (b - chunk_prefix[lo]) * kFilterRuns;
     const uint64_t r1 = r0 + kFilterRuns < G.run_end ? r0 + kFilterRuns : G.run_end;
     const uint32_t *dirf = dir + (uint64_t)G.f * (g.cells + 1);
-    const bool mine = threadIdx.x < G.n_memb;
-    const uint32_t me = mine ? memb[G.memb_off + threadIdx.x] : 0;
+    // with at most 64 (128) members, four (two) copies of the member list
+    // each test a quarter (half) of the staged cells, so that every wave
+    // works; each copy adds its partial count
+    const uint32_t nw = G.n_memb > 64 ? (G.n_memb + 63) / 64 : 1;
+    const uint32_t copies = nw == 1 ? 4 : nw == 2 ? 2 : 1;
+    const uint32_t span = 64 * nw, copy = threadIdx.x / span, mi = threadIdx.x - copy * span;
+    const bool mine = copy < copies && mi < G.n_memb;
+    const uint32_t me = mine ? memb[G.memb_off + mi] : 0;
+    const uint32_t j0 = copy * (kBlock / copies);
     uint64_t M[W];
 #pragma unroll
     for (int w = 0; w < W; w++) M[w] = mine ? cmask[(uint64_t)(me / k) * W + w] : 0;
@@ -690,7 +697,8 @@ __global__ __launch_bounds__(kBlock) void k_filter(const FilterGroup *groups, co
         __syncthreads();
         if (mine) {
             const uint32_t cnt = (uint32_t)(r1 - base < kBlock ? r1 - base : kBlock);
-            for (uint32_t j = 0; j < cnt; j++) {
+            const uint32_t j1 = j0 + kBlock / copies < cnt ? j0 + kBlock / copies : cnt;
+            for (uint32_t j = j0; j < j1; j++) {
                 bool ok = false;
 #pragma unroll
                 for (int dq = 0; dq < 4; dq++) {

@@ -178,24 +178,38 @@ struct ScanTile {
     uint64_t start;
 };
 
+// Global loads and stores of a tile are striped (item r * kBlock + thread,
+// consecutive lanes on consecutive items); each thread's kScanItems
+// consecutive items are exchanged through LDS.  The padded index keeps the
+// contiguous reads off one bank.
+__device__ inline uint32_t scan_pad(uint32_t i) { return i + (i >> 5); }
+
 __global__ __launch_bounds__(kBlock) void k_segscan_reduce(const ScanSegDev *segs, const ScanTile *tiles,
                                                            uint64_t *partial) {
     const ScanTile t = tiles[blockIdx.x];
     const ScanSegDev sg = segs[t.seg];
-    const uint64_t base = t.start + (uint64_t)threadIdx.x * kScanItems;
+    const uint64_t end = sg.n < t.start + kScanTile ? sg.n : t.start + kScanTile;
     uint64_t v = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; i++)
-        if (base + i < sg.n && base + i < t.start + kScanTile) v += sg.in[base + i];
     if (sg.chunked) {
-        // sums of 64 rows (8 threads of 8 items), kept in out until
-        // k_segscan_apply_chunks turns them into prefixes
-        uint64_t g = v;
-        g += __shfl_xor(g, 1, 64);
-        g += __shfl_xor(g, 2, 64);
-        g += __shfl_xor(g, 4, 64);
-        const uint64_t ch = (t.start >> 6) + (threadIdx.x >> 3);
-        if ((threadIdx.x & 7) == 0 && ch < (sg.n + 63) >> 6) sg.out[ch] = g;
+        // sums of 64 rows: chunk r * 4 + wave is row r of the striped tile,
+        // summed over the wave; kept in out until k_segscan_apply_chunks
+        // turns them into prefixes
+#pragma unroll
+        for (int r = 0; r < kScanItems; r++) {
+            const uint64_t idx = t.start + (uint64_t)r * kBlock + threadIdx.x;
+            uint64_t g = idx < end ? sg.in[idx] : 0u;
+            v += g;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) g += __shfl_xor(g, d, 64);
+            const uint64_t ch = (t.start >> 6) + (uint64_t)r * (kBlock / 64) + (threadIdx.x >> 6);
+            if ((threadIdx.x & 63) == 0 && ch < (sg.n + 63) >> 6) sg.out[ch] = g;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kScanItems; r++) {
+            const uint64_t idx = t.start + (uint64_t)r * kBlock + threadIdx.x;
+            if (idx < end) v += sg.in[idx];
+        }
     }
     uint64_t ex;
     const uint64_t tot = block_excl_scan(v, ex);
@@ -245,15 +259,24 @@ __global__ __launch_bounds__(kBlock) void k_segscan_apply_chunks(const ScanSegDe
 
 __global__ __launch_bounds__(kBlock) void k_segscan_apply(const ScanSegDev *segs, const ScanTile *tiles,
                                                           const uint64_t *carry) {
+    __shared__ uint32_t s_in[kScanTile + kScanTile / 32];
+    __shared__ uint64_t s_out[kScanTile + kScanTile / 32];
     const ScanTile t = tiles[blockIdx.x];
     const ScanSegDev sg = segs[t.seg];
     if (sg.chunked) return;  // the whole workgroup: one tile, one segment
-    const uint64_t base = t.start + (uint64_t)threadIdx.x * kScanItems;
+    const uint64_t end = sg.n < t.start + kScanTile ? sg.n : t.start + kScanTile;
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) {
+        const uint32_t e = (uint32_t)r * kBlock + threadIdx.x;
+        const uint64_t idx = t.start + e;
+        s_in[scan_pad(e)] = idx < end ? sg.in[idx] : 0u;
+    }
+    __syncthreads();
     uint64_t v[kScanItems];
     uint64_t sum = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
-        v[i] = (base + i < sg.n) ? (uint64_t)sg.in[base + i] : 0;
+        v[i] = s_in[scan_pad(threadIdx.x * kScanItems + i)];
         sum += v[i];
     }
     uint64_t ex;
@@ -261,8 +284,15 @@ __global__ __launch_bounds__(kBlock) void k_segscan_apply(const ScanSegDev *segs
     uint64_t run = ex + carry[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
-        if (base + i < sg.n) sg.out[base + i] = run;
+        s_out[scan_pad(threadIdx.x * kScanItems + i)] = run;
         run += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) {
+        const uint32_t e = (uint32_t)r * kBlock + threadIdx.x;
+        const uint64_t idx = t.start + e;
+        if (idx < end) sg.out[idx] = s_out[scan_pad(e)];
     }
 }
 

@@ -1180,7 +1180,8 @@ __device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F 
 
 // Batched level-2 count of anchor rank t of feature a (join2_count_anchor
 // with SORTED, entries walked by join2_stream; MASK: test the anchor cell's
-// 3 x 3 mask first).
+// 3 x 3 mask first).  bits_a null: the anchor's own bit is left to the
+// caller.
 template <bool MASK = true>
 __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32_t b, uint32_t t, uint32_t *slots,
                                             uint32_t *bits_a, uint32_t *bits_b) {
@@ -1198,7 +1199,7 @@ __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32
         [] { return true; });
     acc.flush();
     if (slots) slots[t] = cnt;
-    if (cnt) set_bit(bits_a, t);
+    if (cnt && bits_a) set_bit(bits_a, t);
     return cnt;
 }
 
@@ -2342,12 +2343,14 @@ __device__ void wave_rows_multi(const DevData &D, const CandDesc *descs, uint32_
     }
 }
 
-// Runs f(t) for the anchors t of [begin, end) that this wavefront owns (its
-// 64-anchor chunks, one every kBlock anchors) and for which keep(t) holds,
-// with those anchors packed onto the wave's lanes: they are queued in LDS
-// (q: 128 entries for this wave) and run 64 at a time, the last < 64 at the
-// end; skip(t) is called for the others.  Every lane of the wave calls it
-// with the same range.
+// Runs f(t, true) for the anchors t of [begin, end) that this wavefront
+// owns (its 64-anchor chunks, one every kBlock anchors) and for which
+// keep(t) holds, with those anchors packed onto the wave's lanes in
+// ascending order: they are queued in LDS (q: 128 entries for this wave)
+// and run 64 at a time, the last < 64 at the end, where the lanes without
+// one call f(0, false) (f is always called by the whole wave); skip(t) is
+// called for the others.  Every lane of the wave calls it with the same
+// range.
 template <typename P, typename S, typename F>
 __device__ inline void wave_packed(uint64_t begin, uint64_t end, uint32_t *q, P &&keep, S &&skip, F &&f) {
     const uint32_t lane = __lane_id();
@@ -2369,10 +2372,10 @@ __device__ inline void wave_packed(uint64_t begin, uint64_t end, uint32_t *q, P 
             if (lane < qn - 64) q[lane] = rest;
             qn -= 64;
             __builtin_amdgcn_wave_barrier();
-            f(mine);
+            f(mine, true);
         }
     }
-    if (lane < qn) f(q[lane]);
+    if (qn) f(lane < qn ? q[lane] : 0u, lane < qn);
 }
 
 // Level 2: the item's anchor range over the block's lanes; the item's rows
@@ -2403,7 +2406,14 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
                 [&](uint32_t t) {
                     if (sl) sl[t] = 0;
                 },
-                [&](uint32_t t) { sum += join2_count_flat<false>(D, a, b, t, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]); });
+                [&](uint32_t t, bool valid) {
+                    const uint32_t cnt =
+                        valid ? join2_count_flat<false>(D, a, b, t, sl, nullptr, bt + cd.seg_off[1]) : 0u;
+                    sum += cnt;
+                    // the anchors' own bits, one check and atomic per word
+                    // (the packed anchors ascend over the lanes)
+                    wave_set_bits(bt + cd.seg_off[0], t, valid, cnt != 0);
+                });
         } else {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
                 sum += join2_count_flat(D, a, b, (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);

@@ -937,6 +937,46 @@ struct BitAcc {
     }
 };
 
+// BitAcc whose words are first looked up in a workgroup's LDS table of
+// words known to hold bits (cache, kBitCache entries of (arena word, mask),
+// direct-mapped): bits found there need no check and no atomic.  An entry
+// only ever holds bits that were seen set or whose atomicOr was issued, so a
+// stale or overwritten entry loses hits, never bits; 64-bit LDS stores keep
+// each (word, mask) pair whole.
+constexpr uint32_t kBitCache = 256;
+struct BitAccC {
+    uint32_t *bits;
+    const uint32_t *arena;
+    unsigned long long *cache;
+    uint64_t word = ~0ull;
+    uint32_t mask = 0;
+    __device__ BitAccC(uint32_t *b, const uint32_t *a, unsigned long long *c) : bits(b), arena(a), cache(c) {}
+    __device__ inline void flush() {
+        if (mask) {
+            uint32_t *w = bits + word;
+            const uint64_t off = (uint64_t)(w - arena);
+            const uint32_t aw = (uint32_t)off;
+            // words past 2^32 - 1 of the arena bypass the table (keys are 32-bit)
+            unsigned long long *e = cache && off < 0xffffffffull ? &cache[aw % kBitCache] : nullptr;
+            const unsigned long long cur = e ? *e : 0ull;
+            if (!e || (uint32_t)cur != aw || ((uint32_t)(cur >> 32) & mask) != mask) {
+                const uint32_t seen = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((seen & mask) != mask) atomicOr(w, mask);
+                if (e) *e = ((unsigned long long)(seen | mask) << 32) | aw;
+            }
+        }
+        mask = 0;
+    }
+    __device__ inline void set(uint64_t i) {
+        uint64_t w = i >> 5;
+        if (w != word) {
+            flush();
+            word = w;
+        }
+        mask |= 1u << (i & 31);
+    }
+};
+
 // Level-2 work estimate of a (first, last) feature pair: over the non-empty
 // cells c of `first`, count(first, c) x (instances of `last` in the 3x3
 // cells around c) -- the number of distance tests of the 3x3 join.  Cells
@@ -1180,14 +1220,13 @@ __device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F 
 
 // Batched level-2 count of anchor rank t of feature a (join2_count_anchor
 // with SORTED, entries walked by join2_stream; MASK: test the anchor cell's
-// 3 x 3 mask first).  bits_a null: the anchor's own bit is left to the
-// caller.
-template <bool MASK = true>
+// 3 x 3 mask first).  acc collects the bits of feature b; bits_a null: the
+// anchor's own bit is left to the caller.
+template <bool MASK = true, typename Acc = BitAcc>
 __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32_t b, uint32_t t, uint32_t *slots,
-                                            uint32_t *bits_a, uint32_t *bits_b) {
+                                            uint32_t *bits_a, Acc acc) {
     const uint32_t sb = D.fstart[b];
     uint32_t cnt = 0;
-    BitAcc acc(bits_b);
     join2_stream<MASK>(
         D, b, D.fstart[a] + t,
         [&](uint32_t q, bool hit) {
@@ -2387,6 +2426,10 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
                                                          const uint64_t *seg, uint32_t *slots, uint32_t *bits,
                                                          unsigned long long *rows, uint32_t *queue) {
     __shared__ uint32_t s_q[VAR == 1 ? kBlock / 64 : 1][128];
+    __shared__ unsigned long long s_bc[VAR == 1 ? kBitCache : 1];
+    if (VAR == 1)
+        for (uint32_t i = threadIdx.x; i < kBitCache; i += kBlock) s_bc[i] = 0xffffffffull;
+    __syncthreads();
     uint64_t it;
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
@@ -2408,7 +2451,9 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
                 },
                 [&](uint32_t t, bool valid) {
                     const uint32_t cnt =
-                        valid ? join2_count_flat<false>(D, a, b, t, sl, nullptr, bt + cd.seg_off[1]) : 0u;
+                        valid ? join2_count_flat<false>(D, a, b, t, sl, nullptr,
+                                                        BitAccC(bt + cd.seg_off[1], bits, s_bc))
+                              : 0u;
                     sum += cnt;
                     // the anchors' own bits, one check and atomic per word
                     // (the packed anchors ascend over the lanes)
@@ -2416,7 +2461,7 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
                 });
         } else {
             for (uint64_t i = wi.begin + threadIdx.x; i < wi.end; i += kBlock)
-                sum += join2_count_flat(D, a, b, (uint32_t)i, sl, bt + cd.seg_off[0], bt + cd.seg_off[1]);
+                sum += join2_count_flat(D, a, b, (uint32_t)i, sl, bt + cd.seg_off[0], BitAcc(bt + cd.seg_off[1]));
         }
         uint64_t ex;
         const uint64_t tot = block_excl_scan(sum, ex);

@@ -525,10 +525,12 @@ __global__ void k_dir_tiles(const K *skeys, uint32_t n, uint32_t cellbits, uint3
         for (uint64_t t = d / kDirTile + 1; t <= ntiles; t++) tile_lo[t] = n;
 }
 
+// With cellmask, each run start also sets its feature's bit in its cell's
+// presence mask (W words per cell, cleared beforehand).
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_dir_fill(const K *skeys, uint32_t cellbits, uint32_t cells,
                                                      uint64_t dir_len, uint64_t ntiles, const uint32_t *tile_lo,
-                                                     uint32_t *dir) {
+                                                     uint32_t *dir, unsigned long long *cellmask, uint32_t W) {
     __shared__ uint32_t v[kDirTile];
     __shared__ uint32_t wmin[kBlock / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -539,7 +541,13 @@ __global__ __launch_bounds__(kBlock) void k_dir_fill(const K *skeys, uint32_t ce
         __syncthreads();
         for (uint32_t p = lo + tid; p < hi; p += kBlock) {
             const K k = skeys[p];
-            if (p == 0 || skeys[p - 1] != k) v[dir_index(k, cellbits, cells) - e0] = p;
+            if (p == 0 || skeys[p - 1] != k) {
+                v[dir_index(k, cellbits, cells) - e0] = p;
+                if (cellmask) {
+                    const uint32_t f = (uint32_t)(k >> cellbits), cell = (uint32_t)(k & (((K)1 << cellbits) - 1));
+                    atomicOr(&cellmask[(uint64_t)cell * W + (f >> 6)], 1ull << (f & 63));
+                }
+            }
         }
         __syncthreads();
         // suffix minimum: thread tid owns entries [kDirItems * tid, + kDirItems)
@@ -574,20 +582,6 @@ __global__ void k_runs(const uint32_t *run_flag, const uint64_t *run_idx, const 
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n || !run_flag[p]) return;
     run_cell[run_idx[p]] = scell[p];
-}
-
-// Presence mask of every cell (bit f of word f / 64 set when feature f
-// occurs; W words per cell), from the runs.
-__global__ void k_cellmask(const uint32_t *run_cell, const uint64_t *run_off, uint32_t F, const uint64_t *dH,
-                           uint32_t W, unsigned long long *cellmask) {
-    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= *dH) return;
-    uint32_t lo = 0, hi = F;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (run_off[mid] <= h) lo = mid; else hi = mid;
-    }
-    atomicOr(&cellmask[(uint64_t)run_cell[h] * W + (lo >> 6)], 1ull << (lo & 63));
 }
 
 // Quadrant masks: OR of the 4 cells of quadrant (qx, qy), qx < gx-1, qy < gy-1
@@ -3124,8 +3118,12 @@ static int bin_launch(colo_ctx *c, hipStream_t s) {
         hipLaunchKernelGGL((k_gather_sorted<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk,
                            c->d_sidx, n, c->cellbits, c->d_x, c->d_y, c->d_sxy, c->d_spos, c->d_scell);
     }
-    // directory and run starts: each tile's first element (in stmp), then
+    // directory, run starts and cell masks (W 64-bit words per cell, when
+    // the filter's masks fit): each tile's first element (in stmp), then
     // the tiles
+    const uint32_t W = mask_words(c->F);
+    const bool masks = W <= kMaxMaskWords;
+    if (masks && (rc = fill_u32((uint32_t *)c->d_cellmask, 2ull * c->cells * W, 0, s))) return rc;
     uint32_t *run_flag = vals2;  // free once the sorted indices are in d_sidx
     if (n) {
         const uint64_t ntiles = dir_tiles(c);
@@ -3133,7 +3131,8 @@ static int bin_launch(colo_ctx *c, hipStream_t s) {
         hipLaunchKernelGGL((k_dir_tiles<K>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sk, n,
                            c->cellbits, c->cells, ntiles, run_flag, tile_lo);
         hipLaunchKernelGGL((k_dir_fill<K>), dim3((unsigned)std::min<uint64_t>(ntiles, 1u << 20)), dim3(kBlock), 0, s,
-                           sk, c->cellbits, c->cells, c->dir_len, ntiles, (const uint32_t *)tile_lo, c->d_dir);
+                           sk, c->cellbits, c->cells, c->dir_len, ntiles, (const uint32_t *)tile_lo, c->d_dir,
+                           masks ? (unsigned long long *)c->d_cellmask : nullptr, W);
     } else if ((rc = fill_u32(c->d_dir, c->dir_len, 0, s))) {
         return rc;
     }
@@ -3147,14 +3146,9 @@ static int bin_launch(colo_ctx *c, hipStream_t s) {
     if (c->F && n)
         hipLaunchKernelGGL(k_bin_tail, dim3((c->F + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c->d_flag, run_idx,
                            c->d_fstart, c->d_fcount, c->F, c->d_spos, small);
-    // cell and quadrant masks (W 64-bit words each)
-    const uint32_t W = mask_words(c->F);
-    if (W <= kMaxMaskWords) {
+    // quadrant and 3x3 masks from the cell masks
+    if (masks) {
         uint64_t nq = (c->gx >= 2 && c->gy >= 2) ? (uint64_t)(c->gx - 1) * (c->gy - 1) : 0;
-        if ((rc = fill_u32((uint32_t *)c->d_cellmask, 2ull * c->cells * W, 0, s))) return rc;
-        if (n)
-            hipLaunchKernelGGL(k_cellmask, dim3(grid_for(n, 1u << 30)), dim3(kBlock), 0, s, c->d_run_cell, small + 2,
-                               c->F, small + 1, W, (unsigned long long *)c->d_cellmask);
         if (nq) hipLaunchKernelGGL(k_qmask, dim3(grid_for(nq * W, 1u << 30)), dim3(kBlock), 0, s, c->d_cellmask, c->gx,
                                    c->gy, W, c->d_qmask);
         hipLaunchKernelGGL(k_mask3, dim3(grid_for((uint64_t)c->cells * W, 1u << 30)), dim3(kBlock), 0, s,

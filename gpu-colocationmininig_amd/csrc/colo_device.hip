@@ -890,6 +890,10 @@ constexpr int kWriteUnroll = 8;  // the same for the level-k write, which runs f
 // (global rather than flat loads).
 typedef __attribute__((address_space(1))) const uint32_t gcu32;
 __device__ inline const gcu32 *as_global(const uint32_t *p) { return (const gcu32 *)p; }
+// table outputs taken from descriptors, stored with global (not flat)
+// instructions
+typedef __attribute__((address_space(1))) uint32_t gu32;
+__device__ inline gu32 *as_global_out(uint32_t *p) { return (gu32 *)p; }
 
 // List bounds [b0, b1) of anchor rank t from a u64 row-offsets array:
 // entries t and t + 1 in one 16-byte load (8-byte aligned).
@@ -1242,8 +1246,8 @@ __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32
 __device__ inline void join2_write_flat(const DevData &D, uint32_t a, uint32_t b, uint32_t t, uint32_t nrows,
                                         uint64_t off, uint32_t *table, uint32_t *nbr) {
     const uint32_t g = D.fstart[a] + t;
-    uint32_t *out = table + off * 2;
-    uint32_t *nb = nbr ? nbr + off : nullptr;
+    gu32 *out = as_global_out(table + off * 2);
+    gu32 *nb = nbr ? as_global_out(nbr + off) : nullptr;
     uint32_t w = 0;
     join2_stream<false>(
         D, b, g,
@@ -2604,7 +2608,7 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
         const CandDesc &cd = descs[j];
         const uint32_t nrows = slot_of(cd);
         const gcu32 *nbt = as_global(cd.nb_table);
-        uint32_t *out = cd.table + (off1 != ~0ull ? off1 : cd.offsets[i] - cd.off_base) * k;
+        gu32 *out = as_global_out(cd.table + (off1 != ~0ull ? off1 : cd.offsets[i] - cd.off_base) * k);
         uint32_t w = 0;
         auto test = [&](uint32_t q, float qx, float qy) {
             bool ok = true;

@@ -1183,13 +1183,15 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
 
 // Batched level-2 join of one anchor (sorted position p of feature a): its
 // (up to three) row ranges of feature b are walked as one stream, range 0,
-// then 1, then 2 -- the scan order -- kFlat entries per step, so a short
-// range does not cost a dependent round trip of its own.  f(q, hit) is
+// then 1, then 2 -- the scan order -- FLAT entries per step (kFlat; the
+// packed count takes 4: its lanes all have entries, and fewer loads in
+// flight per lane leave room for more waves), so a short range does not
+// cost a dependent round trip of its own.  f(q, hit) is
 // called for every entry in stream order while keep() holds.  With MASK,
 // an anchor whose cell's 3 x 3 mask lacks b has no entries and reads no
 // directory (the write pass visits only anchors with rows).
 constexpr int kFlat = 8;
-template <bool MASK, typename F, typename K>
+template <bool MASK, int FLAT = kFlat, typename F, typename K>
 __device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F &&f, K &&keep) {
     const uint32_t cell = D.scell[p];
     if (MASK && D.mask3 && !((D.mask3[(uint64_t)cell * D.mask_w + (b >> 6)] >> (b & 63)) & 1)) return;
@@ -1201,17 +1203,17 @@ __device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F 
     const uint32_t p1 = m > 0 ? rs.hi[0] - lo0 : 0;
     const uint32_t p2 = p1 + (m > 1 ? rs.hi[1] - lo1 : 0);
     const uint32_t tot = p2 + (m > 2 ? rs.hi[2] - lo2 : 0);
-    for (uint32_t s0 = 0; s0 < tot && keep(); s0 += kFlat) {
-        uint32_t q[kFlat];
-        float2 c[kFlat];
+    for (uint32_t s0 = 0; s0 < tot && keep(); s0 += FLAT) {
+        uint32_t q[FLAT];
+        float2 c[FLAT];
 #pragma unroll
-        for (int u = 0; u < kFlat; u++) {
+        for (int u = 0; u < FLAT; u++) {
             const uint32_t e = s0 + u;
             q[u] = e < p1 ? lo0 + e : e < p2 ? lo1 + (e - p1) : lo2 + (e - p2);
             if (e < tot) c[u] = D.sxy[q[u]];
         }
 #pragma unroll
-        for (int u = 0; u < kFlat; u++)
+        for (int u = 0; u < FLAT; u++)
             if (s0 + u < tot) f(q[u], sqdist(pa.x, pa.y, c[u].x, c[u].y) <= D.g.r2);
     }
 }
@@ -1220,12 +1222,12 @@ __device__ inline void join2_stream(const DevData &D, uint32_t b, uint32_t p, F 
 // with SORTED, entries walked by join2_stream; MASK: test the anchor cell's
 // 3 x 3 mask first).  acc collects the bits of feature b; bits_a null: the
 // anchor's own bit is left to the caller.
-template <bool MASK = true, typename Acc = BitAcc>
+template <bool MASK = true, int FLAT = kFlat, typename Acc = BitAcc>
 __device__ inline uint32_t join2_count_flat(const DevData &D, uint32_t a, uint32_t b, uint32_t t, uint32_t *slots,
                                             uint32_t *bits_a, Acc acc) {
     const uint32_t sb = D.fstart[b];
     uint32_t cnt = 0;
-    join2_stream<MASK>(
+    join2_stream<MASK, FLAT>(
         D, b, D.fstart[a] + t,
         [&](uint32_t q, bool hit) {
             if (hit) {
@@ -2449,7 +2451,7 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
                 },
                 [&](uint32_t t, bool valid) {
                     const uint32_t cnt =
-                        valid ? join2_count_flat<false>(D, a, b, t, sl, nullptr,
+                        valid ? join2_count_flat<false, 4>(D, a, b, t, sl, nullptr,
                                                         BitAccC(bt + cd.seg_off[1], bits, s_bc))
                               : 0u;
                     sum += cnt;

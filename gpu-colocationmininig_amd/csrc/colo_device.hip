@@ -1184,7 +1184,7 @@ __device__ inline void joink_count_row(const DevData &D, const uint32_t *row, ui
 // Batched level-2 join of one anchor (sorted position p of feature a): its
 // (up to three) row ranges of feature b are walked as one stream, range 0,
 // then 1, then 2 -- the scan order -- FLAT entries per step (kFlat; the
-// packed count takes 4: its lanes all have entries, and fewer loads in
+// packed count takes 2: its lanes all have entries, and fewer loads in
 // flight per lane leave room for more waves), so a short range does not
 // cost a dependent round trip of its own.  f(q, hit) is
 // called for every entry in stream order while keep() holds.  With MASK,
@@ -2451,7 +2451,7 @@ __global__ __launch_bounds__(kBlock) void k_count2_batch(DevData D, const CandDe
                 },
                 [&](uint32_t t, bool valid) {
                     const uint32_t cnt =
-                        valid ? join2_count_flat<false, 4>(D, a, b, t, sl, nullptr,
+                        valid ? join2_count_flat<false, 2>(D, a, b, t, sl, nullptr,
                                                         BitAccC(bt + cd.seg_off[1], bits, s_bc))
                               : 0u;
                     sum += cnt;

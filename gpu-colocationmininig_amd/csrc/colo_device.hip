@@ -894,6 +894,8 @@ __device__ inline const gcu32 *as_global(const uint32_t *p) { return (const gcu3
 // instructions
 typedef __attribute__((address_space(1))) uint32_t gu32;
 __device__ inline gu32 *as_global_out(uint32_t *p) { return (gu32 *)p; }
+typedef __attribute__((address_space(1))) const uint64_t gcu64;
+__device__ inline const gcu64 *as_global(const uint64_t *p) { return (const gcu64 *)p; }
 
 // List bounds [b0, b1) of anchor rank t from a u64 row-offsets array:
 // entries t and t + 1 in one 16-byte load (8-byte aligned).
@@ -2658,11 +2660,12 @@ __global__ __launch_bounds__(kBlock) void k_write2_batch(DevData D, const CandDe
     while (next_item(queue, seg, it)) {
         const WorkItem wi = items[it];
         const CandDesc &cd = desc[wi.cand];
-        const uint32_t *sl = cd.wslots ? cd.wslots : slots + cd.slots_off;
+        const gcu32 *sl = as_global(cd.wslots ? cd.wslots : slots + cd.slots_off);
+        const gcu64 *offs = as_global(cd.offsets);
         for (uint64_t t = wi.begin + threadIdx.x; t < wi.end; t += kBlock) {
             uint32_t nr = sl[t];
             if (nr)
-                join2_write_flat(D, cd.f[0], cd.f[1], (uint32_t)t, nr, cd.offsets[t] - cd.off_base, cd.table, cd.nbr_out);
+                join2_write_flat(D, cd.f[0], cd.f[1], (uint32_t)t, nr, offs[t] - cd.off_base, cd.table, cd.nbr_out);
         }
     }
 }

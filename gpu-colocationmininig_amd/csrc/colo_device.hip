@@ -2577,10 +2577,10 @@ __device__ inline void joink_write_row_multi(const DevData &D, const CandDesc *d
     const CandDesc &c0 = descs[0];
     const uint32_t km1 = KM1 ? (uint32_t)KM1 : c0.k - 1, k = km1 + 1;
     constexpr int kArr = KM1 ? KM1 : kMaxK;
-    const uint32_t *row = c0.parent + i * km1;
+    const gcu32 *row = as_global(c0.parent + i * km1);
     uint32_t ids[kArr];
     float mx[kArr], my[kArr];
-    auto slot_of = [&](const CandDesc &cd) { return cd.wslots ? cd.wslots[i] : slots[cd.slots_off + i]; };
+    auto slot_of = [&](const CandDesc &cd) { return as_global(cd.wslots ? cd.wslots : slots + cd.slots_off)[i]; };
     // candidates with rows from this parent row (a run holds at most 64)
     uint64_t todo = 0;
     for (uint32_t j = 0; j < ncand; j++)
@@ -2690,11 +2690,12 @@ __global__ __launch_bounds__(kBlock) void k_writek_batch(DevData D, const CandDe
             const uint32_t lane = threadIdx.x & 63;
             for (uint32_t j = 0; j < wi.ncand; j++) {
                 const CandDesc &cd = desc[wi.cand + j];
-                const uint32_t *sl = cd.wslots ? cd.wslots : slots + cd.slots_off;
+                const gcu32 *sl = as_global(cd.wslots ? cd.wslots : slots + cd.slots_off);
+                const gcu64 *offs = as_global(cd.offsets);
                 for (uint64_t c0 = wi.begin + (threadIdx.x & ~63u); c0 < wi.end; c0 += kBlock) {
                     const uint64_t i = c0 + lane;
                     const uint64_t nr = i < wi.end ? sl[i] : 0;
-                    const uint64_t off = cd.offsets[c0 >> 6] - cd.off_base + wave_incl_scan(nr) - nr;
+                    const uint64_t off = offs[c0 >> 6] - cd.off_base + wave_incl_scan(nr) - nr;
                     if (nr) joink_write_row_multi<KM1>(D, &cd, 1, i, slots, off);
                 }
             }
